@@ -1,0 +1,549 @@
+/*
+ * edx_oracle.c -- CPU restatement of the EDRIXS Fortran exact-diagonalization core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * edrixs_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  Nothing in the product path links it.
+ *
+ * It follows the reference algorithms step by step (same term order, same
+ * accumulation order, same retained zeros, 0-based indices instead of 1-based):
+ *
+ *   orc_make_newfock      src/fock.f90:38-68
+ *   orc_binary_search     src/utils.f90:2-40
+ *   orc_build_*           src/ham.f90:2-349 (build_ham_i), 352-711 (build_ham_n),
+ *                         714-912 (build_transop_i), 915-1115 (build_transop_f),
+ *                         with the sorted insert / duplicate sum of
+ *                         src/m_types.f90:59-121 (init_col, insert_into_row)
+ *   orc_matvec_csr        src/spmv.f90:73-103 (y += A x)
+ *   orc_build_krylov      src/lanczos.f90:210-303 (build_krylov_mp)
+ *   orc_lanczos           src/lanczos.f90:6-167 (diag_ham_lanczos), start vector
+ *                         supplied by the caller instead of random_number
+ *   orc_tridiag_eigs      stands in for LAPACK dsteqr('I') (src/lanczos.f90:169-206):
+ *                         implicit QL with Wilkinson shifts (Golub & Van Loan, 8.3)
+ *   orc_pminres           src/linsys.f90:3-128
+ *   orc_denmat            src/ed_driver.f90:219-248
+ *   orc_cf_spectrum       edrixs/plot_spectrum.py:52-59 (and src/lanczos.f90:306-336)
+ *
+ * CSR layout used everywhere: rowptr int64[m+1] (0-based), cols int32[nnz]
+ * (0-based global columns), vals double[2*nnz] (re, im interleaved).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double _Complex zdouble;
+
+typedef struct { int32_t ind1, ind2; double re, im; } orc_t2;
+typedef struct { int32_t ind1, ind2, ind3, ind4; double re, im; } orc_t4;
+
+/* ---------------------------------------------------------------- fock ops */
+
+/* src/fock.f90:38-68: sign = (-1)^(#occupied orbitals below pos), pos 1-based. */
+int orc_make_newfock(char c_type, int pos, int64_t old, int64_t *newf, int *sgn)
+{
+    int cnt = 0;
+    for (int i = 1; i <= pos - 1; ++i)
+        if ((old >> (i - 1)) & 1) cnt++;
+    *sgn = (cnt % 2) ? -1 : 1;
+    if (c_type == '+') *newf = old + ((int64_t)1 << (pos - 1));
+    else if (c_type == '-') *newf = old - ((int64_t)1 << (pos - 1));
+    else return -1;
+    return 0;
+}
+
+/* src/utils.f90:2-40, returns the 0-based index or -1. */
+int64_t orc_binary_search(int64_t n, const int64_t *vec, int64_t m)
+{
+    int64_t left = 1, right = n;
+    while (left <= right) {
+        int64_t mid = left + (right - left) / 2;
+        if (vec[mid - 1] < m) left = mid + 1;
+        else if (vec[mid - 1] > m) right = mid - 1;
+        else return mid - 1;
+    }
+    return -1;
+}
+
+/* --------------------------------------------------------- sorted row list */
+
+typedef struct { int32_t col; double re, im; } orc_entry;
+typedef struct { orc_entry *e; int64_t n, cap; } orc_row;
+
+/* src/m_types.f90:59-121: keep ascending columns; equal column -> val += new. */
+static void row_insert(orc_row *r, int32_t col, double re, double im)
+{
+    int64_t lo = 0, hi = r->n;
+    while (lo < hi) {               /* first position with e.col >= col */
+        int64_t mid = (lo + hi) / 2;
+        if (r->e[mid].col < col) lo = mid + 1; else hi = mid;
+    }
+    if (lo < r->n && r->e[lo].col == col) {
+        r->e[lo].re = r->e[lo].re + re;
+        r->e[lo].im = r->e[lo].im + im;
+        return;
+    }
+    if (r->n == r->cap) {
+        r->cap = r->cap ? 2 * r->cap : 64;
+        r->e = (orc_entry *)realloc(r->e, (size_t)r->cap * sizeof(orc_entry));
+    }
+    memmove(r->e + lo + 1, r->e + lo, (size_t)(r->n - lo) * sizeof(orc_entry));
+    r->e[lo].col = col; r->e[lo].re = re; r->e[lo].im = im;
+    r->n++;
+}
+
+/* ------------------------------------------------------------- CSR builder */
+
+enum { ROW_PLAIN = 0, ROW_COREHOLE = 1, ROW_FULLCORE = 2 };
+enum { COL_PLAIN = 0, COL_COREHOLE = 1, COL_VALONLY = 2 };
+
+typedef struct {
+    int64_t *rowptr; int32_t *cols; double *vals; int64_t m, nnz, cap;
+} orc_csr_acc;
+
+static void acc_push_row(orc_csr_acc *a, const orc_row *r)
+{
+    if (a->nnz + r->n > a->cap) {
+        int64_t nc = a->cap ? a->cap : 1024;
+        while (nc < a->nnz + r->n) nc *= 2;
+        a->cols = (int32_t *)realloc(a->cols, (size_t)nc * sizeof(int32_t));
+        a->vals = (double *)realloc(a->vals, (size_t)nc * 2 * sizeof(double));
+        a->cap = nc;
+    }
+    for (int64_t k = 0; k < r->n; ++k) {
+        a->cols[a->nnz + k] = r->e[k].col;
+        a->vals[2 * (a->nnz + k)] = r->e[k].re;
+        a->vals[2 * (a->nnz + k) + 1] = r->e[k].im;
+    }
+    a->nnz += r->n;
+}
+
+/*
+ * One generic row loop that covers the four reference builders:
+ *   build_ham_i     rowkind=ROW_PLAIN    colkind=COL_PLAIN     (src/ham.f90:59-176)
+ *   build_ham_n     rowkind=ROW_COREHOLE colkind=COL_COREHOLE  (src/ham.f90:419-539)
+ *   build_transop_i rowkind=ROW_COREHOLE colkind=COL_VALONLY   (src/ham.f90:780-821)
+ *   build_transop_f rowkind=ROW_FULLCORE colkind=COL_COREHOLE  (src/ham.f90:981-1022)
+ * Row r's source word, the term loop (hoppings, then Coulombs, then omega on the
+ * diagonal if ham_sign<0 and with_omega), and the column lookup follow those lines.
+ * Returns the CSR through malloc'd arrays that the caller frees with orc_free.
+ */
+int orc_build(int rowkind, int64_t nleft, const int64_t *fock_left,
+              int colkind, int64_t nright, const int64_t *fock_right,
+              int num_val_orbs, int num_core_orbs,
+              int64_t nhopp, const orc_t2 *hop, int64_t ncoul, const orc_t4 *coul,
+              double omega_re, double omega_im, double ham_sign, int with_omega,
+              int64_t row_begin, int64_t row_end,
+              int64_t *m_out, int64_t *nnz_out,
+              int64_t **rowptr_out, int32_t **cols_out, double **vals_out)
+{
+    const int nc = num_core_orbs, nv = num_val_orbs;
+    int64_t fock_core[64];
+    for (int i = 1; i <= nc; ++i)   /* src/ham.f90:419-421 */
+        fock_core[i - 1] = ((int64_t)1 << nc) - 1 - ((int64_t)1 << (nc - i));
+    const int64_t full_core = ((int64_t)1 << nc) - 1;
+    int64_t m_total = (rowkind == ROW_COREHOLE) ? nleft * nc : nleft;
+    if (row_end < 0 || row_end > m_total) row_end = m_total;
+    if (row_begin < 0) row_begin = 0;
+    int64_t m = row_end - row_begin;
+
+    orc_csr_acc acc = {0};
+    acc.rowptr = (int64_t *)malloc((size_t)(m + 1) * sizeof(int64_t));
+    acc.rowptr[0] = 0;
+    orc_row row = {0};
+
+    for (int64_t r = row_begin; r < row_end; ++r) {
+        row.n = 0;
+        int64_t base;
+        if (rowkind == ROW_PLAIN) base = fock_left[r];
+        else if (rowkind == ROW_COREHOLE) {
+            int64_t core_indx = r / nleft, val_indx = r % nleft;
+            base = fock_core[core_indx] * ((int64_t)1 << nv) + fock_left[val_indx];
+        } else base = full_core * ((int64_t)1 << nv) + fock_left[r];
+
+        for (int64_t t = 0; t < nhopp + ncoul; ++t) {
+            int ops[4]; char kinds[4]; int nop; double vre, vim;
+            if (t < nhopp) {
+                ops[0] = hop[t].ind1; kinds[0] = '-';
+                ops[1] = hop[t].ind2; kinds[1] = '+';
+                nop = 2; vre = hop[t].re; vim = hop[t].im;
+            } else {
+                const orc_t4 *c = &coul[t - nhopp];
+                ops[0] = c->ind1; kinds[0] = '-';
+                ops[1] = c->ind2; kinds[1] = '-';
+                ops[2] = c->ind3; kinds[2] = '+';
+                ops[3] = c->ind4; kinds[3] = '+';
+                nop = 4; vre = c->re; vim = c->im;
+            }
+            int64_t old = base, nw = 0; int tot_sign = 1, sgn, ok = 1;
+            for (int q = 0; q < nop; ++q) {
+                int occ = (old >> (ops[q] - 1)) & 1;
+                if ((kinds[q] == '-' && !occ) || (kinds[q] == '+' && occ)) { ok = 0; break; }
+                orc_make_newfock(kinds[q], ops[q], old, &nw, &sgn);
+                tot_sign *= sgn;
+                old = nw;
+            }
+            if (!ok) continue;
+            int64_t jcfg;
+            if (colkind == COL_PLAIN) {
+                jcfg = orc_binary_search(nright, fock_right, nw);
+            } else {
+                int64_t new_core = nw / ((int64_t)1 << nv);
+                int64_t new_val = nw - new_core * ((int64_t)1 << nv);
+                if (colkind == COL_VALONLY) {
+                    jcfg = orc_binary_search(nright, fock_right, new_val);
+                } else {
+                    int64_t k = orc_binary_search(nc, fock_core, new_core);
+                    int64_t l = orc_binary_search(nright, fock_right, new_val);
+                    jcfg = (k < 0 || l < 0) ? -1 : k * nright + l;
+                }
+            }
+            if (jcfg < 0) continue;
+            /* val * tot_sign * ham_sign, evaluated left to right on each part */
+            double re = vre * (double)tot_sign * ham_sign;
+            double im = vim * (double)tot_sign * ham_sign;
+            row_insert(&row, (int32_t)jcfg, re, im);
+        }
+        if (with_omega && ham_sign < 0) /* src/ham.f90:156-176 */
+            row_insert(&row, (int32_t)r, omega_re, omega_im);
+        acc_push_row(&acc, &row);
+        acc.rowptr[r - row_begin + 1] = acc.nnz;
+    }
+    free(row.e);
+    *m_out = m; *nnz_out = acc.nnz;
+    *rowptr_out = acc.rowptr; *cols_out = acc.cols; *vals_out = acc.vals;
+    return 0;
+}
+
+void orc_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------- SpMV */
+
+/* src/spmv.f90:91-100: tmp accumulated left to right, then y(i) = y(i) + tmp. */
+void orc_matvec_csr(int64_t m, const int64_t *rowptr, const int32_t *cols,
+                    const double *vals, const double *x, double *y)
+{
+    for (int64_t i = 0; i < m; ++i) {
+        int64_t b = rowptr[i], e = rowptr[i + 1];
+        if (b >= e) continue;
+        double tr = 0.0, ti = 0.0;
+        for (int64_t j = b; j < e; ++j) {
+            double ar = vals[2 * j], ai = vals[2 * j + 1];
+            double xr = x[2 * (int64_t)cols[j]], xi = x[2 * (int64_t)cols[j] + 1];
+            tr += ar * xr - ai * xi;
+            ti += ar * xi + ai * xr;
+        }
+        y[2 * i] += tr;
+        y[2 * i + 1] += ti;
+    }
+}
+
+/* Fortran dot_product on complex vectors: sum(conjg(a) * b). */
+static zdouble zdot(int64_t n, const zdouble *a, const zdouble *b)
+{
+    zdouble s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += conj(a[i]) * b[i];
+    return s;
+}
+
+/* ------------------------------------------------------ tridiagonal eigens */
+
+/*
+ * Eigenvalues (ascending) and optionally eigenvectors (column-major n*n, z[j*n+i]
+ * is component i of vector j) of the symmetric tridiagonal matrix with diagonal d
+ * and off-diagonal e (length n-1).  Implicit QL with Wilkinson shifts.
+ */
+int orc_tridiag_eigs(int n, const double *d_in, const double *e_in, double *w, double *z)
+{
+    double *d = (double *)malloc((size_t)n * sizeof(double));
+    double *e = (double *)calloc((size_t)n + 1, sizeof(double));
+    memcpy(d, d_in, (size_t)n * sizeof(double));
+    for (int i = 0; i < n - 1; ++i) e[i] = e_in[i];
+    if (z) {
+        memset(z, 0, (size_t)n * n * sizeof(double));
+        for (int i = 0; i < n; ++i) z[i * n + i] = 1.0;
+    }
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < n - 1; ++mm) {
+                double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (fabs(e[mm]) <= 1e-300 || fabs(e[mm]) <= 2.220446049250313e-16 * dd) break;
+            }
+            if (mm != l) {
+                if (++iter > 60 * n) { free(d); free(e); return -1; }
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = mm - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    r = hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) { d[i + 1] -= p; e[mm] = 0.0; break; }
+                    s = f / r; c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    if (z) {
+                        for (int k = 0; k < n; ++k) {
+                            double zk1 = z[(i + 1) * n + k], zk = z[i * n + k];
+                            z[(i + 1) * n + k] = s * zk + c * zk1;
+                            z[i * n + k] = c * zk - s * zk1;
+                        }
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p; e[l] = g; e[mm] = 0.0;
+            }
+        } while (mm != l);
+    }
+    /* selection sort into ascending order (carrying vectors) */
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i; double p = d[i];
+        for (int j = i + 1; j < n; ++j) if (d[j] < p) { k = j; p = d[j]; }
+        if (k != i) {
+            d[k] = d[i]; d[i] = p;
+            if (z) for (int q = 0; q < n; ++q) {
+                double t = z[i * n + q]; z[i * n + q] = z[k * n + q]; z[k * n + q] = t;
+            }
+        }
+    }
+    memcpy(w, d, (size_t)n * sizeof(double));
+    free(d); free(e);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ Krylov */
+
+/*
+ * src/lanczos.f90:210-303.  alpha[0..maxn-1] = alpha_1..alpha_maxn,
+ * beta[0..maxn] = beta_0..beta_maxn (beta_0 = 0).
+ */
+int orc_build_krylov(int64_t n, const int64_t *rowptr, const int32_t *cols,
+                     const double *vals, const double *init_vec, int maxn,
+                     int *neff, double *alpha, double *beta, double *norm)
+{
+    zdouble *w = (zdouble *)calloc((size_t)2 * n, sizeof(zdouble));
+    zdouble *v[2] = { w, w + n };
+    memset(alpha, 0, (size_t)maxn * sizeof(double));
+    memset(beta, 0, (size_t)(maxn + 1) * sizeof(double));
+    *neff = 0;
+    memcpy(v[0], init_vec, (size_t)n * sizeof(zdouble));
+    zdouble t = zdot(n, v[0], v[0]);
+    *norm = creal(t);
+    double s = sqrt(creal(t));
+    for (int64_t k = 0; k < n; ++k) v[0][k] = v[0][k] / s;
+    int curr = 0, prev = 1;
+    for (int i = 1; i <= maxn; ++i) {
+        *neff = i;
+        for (int64_t k = 0; k < n; ++k) v[prev][k] = -beta[i - 1] * v[prev][k];
+        orc_matvec_csr(n, rowptr, cols, vals, (const double *)v[curr], (double *)v[prev]);
+        t = zdot(n, v[curr], v[prev]);
+        alpha[i - 1] = creal(t);
+        for (int64_t k = 0; k < n; ++k) v[prev][k] = v[prev][k] - alpha[i - 1] * v[curr][k];
+        t = zdot(n, v[prev], v[prev]);
+        beta[i] = sqrt(creal(t));
+        if (fabs(beta[i]) < 1e-10) break;
+        for (int64_t k = 0; k < n; ++k) v[prev][k] = v[prev][k] / beta[i];
+        int tmp = curr; curr = prev; prev = tmp;
+    }
+    free(w);
+    return 0;
+}
+
+/*
+ * src/lanczos.f90:6-167 with the random start replaced by start_vec (it is
+ * normalised here as at lines 63-71).  eval[nev], evec[n*n_vec] (complex,
+ * vector j at evec + 2*n*j).  Returns the number of Krylov steps in *nkrylov.
+ */
+int orc_lanczos(int64_t n, const int64_t *rowptr, const int32_t *cols, const double *vals,
+                const double *start_vec, int nev, int max_niter, double eval_tol,
+                int n_vec, double *eval, double *evec, int *nkrylov_out)
+{
+    zdouble *w = (zdouble *)calloc((size_t)2 * n, sizeof(zdouble));
+    zdouble *v[2] = { w, w + n };
+    zdouble *ev = (zdouble *)evec;
+    double *alpha = (double *)calloc((size_t)max_niter + 1, sizeof(double));
+    double *beta = (double *)calloc((size_t)max_niter + 1, sizeof(double));
+    double *kval = (double *)calloc((size_t)max_niter + 1, sizeof(double));
+    double *kvec = (double *)calloc((size_t)max_niter * max_niter, sizeof(double));
+    memset(eval, 0, (size_t)nev * sizeof(double));
+    memcpy(v[0], start_vec, (size_t)n * sizeof(zdouble));
+    zdouble t = zdot(n, v[0], v[0]);
+    double s = sqrt(creal(t));
+    for (int64_t k = 0; k < n; ++k) v[0][k] /= s;
+    for (int64_t k = 0; k < n; ++k) ev[k] = v[0][k];
+    int curr = 0, prev = 1, nkrylov = 0, kdim = 0;
+    for (int i = 1; i <= max_niter; ++i) {
+        for (int64_t k = 0; k < n; ++k) v[prev][k] = -beta[i - 1] * v[prev][k];
+        orc_matvec_csr(n, rowptr, cols, vals, (const double *)v[curr], (double *)v[prev]);
+        t = zdot(n, v[curr], v[prev]);
+        alpha[i - 1] = creal(t);
+        if (i % 10 == 0 || i == max_niter) {
+            if (orc_tridiag_eigs(i, alpha, beta + 1, kval, kvec)) return -2;
+            kdim = i;
+            if (i >= nev) {
+                double res = kval[nev - 1] - eval[nev - 1];
+                if (fabs(res) < eval_tol) {
+                    memcpy(eval, kval, (size_t)nev * sizeof(double));
+                    nkrylov = i;
+                    break;
+                }
+            }
+            for (int k = 0; k < nev; ++k) eval[k] = k < i ? kval[k] : 0.0;
+        }
+        for (int64_t k = 0; k < n; ++k) v[prev][k] -= alpha[i - 1] * v[curr][k];
+        t = zdot(n, v[prev], v[prev]);
+        beta[i] = sqrt(creal(t));
+        if (fabs(beta[i]) < 1e-10) {
+            if (i >= 2) {
+                if (orc_tridiag_eigs(i, alpha, beta + 1, kval, kvec)) return -2;
+                memcpy(eval, kval, (size_t)(nev < i ? nev : i) * sizeof(double));
+            } else {
+                eval[0] = alpha[0];
+                kvec[0] = 1.0;
+            }
+            kdim = i;
+            nkrylov = i;
+            break;
+        }
+        for (int64_t k = 0; k < n; ++k) v[prev][k] /= beta[i];
+        int tmp = curr; curr = prev; prev = tmp;
+    }
+    /* eigenvectors by re-running the recurrence, src/lanczos.f90:137-157 */
+    memset(w, 0, (size_t)2 * n * sizeof(zdouble));
+    for (int64_t k = 0; k < n; ++k) v[0][k] = ev[k];
+    memset(ev, 0, (size_t)n * n_vec * sizeof(zdouble));
+    curr = 0; prev = 1;
+    for (int i = 1; i <= nkrylov; ++i) {
+        for (int j = 0; j < n_vec; ++j) {
+            double c = kvec[j * kdim + (i - 1)];
+            for (int64_t k = 0; k < n; ++k) ev[j * n + k] += c * v[curr][k];
+        }
+        if (i < nkrylov) {
+            for (int64_t k = 0; k < n; ++k)
+                v[prev][k] = -alpha[i - 1] * v[curr][k] - beta[i - 1] * v[prev][k];
+            orc_matvec_csr(n, rowptr, cols, vals, (const double *)v[curr], (double *)v[prev]);
+            for (int64_t k = 0; k < n; ++k) v[prev][k] /= beta[i];
+            int tmp = curr; curr = prev; prev = tmp;
+        }
+    }
+    *nkrylov_out = nkrylov;
+    free(w); free(alpha); free(beta); free(kval); free(kvec);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ MINRES */
+
+/* src/linsys.f90:3-128 (complex-symmetric Givens rotation, absolute stop). */
+int orc_pminres(int64_t n, const int64_t *rowptr, const int32_t *cols, const double *vals,
+                const double *b_in, double *x_out, double linsys_tol, int linsys_max,
+                int *iters, double *final_res)
+{
+    zdouble *buf = (zdouble *)calloc((size_t)5 * n, sizeof(zdouble));
+    zdouble *v[2] = { buf, buf + n }, *wv[2] = { buf + 2 * n, buf + 3 * n };
+    zdouble *tmp = buf + 4 * n, *x = (zdouble *)x_out;
+    const zdouble *b = (const zdouble *)b_in;
+    zdouble sigma0 = 0, sigma1 = 0, gamma0 = 1, gamma1 = 1, eta, delta, res;
+    zdouble alpha, rho1, rho2, rho3;
+    double beta0, beta1;
+    int curr = 0, prev = 1;
+    memset(x, 0, (size_t)n * sizeof(zdouble));
+    memcpy(v[curr], b, (size_t)n * sizeof(zdouble));
+    beta0 = creal(zdot(n, v[curr], v[curr]));
+    beta0 = sqrt(beta0);
+    eta = beta0; res = beta0;
+    int iter = 0;
+    while (cabs(res) > linsys_tol && iter < linsys_max) {
+        iter++;
+        for (int64_t k = 0; k < n; ++k) v[curr][k] = v[curr][k] / beta0;
+        memset(tmp, 0, (size_t)n * sizeof(zdouble));
+        orc_matvec_csr(n, rowptr, cols, vals, (const double *)v[curr], (double *)tmp);
+        alpha = zdot(n, v[curr], tmp);
+        for (int64_t k = 0; k < n; ++k)
+            v[prev][k] = tmp[k] - alpha * v[curr][k] - beta0 * v[prev][k];
+        beta1 = creal(zdot(n, v[prev], v[prev]));
+        beta1 = sqrt(beta1);
+        delta = gamma0 * alpha - gamma1 * sigma0 * beta0;
+        rho1 = csqrt(delta * delta + beta1 * beta1);
+        rho2 = sigma0 * alpha + gamma1 * gamma0 * beta0;
+        rho3 = sigma1 * beta0;
+        gamma1 = gamma0; sigma1 = sigma0;
+        gamma0 = delta / rho1; sigma0 = beta1 / rho1;
+        for (int64_t k = 0; k < n; ++k)
+            wv[prev][k] = (v[curr][k] - rho3 * wv[prev][k] - rho2 * wv[curr][k]) / rho1;
+        for (int64_t k = 0; k < n; ++k) x[k] = x[k] + gamma0 * eta * wv[prev][k];
+        res = cabs(sigma0) * res;
+        eta = -sigma0 * eta;
+        beta0 = beta1;
+        int t = curr; curr = prev; prev = t;
+    }
+    memset(tmp, 0, (size_t)n * sizeof(zdouble));
+    orc_matvec_csr(n, rowptr, cols, vals, (const double *)x, (double *)tmp);
+    for (int64_t k = 0; k < n; ++k) tmp[k] -= b[k];
+    *final_res = sqrt(cabs(zdot(n, tmp, tmp)));
+    *iters = iter;
+    free(buf);
+    return 0;
+}
+
+/* ---------------------------------------------------------- density matrix */
+
+/*
+ * src/ed_driver.f90:219-248 for one vector.  denmat is num_val_orbs^2 complex,
+ * row-major denmat[(i*norb + j)] = denmat(i+1, j+1) of the reference.
+ */
+int orc_denmat(int64_t ndim, const int64_t *fock, int num_val_orbs, const double *vec_in,
+               double *denmat_out)
+{
+    const zdouble *v = (const zdouble *)vec_in;
+    zdouble *dm = (zdouble *)denmat_out;
+    int no = num_val_orbs;
+    memset(dm, 0, (size_t)no * no * sizeof(zdouble));
+    for (int64_t icfg = 0; icfg < ndim; ++icfg) {
+        for (int j = 1; j <= no; ++j)
+            if ((fock[icfg] >> (j - 1)) & 1)
+                dm[(j - 1) * no + (j - 1)] += conj(v[icfg]) * v[icfg];
+        if (cabs(v[icfg]) < 1e-10) continue;
+        for (int i = 1; i <= no - 1; ++i) {
+            for (int j = i + 1; j <= no; ++j) {
+                if (!((fock[icfg] >> (i - 1)) & 1)) continue;
+                int64_t old = fock[icfg], nw; int sgn, tot = 1;
+                orc_make_newfock('-', i, old, &nw, &sgn); tot *= sgn; old = nw;
+                if ((old >> (j - 1)) & 1) continue;
+                orc_make_newfock('+', j, old, &nw, &sgn); tot *= sgn;
+                int64_t jcfg = orc_binary_search(ndim, fock, nw);
+                if (jcfg < 0) continue;
+                dm[(i - 1) * no + (j - 1)] += conj(v[icfg]) * v[jcfg] * (double)tot;
+                dm[(j - 1) * no + (i - 1)] += conj(v[jcfg]) * v[icfg] * (double)tot;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------ continued fraction */
+
+/*
+ * edrixs/plot_spectrum.py:52-59 for one ground state (weight not applied):
+ * spec[w] = -1/pi * Im[ 1/(z - a0 - b0^2/(z - a1 - ...)) ] * norm,
+ * z = omega[w] + i*gamma[w] + e_gs.
+ */
+void orc_cf_spectrum(int neff, const double *alpha, const double *beta, double norm,
+                     double e_gs, int nw, const double *omega, const double *gamma,
+                     double *spec)
+{
+    const double pi = 3.14159265358979323846;
+    for (int w = 0; w < nw; ++w) {
+        zdouble z = omega[w] + I * gamma[w] + e_gs;
+        zdouble tv = 0.0;
+        for (int j = neff - 1; j >= 1; --j) tv = beta[j - 1] * beta[j - 1] / (z - alpha[j] - tv);
+        tv = 1.0 / (z - alpha[0] - tv);
+        spec[w] = -1.0 / pi * cimag(tv) * norm;
+    }
+}
