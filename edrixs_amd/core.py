@@ -1,0 +1,304 @@
+"""In-memory interface to the device ED core (libedx through ctypes).
+
+Objects hold device-resident data: :class:`Basis` (sorted Fock words) and
+:class:`CSRMatrix` (row pointers int64, columns int32, complex128 values plus an
+optional complex64 copy).  Vectors are torch tensors on the HIP device
+(complex128); torch only provides the device memory.
+
+The functions mirror the Fortran routines they replace:
+
+==========================  =====================================================
+``build_ham_i``             ``build_ham_i``      src/ham.f90:2-349
+``build_ham_n``             ``build_ham_n``      src/ham.f90:352-711
+``build_transop_i``         ``build_transop_i``  src/ham.f90:714-912
+``build_transop_f``         ``build_transop_f``  src/ham.f90:915-1115
+``CSRMatrix.spmv``          ``matvec_csr``       src/spmv.f90:73-103 (y += A x)
+``lanczos_eigs``            ``diag_ham_lanczos`` src/lanczos.f90:6-167
+``build_krylov``            ``build_krylov_mp``  src/lanczos.f90:210-303
+``minres``                  ``pminres_csr``      src/linsys.f90:3-128
+``denmat``                  density matrix       src/ed_driver.f90:219-248
+``cf_spectrum``             ``get_spectra_from_poles`` edrixs/plot_spectrum.py:9-61
+==========================  =====================================================
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import EDX_C128, EDX_C64
+
+T2 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("re", "<f8"), ("im", "<f8")])
+T4 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("ind3", "<i4"), ("ind4", "<i4"),
+               ("re", "<f8"), ("im", "<f8")])
+
+PRECISIONS = {"c128": EDX_C128, "fp64": EDX_C128, "complex128": EDX_C128,
+              "c64": EDX_C64, "mixed": EDX_C64, "complex64": EDX_C64,
+              EDX_C128: EDX_C128, EDX_C64: EDX_C64}
+
+
+def _prec(p):
+    try:
+        return PRECISIONS[p]
+    except KeyError:
+        raise ValueError("unknown precision %r" % (p,))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def torch():
+    import torch as _t
+    return _t
+
+
+def device():
+    return torch().device("cuda", 0)
+
+
+def as_terms2(t):
+    if t is None or len(t) == 0:
+        return np.zeros(0, dtype=T2)
+    t = np.asarray(t)
+    if t.dtype.names:
+        out = np.zeros(len(t), dtype=T2)
+        for f in T2.names:
+            out[f] = t[f]
+        return out
+    a = np.asarray(t, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros(len(a), dtype=T2)
+    out["ind1"], out["ind2"], out["re"], out["im"] = a[:, 0], a[:, 1], a[:, 2], a[:, 3]
+    return out
+
+
+def as_terms4(t):
+    if t is None or len(t) == 0:
+        return np.zeros(0, dtype=T4)
+    t = np.asarray(t)
+    if t.dtype.names:
+        out = np.zeros(len(t), dtype=T4)
+        for f in T4.names:
+            out[f] = t[f]
+        return out
+    a = np.asarray(t, dtype=np.float64).reshape(-1, 6)
+    out = np.zeros(len(a), dtype=T4)
+    for k, f in enumerate(("ind1", "ind2", "ind3", "ind4")):
+        out[f] = a[:, k]
+    out["re"], out["im"] = a[:, 4], a[:, 5]
+    return out
+
+
+class Basis:
+    """Sorted Fock words resident on the device."""
+
+    def __init__(self, handle):
+        self._h = handle
+        n = ctypes.c_int64()
+        c = ctypes.c_int()
+        _lib.call("edx_basis_size", handle, ctypes.byref(n), ctypes.byref(c))
+        self.n = n.value
+        self.complete = bool(c.value)
+
+    @classmethod
+    def combinations(cls, nbits, nocc):
+        h = ctypes.c_void_p()
+        _lib.call("edx_basis_combinations", nbits, nocc, ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_words(cls, words, nbits):
+        w = np.ascontiguousarray(words, dtype=np.int64)
+        h = ctypes.c_void_p()
+        _lib.call("edx_basis_from_words", _ptr(w), len(w), nbits, ctypes.byref(h))
+        return cls(h)
+
+    def words(self):
+        out = np.zeros(self.n, dtype=np.int64)
+        _lib.call("edx_basis_download", self._h, _ptr(out))
+        return out
+
+    def lookup(self, words):
+        w = np.ascontiguousarray(words, dtype=np.int64)
+        out = np.zeros(len(w), dtype=np.int64)
+        _lib.call("edx_basis_lookup", self._h, _ptr(w), len(w), _ptr(out))
+        return out
+
+    def __len__(self):
+        return self.n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.edx_basis_free(h)
+            self._h = None
+
+
+class CSRMatrix:
+    """CSR matrix resident on the device (0-based, int64 row pointers)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        m, n, nnz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        p = ctypes.c_int()
+        _lib.call("edx_csr_info", handle, ctypes.byref(m), ctypes.byref(n), ctypes.byref(nnz),
+                  ctypes.byref(p))
+        self.m, self.n, self.nnz = m.value, n.value, nnz.value
+        self.has_c64 = p.value == EDX_C64
+
+    @property
+    def shape(self):
+        return (self.m, self.n)
+
+    def download(self):
+        rowptr = np.zeros(self.m + 1, dtype=np.int64)
+        cols = np.zeros(self.nnz, dtype=np.int32)
+        vals = np.zeros(self.nnz, dtype=np.complex128)
+        _lib.call("edx_csr_download", self._h, _ptr(rowptr), _ptr(cols), _ptr(vals))
+        return rowptr, cols, vals
+
+    def values(self, precision="c128"):
+        p = _prec(precision)
+        out = np.zeros(self.nnz, dtype=np.complex128 if p == EDX_C128 else np.complex64)
+        _lib.call("edx_csr_download_values", self._h, p, _ptr(out))
+        return out
+
+    def add_c64(self):
+        _lib.call("edx_csr_add_c64", self._h)
+        self.has_c64 = True
+        return self
+
+    def spmv(self, x, y, precision="c128"):
+        """y += A x for complex128 device tensors (src/spmv.f90:73-103)."""
+        _check_vec(x, self.n)
+        _check_vec(y, self.m)
+        _lib.call("edx_spmv", self._h, _prec(precision), _dptr(x), _dptr(y))
+        return y
+
+    def matvec(self, x, precision="c128"):
+        y = torch().zeros(self.m, dtype=torch().complex128, device=x.device)
+        return self.spmv(x, y, precision)
+
+    def spmm(self, X, Y, precision="c128"):
+        """Y[:, k] += A X[:, k]; X, Y column-major complex128 given as (nrhs, n) tensors."""
+        assert X.is_contiguous() and Y.is_contiguous()
+        assert X.shape[1] == self.n and Y.shape[1] == self.m and X.shape[0] == Y.shape[0]
+        _lib.call("edx_spmm", self._h, _prec(precision), X.shape[0], _dptr(X), _dptr(Y))
+        return Y
+
+    def todense(self):
+        rp, c, v = self.download()
+        d = np.zeros((self.m, self.n), np.complex128)
+        for r in range(self.m):
+            d[r, c[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
+        return d
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.edx_csr_free(h)
+            self._h = None
+
+
+def _check_vec(t, n):
+    T = torch()
+    if t.dtype != T.complex128 or not t.is_cuda or not t.is_contiguous() or t.numel() != n:
+        raise ValueError("expected a contiguous complex128 device tensor of length %d" % n)
+
+
+def _handle_out(fn, *args):
+    h = ctypes.c_void_p()
+    _lib.call(fn, *args, ctypes.byref(h))
+    return CSRMatrix(h)
+
+
+def build_ham_i(basis, hop, coul, omega=0j, ham_sign=1.0, precision="c128"):
+    h, c = as_terms2(hop), as_terms4(coul)
+    return _handle_out("edx_build_ham_i", basis._h, _ptr(h), len(h), _ptr(c), len(c),
+                       float(np.real(omega)), float(np.imag(omega)), float(ham_sign),
+                       _prec(precision))
+
+
+def build_ham_n(basis_n, num_val_orbs, num_core_orbs, hop, coul, omega=0j, ham_sign=1.0,
+                precision="c128"):
+    h, c = as_terms2(hop), as_terms4(coul)
+    return _handle_out("edx_build_ham_n", basis_n._h, num_val_orbs, num_core_orbs, _ptr(h),
+                       len(h), _ptr(c), len(c), float(np.real(omega)), float(np.imag(omega)),
+                       float(ham_sign), _prec(precision))
+
+
+def build_transop_i(basis_n, basis_i, num_val_orbs, num_core_orbs, trans, precision="c128"):
+    t = as_terms2(trans)
+    return _handle_out("edx_build_transop_i", basis_n._h, basis_i._h, num_val_orbs,
+                       num_core_orbs, _ptr(t), len(t), _prec(precision))
+
+
+def build_transop_f(basis_f, basis_n, num_val_orbs, num_core_orbs, trans, precision="c128"):
+    t = as_terms2(trans)
+    return _handle_out("edx_build_transop_f", basis_f._h, basis_n._h, num_val_orbs,
+                       num_core_orbs, _ptr(t), len(t), _prec(precision))
+
+
+def lanczos_eigs(H, nev, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None,
+                 precision="c128"):
+    """Lowest nev eigenvalues (numpy) and nvec eigenvectors (device tensor, nvec x n)."""
+    T = torch()
+    nvec = nev if nvec is None else nvec
+    ev = np.zeros(nev)
+    vecs = T.zeros((max(nvec, 1), H.n), dtype=T.complex128, device=device())
+    it = ctypes.c_int()
+    _lib.call("edx_lanczos_eigs", H._h, _prec(precision), nev, maxiter, float(tol),
+              int(seed), int(bool(deflate)), nvec, _ptr(ev), _dptr(vecs), ctypes.byref(it))
+    return ev, vecs[:nvec], it.value
+
+
+def build_krylov(H, v0, maxn, precision="c128"):
+    """alpha_1..neff, beta_1..neff (pole-file convention) and norm = <v0|v0>."""
+    _check_vec(v0, H.n)
+    alpha = np.zeros(maxn)
+    beta = np.zeros(maxn)
+    neff = ctypes.c_int()
+    norm = ctypes.c_double()
+    _lib.call("edx_build_krylov", H._h, _prec(precision), _dptr(v0), maxn, ctypes.byref(neff),
+              _ptr(alpha), _ptr(beta), ctypes.byref(norm))
+    n = neff.value
+    return alpha[:n].copy(), beta[:n].copy(), norm.value
+
+
+def minres(A, b, tol=1e-10, maxit=500, precision="c128"):
+    T = torch()
+    _check_vec(b, A.n)
+    x = T.zeros_like(b)
+    it = ctypes.c_int()
+    res = ctypes.c_double()
+    _lib.call("edx_minres", A._h, _prec(precision), _dptr(b), _dptr(x), float(tol), int(maxit),
+              ctypes.byref(it), ctypes.byref(res))
+    return x, it.value, res.value
+
+
+def denmat(basis, num_val_orbs, vec):
+    _check_vec(vec, basis.n)
+    dm = np.zeros((num_val_orbs, num_val_orbs), dtype=np.complex128)
+    _lib.call("edx_denmat", basis._h, num_val_orbs, _dptr(vec), _ptr(dm))
+    return dm
+
+
+def cf_spectrum(alpha, beta, norm, e_gs, omega, gamma, weight=1.0, out=None):
+    """Continued fraction of one pole set on the device, added into ``out``."""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    om = np.ascontiguousarray(omega, dtype=np.float64)
+    ga = np.ascontiguousarray(np.broadcast_to(gamma, om.shape), dtype=np.float64)
+    spec = np.zeros(len(om)) if out is None else out
+    _lib.call("edx_cf_spectrum", len(a), _ptr(a), _ptr(b), float(norm), float(e_gs),
+              float(weight), len(om), _ptr(om), _ptr(ga), _ptr(spec))
+    return spec
+
+
+def mem_in_use():
+    v = ctypes.c_int64()
+    _lib.call("edx_mem_in_use", ctypes.byref(v))
+    return v.value

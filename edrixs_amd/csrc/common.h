@@ -1,0 +1,87 @@
+// Internal runtime pieces shared by the libedx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/edx.h"
+
+namespace edx {
+
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// Device allocation with accounting; returns nullptr (error set) on failure.
+void *dmalloc(size_t bytes, const char *what);
+void dfree(void *p);
+int64_t mem_in_use();
+
+// The stream all library work is queued on (the legacy default stream, so it
+// is ordered with torch's default stream and with synchronous hipMemcpy).
+inline hipStream_t stream() { return 0; }
+int current_device();
+int num_cus();
+
+}  // namespace edx
+
+#define EDX_HIP(call)                                                            \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) return ::edx::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define EDX_CHECK(cond, code, ...)                                               \
+    do {                                                                         \
+        if (!(cond)) return ::edx::set_error((code), __VA_ARGS__);               \
+    } while (0)
+
+#define EDX_TRY(call)                                                            \
+    do {                                                                         \
+        int r_ = (call);                                                         \
+        if (r_) return r_;                                                       \
+    } while (0)
+
+#define EDX_LAUNCH_CHECK() EDX_HIP(hipGetLastError())
+
+// Sorted Fock words resident on the device.
+struct edx_basis {
+    int64_t n;          // number of words
+    int nbits;          // valence orbital count the words live in
+    int nocc;           // popcount shared by every word (when complete)
+    int complete;       // 1: words == all nbits-bit words with popcount nocc, ascending
+    int64_t *d_words;   // device, ascending
+};
+
+// CSR matrix resident on the device.  Rows are 0-based, columns are 0-based
+// global indices.  d_vals is always present (complex128, the exact sums of the
+// reference builder); d_vals32 is an optional complex64 rounding of it.
+struct edx_csr {
+    int64_t m, n, nnz;
+    int64_t *d_rowptr;
+    int32_t *d_cols;
+    double2 *d_vals;
+    float2 *d_vals32;
+    int spmv_lanes;      // lanes per row chosen for SpMV (0 = not tuned)
+    int spmv_lanes32;
+};
+
+// Lookup descriptor for a column space basis.
+struct BasisView {
+    const int64_t *words;
+    int64_t n;
+    int nbits;
+    int nocc;
+    int complete;
+};
+
+inline BasisView view_of(const edx_basis *b) {
+    return BasisView{b->d_words, b->n, b->nbits, b->nocc, b->complete};
+}
+
+namespace edx {
+// basis.hip
+int ensure_binomials();
+}  // namespace edx

@@ -1,0 +1,16 @@
+// Single HIP translation unit for all device code of libedx, so the
+// constant-memory binomial table and the shared kernels need no relocatable
+// device code.
+#include <algorithm>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "device.h"
+
+#include "basis.hip"
+#include "csr_build.hip"
+#include "spmv.hip"
+#include "blas.hip"
+#include "krylov.hip"
+#include "observables.hip"

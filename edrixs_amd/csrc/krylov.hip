@@ -1,0 +1,407 @@
+// Host drivers of the Krylov solvers (part of edx_kernels.hip).
+//
+//   edx_build_krylov  <- build_krylov_mp, src/lanczos.f90:210-303
+//   edx_minres        <- pminres_csr,     src/linsys.f90:3-128
+//   edx_lanczos_eigs  <- diag_ham_lanczos, src/lanczos.f90:6-167, with full
+//                        reorthogonalization (classical Gram-Schmidt, two passes)
+//                        instead of the eigenvector re-run, and deflated
+//                        restarts so degenerate levels are all found.
+//
+// The Krylov and MINRES recurrences keep their scalars in device memory: one
+// iteration is a fixed sequence of launches (SpMV with fused epilogue and dot,
+// one-block scalar kernels, fused vector updates) gated by a per-iteration run
+// flag, and the host only synchronises every few iterations to see whether the
+// recurrence has stopped.
+
+namespace edx {
+int tridiag_ql(int n, const double *d, const double *e, double *w, double *z, double *zlast);
+}
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { edx::dfree(p); }
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+    int alloc(size_t bytes, const char *what) {
+        p = edx::dmalloc(bytes, what);
+        return p ? EDX_OK : EDX_ERR_NOMEM;
+    }
+};
+
+int check_square(const edx_csr *h, int precision) {
+    EDX_CHECK(h, EDX_ERR_ARG, "null matrix");
+    EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix is %lld x %lld, not square", (long long)h->m,
+              (long long)h->n);
+    EDX_CHECK(precision == EDX_C128 || precision == EDX_C64, EDX_ERR_ARG, "bad precision");
+    if (precision == EDX_C64 && !h->d_vals32) EDX_TRY(edx_csr_add_c64(const_cast<edx_csr *>(h)));
+    return EDX_OK;
+}
+
+// y = c0*H x + c1*z with optional dot partials; converts x for the C64 path.
+int apply(const edx_csr *h, int precision, const double2 *x, float2 *x32, const SpmvEpi &e,
+          int *nb, const int *run) {
+    if (precision == EDX_C64) {
+        EDX_TRY(edx::to_c64(h->n, x, x32, run));
+        return edx::spmv_launch(h, precision, x32, e, 0, nb);
+    }
+    return edx::spmv_launch(h, precision, x, e, 0, nb);
+}
+
+int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int *neff,
+               double *alpha, double *beta, double *norm) {
+    const int64_t n = h->n;
+    DevBuf vb, x32, part, st, ab, runf;
+    EDX_TRY(vb.alloc(2 * n * sizeof(double2), "krylov vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "krylov x32"));
+    int nbs = edx::spmv_blocks(h, 64);
+    int nbv = vec_blocks(n);
+    EDX_TRY(part.alloc((size_t)std::max(nbs, nbv) * 4 * sizeof(double2) + 64, "partials"));
+    EDX_TRY(st.alloc(sizeof(KrylovDev), "krylov state"));
+    EDX_TRY(ab.alloc((2 * (size_t)maxn + 2) * sizeof(double), "alpha beta"));
+    EDX_TRY(runf.alloc(((size_t)maxn + 2) * sizeof(int), "run flags"));
+    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + n};
+    double *d_alpha = ab.as<double>(), *d_beta = ab.as<double>() + maxn + 1;
+    int *run = runf.as<int>();
+    KrylovDev *S = st.as<KrylovDev>();
+    hipStream_t s = edx::stream();
+    EDX_HIP(hipMemsetAsync(vb.p, 0, 2 * n * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(ab.p, 0, (2 * (size_t)maxn + 2) * sizeof(double), s));
+    EDX_HIP(hipMemsetAsync(run, 0, ((size_t)maxn + 2) * sizeof(int), s));
+    EDX_HIP(hipMemcpyAsync(v[0], v0, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_krylov_init, dim3(1), dim3(1), 0, s, S, run);
+    // norm = <v0|v0>; v1 = v0 / sqrt(norm)   (src/lanczos.f90:256-262)
+    hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, v[0], v[0], part.as<double2>(), nullptr);
+    hipLaunchKernelGGL(k_krylov_norm, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S);
+    hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[0], &S->sqrt_norm, nullptr);
+    EDX_LAUNCH_CHECK();
+    int curr = 0, prev = 1;
+    int checked = 0;
+    for (int i = 1; i <= maxn; ++i) {
+        // prev = -beta_{i-1} prev + H curr ; partial <curr|prev>
+        SpmvEpi e{};
+        e.y = v[prev];
+        e.z = v[prev];
+        e.coef = S->coef;
+        e.dvec = v[curr];
+        e.partials = part.as<double2>();
+        e.run = run + (i - 1);
+        int nb = 0;
+        EDX_TRY(apply(h, precision, v[curr], x32.as<float2>(), e, &nb, run + (i - 1)));
+        hipLaunchKernelGGL(k_krylov_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
+                           d_alpha, i, run);
+        hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
+                           d_alpha + (i - 1), part.as<double2>(), run + (i - 1));
+        hipLaunchKernelGGL(k_krylov_beta, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S,
+                           d_beta, i, run);
+        hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[prev], d_beta + i, run + i);
+        EDX_LAUNCH_CHECK();
+        std::swap(curr, prev);
+        if (i - checked >= 32 && i < maxn) {   // stop launching once the recurrence broke down
+            int r = 1;
+            EDX_HIP(hipMemcpy(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
+            checked = i;
+            if (!r) break;
+        }
+    }
+    KrylovDev hs;
+    EDX_HIP(hipMemcpy(&hs, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
+    std::vector<double> hb(maxn + 1);
+    EDX_HIP(hipMemcpy(alpha, d_alpha, maxn * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(hipMemcpy(hb.data(), d_beta, (maxn + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    *neff = hs.neff;
+    *norm = hs.norm;
+    for (int i = 0; i < maxn; ++i) beta[i] = hb[i + 1];   // beta_1 .. beta_maxn
+    return EDX_OK;
+}
+
+int minres_run(const edx_csr *a, int precision, const double2 *b, double2 *x, double tol, int maxit,
+               int *iters, double *final_res) {
+    const int64_t n = a->n;
+    DevBuf vb, x32, part, st, runf;
+    EDX_TRY(vb.alloc(5 * n * sizeof(double2), "minres vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "minres x32"));
+    int nbs = edx::spmv_blocks(a, 64);
+    int nbv = vec_blocks(n);
+    EDX_TRY(part.alloc((size_t)std::max(nbs, nbv) * 4 * sizeof(double2) + 64, "partials"));
+    EDX_TRY(st.alloc(sizeof(MinresDev), "minres state"));
+    const int nflags = (maxit > 0 ? maxit : 0) + 2;
+    EDX_TRY(runf.alloc((size_t)nflags * sizeof(int), "run flags"));
+    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + n};
+    double2 *w[2] = {vb.as<double2>() + 2 * n, vb.as<double2>() + 3 * n};
+    double2 *tmp = vb.as<double2>() + 4 * n;
+    int *run = runf.as<int>();
+    MinresDev *S = st.as<MinresDev>();
+    hipStream_t s = edx::stream();
+    EDX_HIP(hipMemsetAsync(vb.p, 0, 5 * n * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(x, 0, n * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(run, 0, (size_t)nflags * sizeof(int), s));
+    EDX_HIP(hipMemcpyAsync(v[0], b, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, v[0], v[0], part.as<double2>(), nullptr);
+    hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S, run, tol,
+                       maxit);
+    EDX_LAUNCH_CHECK();
+    int curr = 0, prev = 1, checked = 0;
+    for (int i = 0; i < maxit; ++i) {
+        hipLaunchKernelGGL(k_minres_divbeta, dim3(nbv), dim3(VB), 0, s, n, v[curr], S, run, i);
+        SpmvEpi e{};
+        e.y = tmp;
+        e.dvec = v[curr];
+        e.partials = part.as<double2>();
+        e.run = run + i;
+        int nb = 0;
+        EDX_TRY(apply(a, precision, v[curr], x32.as<float2>(), e, &nb, run + i));
+        hipLaunchKernelGGL(k_minres_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S, run, i);
+        hipLaunchKernelGGL(k_minres_vupdate, dim3(nbv), dim3(VB), 0, s, n, tmp, v[curr], v[prev], S,
+                           part.as<double2>(), run, i);
+        hipLaunchKernelGGL(k_minres_scalars, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S,
+                           run, i);
+        hipLaunchKernelGGL(k_minres_wx, dim3(nbv), dim3(VB), 0, s, n, v[curr], w[prev], w[curr], x,
+                           S, run, i);
+        EDX_LAUNCH_CHECK();
+        std::swap(curr, prev);
+        if (i + 1 - checked >= 16) {
+            int r = 1;
+            EDX_HIP(hipMemcpy(&r, run + i + 1, sizeof(int), hipMemcpyDeviceToHost));
+            checked = i + 1;
+            if (!r) break;
+        }
+    }
+    // final true residual |A x - b| (src/linsys.f90:117-125)
+    SpmvEpi e{};
+    e.y = tmp;
+    EDX_TRY(apply(a, precision, x, x32.as<float2>(), e, nullptr, nullptr));
+    hipLaunchKernelGGL(k_sub_vec, dim3(nbv), dim3(VB), 0, s, n, tmp, b);
+    hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, tmp, tmp, part.as<double2>(), nullptr);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv,
+                       part.as<double2>() + nbv, nullptr);
+    EDX_LAUNCH_CHECK();
+    double2 r2;
+    EDX_HIP(hipMemcpy(&r2, part.as<double2>() + nbv, sizeof(double2), hipMemcpyDeviceToHost));
+    MinresDev hs;
+    EDX_HIP(hipMemcpy(&hs, S, sizeof(MinresDev), hipMemcpyDeviceToHost));
+    if (iters) *iters = hs.iters;
+    if (final_res) *final_res = sqrt(fabs(r2.x));
+    return EDX_OK;
+}
+
+// ------------------------------------------------------------- Lanczos
+
+struct Lanczos {
+    const edx_csr *h;
+    int precision;
+    int64_t n;
+    double2 *basis;        // columns: [locked vectors | Krylov vectors]
+    int64_t ld;
+    int cap_cols;
+    float2 *x32;
+    double2 *w;
+    double2 *part;
+    double2 *hvec;
+    int nbv;
+};
+
+int host_dot(Lanczos &L, const double2 *a, const double2 *b, double2 *out) {
+    hipLaunchKernelGGL(k_dot, dim3(L.nbv), dim3(VB), 0, edx::stream(), L.n, a, b, L.part, nullptr);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(VB), 0, edx::stream(), L.part, L.nbv,
+                       L.part + L.nbv, nullptr);
+    EDX_LAUNCH_CHECK();
+    EDX_HIP(hipMemcpy(out, L.part + L.nbv, sizeof(double2), hipMemcpyDeviceToHost));
+    return EDX_OK;
+}
+
+// w -= Q Q^H w over the first k basis columns; returns the coefficients in hk (host)
+int orthogonalize(Lanczos &L, int k, double2 *w, std::vector<double2> *hk) {
+    if (k <= 0) return EDX_OK;
+    const int nbx = std::min(L.nbv, 64);
+    dim3 grid(nbx, (k + 7) / 8);
+    hipLaunchKernelGGL(k_gemv_ct, grid, dim3(VB), 0, edx::stream(), L.n, L.basis, L.ld, k, w, L.part);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((k + 255) / 256), dim3(256), 0, edx::stream(), L.part,
+                       nbx, k, L.hvec);
+    hipLaunchKernelGGL(k_gemv_n_sub, dim3(L.nbv), dim3(VB), 0, edx::stream(), L.n, L.basis, L.ld, k,
+                       L.hvec, w);
+    EDX_LAUNCH_CHECK();
+    if (hk) {
+        hk->resize(k);
+        EDX_HIP(hipMemcpy(hk->data(), L.hvec, k * sizeof(double2), hipMemcpyDeviceToHost));
+    }
+    return EDX_OK;
+}
+
+// One Lanczos run started from basis column `first` (already normalised and
+// orthogonal to the locked columns [0, first)).  Returns Ritz pairs of the run.
+int lanczos_run(Lanczos &L, int first, int nev, int maxiter, double tol, std::vector<double> &ritz,
+                std::vector<double> &S, int &m_out, int &iters) {
+    std::vector<double> al, be;   // be[j] couples v_j and v_{j+1}
+    const int maxm = std::min(maxiter, L.cap_cols - first - 1);
+    int m = 0;
+    hipStream_t s = edx::stream();
+    for (int j = 0; j < maxm; ++j) {
+        double2 *vj = L.basis + (int64_t)(first + j) * L.ld;
+        SpmvEpi e{};
+        e.y = L.w;
+        EDX_TRY(apply(L.h, L.precision, vj, L.x32, e, nullptr, nullptr));
+        double2 a;
+        EDX_TRY(host_dot(L, vj, L.w, &a));
+        double alpha = a.x;
+        const double2 *vprev = j > 0 ? L.basis + (int64_t)(first + j - 1) * L.ld : nullptr;
+        hipLaunchKernelGGL(k_axpby_sub, dim3(L.nbv), dim3(VB), 0, s, L.n, L.w, vj, alpha, vprev,
+                           j > 0 ? be[j - 1] : 0.0);
+        EDX_LAUNCH_CHECK();
+        std::vector<double2> h1;
+        EDX_TRY(orthogonalize(L, first + j + 1, L.w, &h1));
+        EDX_TRY(orthogonalize(L, first + j + 1, L.w, nullptr));
+        alpha += h1[first + j].x;
+        double2 nb2;
+        EDX_TRY(host_dot(L, L.w, L.w, &nb2));
+        const double beta = sqrt(fabs(nb2.x));
+        al.push_back(alpha);
+        be.push_back(beta);
+        m = j + 1;
+        ++iters;
+        bool last = (beta < 1e-10) || (m == maxm);
+        bool check = last || (m >= nev && (m % 5 == 0));
+        if (check) {
+            std::vector<double> w(m), zl(m);
+            EDX_CHECK(edx::tridiag_ql(m, al.data(), be.data(), w.data(), nullptr, zl.data()) == 0,
+                      EDX_ERR_NUMERIC, "tridiagonal eigensolver did not converge");
+            int nconv = 0;
+            for (int k = 0; k < std::min(nev, m); ++k)
+                if (fabs(beta * zl[k]) < tol) ++nconv;
+            if (nconv >= std::min(nev, m) || last) break;
+        }
+        if (beta < 1e-10) break;
+        hipLaunchKernelGGL(k_copy_div, dim3(L.nbv), dim3(VB), 0, s, L.n, L.w, beta,
+                           L.basis + (int64_t)(first + j + 1) * L.ld);
+        EDX_LAUNCH_CHECK();
+    }
+    ritz.assign(m, 0.0);
+    S.assign((size_t)m * m, 0.0);
+    EDX_CHECK(edx::tridiag_ql(m, al.data(), be.data(), ritz.data(), S.data(), nullptr) == 0,
+              EDX_ERR_NUMERIC, "tridiagonal eigensolver did not converge");
+    m_out = m;
+    return EDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_build_krylov(const edx_csr *h, int precision, const void *d_v0, int maxn, int *neff,
+                     double *alpha, double *beta, double *norm) {
+    EDX_TRY(check_square(h, precision));
+    EDX_CHECK(d_v0 && neff && alpha && beta && norm && maxn >= 1, EDX_ERR_ARG, "bad argument");
+    int rc = krylov_run(h, precision, (const double2 *)d_v0, maxn, neff, alpha, beta, norm);
+    if (rc) return rc;
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    return EDX_OK;
+}
+
+int edx_minres(const edx_csr *a, int precision, const void *d_b, void *d_x, double tol, int maxit,
+               int *iters, double *final_res) {
+    EDX_TRY(check_square(a, precision));
+    EDX_CHECK(d_b && d_x && maxit >= 0, EDX_ERR_ARG, "bad argument");
+    int rc = minres_run(a, precision, (const double2 *)d_b, (double2 *)d_x, tol, maxit, iters,
+                        final_res);
+    if (rc) return rc;
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    return EDX_OK;
+}
+
+int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                     uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
+                     int *niter_out) {
+    EDX_TRY(check_square(h, precision));
+    EDX_CHECK(nev >= 1 && maxiter >= 1 && evals && nvec >= 0 && nvec <= nev &&
+                  (nvec == 0 || d_evecs),
+              EDX_ERR_ARG, "bad argument (nev=%d maxiter=%d nvec=%d)", nev, maxiter, nvec);
+    const int64_t n = h->n;
+    if (nev > n) nev = (int)n;
+    if (nvec > nev) nvec = nev;
+    if (maxiter > n) maxiter = (int)n;
+    const int max_rounds = deflate ? 2 * nev + 2 : 1;
+    const int cap_cols = nev + maxiter + 2;
+    DevBuf basis, x32, w, part, hv, tmpS, evec_all;
+    EDX_TRY(basis.alloc((size_t)n * cap_cols * sizeof(double2), "lanczos basis"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "lanczos x32"));
+    EDX_TRY(w.alloc(n * sizeof(double2), "lanczos w"));
+    const int nbv = vec_blocks(n);
+    EDX_TRY(part.alloc(((size_t)cap_cols * 64 + nbv + 64) * sizeof(double2), "partials"));
+    EDX_TRY(hv.alloc((size_t)cap_cols * sizeof(double2), "coefficients"));
+    EDX_TRY(tmpS.alloc((size_t)cap_cols * cap_cols * sizeof(double), "ritz vectors"));
+    EDX_TRY(evec_all.alloc((size_t)n * nev * sizeof(double2), "eigenvectors"));
+    Lanczos L{h, precision, n, basis.as<double2>(), n, cap_cols, x32.as<float2>(), w.as<double2>(),
+              part.as<double2>(), hv.as<double2>(), nbv};
+    hipStream_t s = edx::stream();
+    std::vector<double> locked_vals;   // eigenvalues of locked columns [0, nlock)
+    int nlock = 0, iters = 0;
+    for (int round = 0; round < max_rounds; ++round) {
+        double2 *v0 = L.basis + (int64_t)nlock * n;
+        hipLaunchKernelGGL(k_random_real, dim3(nbv), dim3(VB), 0, s, n,
+                           seed + 0x9e37ull * (uint64_t)round, v0);
+        EDX_LAUNCH_CHECK();
+        EDX_TRY(orthogonalize(L, nlock, v0, nullptr));
+        EDX_TRY(orthogonalize(L, nlock, v0, nullptr));
+        double2 nn;
+        EDX_TRY(host_dot(L, v0, v0, &nn));
+        if (sqrt(nn.x) < 1e-12) break;   // nothing left outside the locked space
+        hipLaunchKernelGGL(k_scale_real, dim3(nbv), dim3(VB), 0, s, n, v0, 1.0 / sqrt(nn.x));
+        std::vector<double> ritz, S;
+        int m = 0;
+        EDX_TRY(lanczos_run(L, nlock, nev, maxiter, tol, ritz, S, m, iters));
+        // Ritz vectors of this run for its lowest values
+        const int take = std::min(nev, m);
+        std::vector<double> Sk((size_t)m * take);
+        for (int j = 0; j < take; ++j)
+            for (int c = 0; c < m; ++c) Sk[(size_t)j * m + c] = S[(size_t)j * m + c];
+        EDX_HIP(hipMemcpy(tmpS.p, Sk.data(), Sk.size() * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_combine, dim3(nbv), dim3(VB), 0, s, n, L.basis + (int64_t)nlock * n, n,
+                           m, tmpS.as<double>(), take, evec_all.as<double2>(), n);
+        EDX_LAUNCH_CHECK();
+        // merge: locked (already lowest of previous rounds) with this run's Ritz values
+        std::vector<std::pair<double, int>> cand;   // (value, source: <0 locked k, >=0 new j)
+        for (int k = 0; k < nlock; ++k) cand.push_back({locked_vals[k], -1 - k});
+        for (int j = 0; j < take; ++j) cand.push_back({ritz[j], j});
+        std::stable_sort(cand.begin(), cand.end(),
+                         [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+                             return a.first < b.first;
+                         });
+        const int keep = std::min(nev, (int)cand.size());
+        int new_taken = 0;
+        for (int q = 0; q < keep; ++q)
+            if (cand[q].second >= 0) ++new_taken;
+        if (round > 0 && new_taken == 0) break;   // no missed level below the nev-th
+        // rebuild locked set [0, keep) in value order
+        DevBuf keepbuf;
+        EDX_TRY(keepbuf.alloc((size_t)n * keep * sizeof(double2), "locked copy"));
+        for (int q = 0; q < keep; ++q) {
+            const double2 *src = cand[q].second >= 0
+                                     ? evec_all.as<double2>() + (int64_t)cand[q].second * n
+                                     : L.basis + (int64_t)(-1 - cand[q].second) * n;
+            EDX_HIP(hipMemcpyAsync(keepbuf.as<double2>() + (int64_t)q * n, src, n * sizeof(double2),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        EDX_HIP(hipMemcpyAsync(L.basis, keepbuf.p, (size_t)n * keep * sizeof(double2),
+                               hipMemcpyDeviceToDevice, s));
+        locked_vals.assign(keep, 0.0);
+        for (int q = 0; q < keep; ++q) locked_vals[q] = cand[q].first;
+        nlock = keep;
+        if (!deflate) break;
+        if (nlock + 2 >= cap_cols || nlock >= n) break;
+    }
+    for (int k = 0; k < nev; ++k) evals[k] = k < nlock ? locked_vals[k] : 0.0;
+    if (nvec > 0)
+        EDX_HIP(hipMemcpyAsync(d_evecs, L.basis, (size_t)n * std::min(nvec, nlock) * sizeof(double2),
+                               hipMemcpyDeviceToDevice, s));
+    EDX_HIP(hipStreamSynchronize(s));
+    if (niter_out) *niter_out = iters;
+    return EDX_OK;
+}
+
+}  // extern "C"

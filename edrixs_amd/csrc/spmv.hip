@@ -1,0 +1,339 @@
+// Complex CSR SpMV / SpMM (part of edx_kernels.hip).
+//
+// Reference: matvec_csr, src/spmv.f90:73-103, y(i) += sum_j aa(j) x(jaa(j)).
+// Here a group of G lanes (G = 4..64, chosen from the mean row length) owns a
+// row: lane l streams nonzeros l, l+G, ... of the row with non-temporal loads
+// (values and column indices are read once per product, x is gathered through
+// the caches), the G partial sums are combined with cross-lane shuffles, and
+// the group leader applies a fused epilogue
+//     y[r] = c0 * (A x)[r] + c1 * z[r]           (c0, c1 in device memory)
+// optionally accumulating conj(d[r]) * y[r] into per-block partial sums for the
+// next Krylov scalar.  Reductions are per block in a fixed order, so results
+// are identical run to run.
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ T nt_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+__device__ __forceinline__ double2 nt_load2(const double2 *p) {
+    const double *q = reinterpret_cast<const double *>(p);
+    return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+
+__device__ __forceinline__ float2 nt_load2(const float2 *p) {
+    const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(p));
+    return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+}
+
+template <class A>
+struct AccT;
+template <>
+struct AccT<double2> {
+    using type = double;
+};
+template <>
+struct AccT<float2> {
+    using type = float;
+};
+
+struct SpmvEpi {
+    double2 *y;
+    const double2 *z;       // may alias y; nullptr -> no addend
+    const double2 *coef;    // coef[0] scales A x, coef[1] scales z; nullptr -> 1, 1
+    const double2 *dvec;    // nullptr -> no dot
+    double2 *partials;      // one per block when dvec != nullptr
+    const int *run;         // nullptr or flag: skip the launch when *run == 0
+};
+
+// block-level sum of a complex value (256 threads), result valid in thread 0
+__device__ __forceinline__ double2 block_sum_256(double2 v) {
+    __shared__ double sx[4], sy[4];
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sx[wave] = v.x;
+        sy[wave] = v.y;
+    }
+    __syncthreads();
+    double2 r = make_double2(0, 0);
+    if (threadIdx.x == 0) {
+        r.x = (sx[0] + sx[1]) + (sx[2] + sx[3]);
+        r.y = (sy[0] + sy[1]) + (sy[2] + sy[3]);
+    }
+    __syncthreads();
+    return r;
+}
+
+template <int G, class VT, class XT>
+__global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ cols,
+                                              const VT *__restrict__ vals,
+                                              const XT *__restrict__ x, SpmvEpi e) {
+    if (e.run && *e.run == 0) return;
+    using acc_t = typename AccT<VT>::type;
+    const int lig = threadIdx.x & (G - 1);
+    const int64_t group = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+    const int64_t ngroups = (int64_t)gridDim.x * (256 / G);
+    double2 c0 = make_double2(1.0, 0.0), c1 = make_double2(1.0, 0.0);
+    if (e.coef) {
+        c0 = e.coef[0];
+        c1 = e.coef[1];
+    }
+    double2 dot = make_double2(0.0, 0.0);
+    for (int64_t r = group; r < m; r += ngroups) {
+        const int64_t b = rowptr[r], end = rowptr[r + 1];
+        acc_t sr = 0, si = 0;
+        int64_t j = b + lig;
+        for (; j + G < end; j += 2 * G) {   // two independent loads in flight per lane
+            const VT a0 = nt_load2(vals + j);
+            const VT a1 = nt_load2(vals + j + G);
+            const int32_t k0 = nt_load(cols + j);
+            const int32_t k1 = nt_load(cols + j + G);
+            const XT x0 = x[k0];
+            const XT x1 = x[k1];
+            sr += (acc_t)a0.x * (acc_t)x0.x - (acc_t)a0.y * (acc_t)x0.y;
+            si += (acc_t)a0.x * (acc_t)x0.y + (acc_t)a0.y * (acc_t)x0.x;
+            sr += (acc_t)a1.x * (acc_t)x1.x - (acc_t)a1.y * (acc_t)x1.y;
+            si += (acc_t)a1.x * (acc_t)x1.y + (acc_t)a1.y * (acc_t)x1.x;
+        }
+        if (j < end) {
+            const VT a0 = nt_load2(vals + j);
+            const XT x0 = x[nt_load(cols + j)];
+            sr += (acc_t)a0.x * (acc_t)x0.x - (acc_t)a0.y * (acc_t)x0.y;
+            si += (acc_t)a0.x * (acc_t)x0.y + (acc_t)a0.y * (acc_t)x0.x;
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, G);
+            si += __shfl_xor(si, off, G);
+        }
+        if (lig == 0) {
+            double2 out = cmul(c0, make_double2((double)sr, (double)si));
+            if (e.z) out = cadd(out, cmul(c1, e.z[r]));
+            e.y[r] = out;
+            if (e.dvec) dot = cadd(dot, cconjmul(e.dvec[r], out));
+        }
+    }
+    if (e.dvec) {
+        double2 s = block_sum_256(dot);
+        if (threadIdx.x == 0) e.partials[blockIdx.x] = s;
+    }
+}
+
+// Multi-RHS: Y[:, k] = c0 * A X[:, k] + c1 * Z[:, k], column-major blocks of
+// nrhs columns.  Each lane group handles one row and NR right-hand sides at a
+// time, so every value / column index read serves NR products.
+template <int G, int NR, class VT, class XT>
+__global__ __launch_bounds__(256) void k_spmm(int64_t m, int64_t n, const int64_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ cols,
+                                              const VT *__restrict__ vals,
+                                              const XT *__restrict__ X, int nrhs,
+                                              double2 *__restrict__ Y,
+                                              const double2 *__restrict__ Z) {
+    using acc_t = typename AccT<VT>::type;
+    const int lig = threadIdx.x & (G - 1);
+    const int64_t group = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+    const int64_t ngroups = (int64_t)gridDim.x * (256 / G);
+    const int k0 = blockIdx.y * NR;
+    for (int64_t r = group; r < m; r += ngroups) {
+        const int64_t b = rowptr[r], end = rowptr[r + 1];
+        acc_t sr[NR], si[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) sr[q] = si[q] = 0;
+        for (int64_t j = b + lig; j < end; j += G) {
+            const VT a = nt_load2(vals + j);
+            const int64_t c = nt_load(cols + j);
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                if (k0 + q < nrhs) {
+                    const XT xv = X[(int64_t)(k0 + q) * n + c];
+                    sr[q] += (acc_t)a.x * (acc_t)xv.x - (acc_t)a.y * (acc_t)xv.y;
+                    si[q] += (acc_t)a.x * (acc_t)xv.y + (acc_t)a.y * (acc_t)xv.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+#pragma unroll
+            for (int off = G / 2; off > 0; off >>= 1) {
+                sr[q] += __shfl_xor(sr[q], off, G);
+                si[q] += __shfl_xor(si[q], off, G);
+            }
+        }
+        if (lig == 0) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                if (k0 + q < nrhs) {
+                    const int64_t o = (int64_t)(k0 + q) * m + r;
+                    double2 out = make_double2((double)sr[q], (double)si[q]);
+                    if (Z) out = cadd(out, Z[o]);
+                    Y[o] = out;
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_to_c64_vec(int64_t n, const double2 *in, float2 *out, const int *run) {
+    if (run && *run == 0) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+int lanes_for(const edx_csr *a) {
+    double avg = a->m ? (double)a->nnz / (double)a->m : 0.0;
+    if (avg < 12) return 4;
+    if (avg < 40) return 8;
+    if (avg < 160) return 16;
+    if (avg < 480) return 32;
+    return 64;
+}
+
+}  // namespace
+
+namespace edx {
+
+int spmv_blocks(const edx_csr *a, int G) {
+    int64_t groups_needed = a->m;
+    int64_t blocks = (groups_needed * G + 255) / 256;
+    int64_t cap = (int64_t)num_cus() * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    return (int)blocks;
+}
+
+// Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
+// complex64 (already converted) for C64.  Returns the block count used.
+int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e, int lanes,
+                int *nblocks_out) {
+    int G = lanes ? lanes : (precision == EDX_C64 ? (a->spmv_lanes32 ? a->spmv_lanes32 : lanes_for(a))
+                                                  : (a->spmv_lanes ? a->spmv_lanes : lanes_for(a)));
+    int nb = spmv_blocks(a, G);
+    if (nblocks_out) *nblocks_out = nb;
+    if (a->m == 0) return EDX_OK;
+#define EDX_SPMV_CASE(GG)                                                                     \
+    case GG:                                                                                  \
+        if (precision == EDX_C128)                                                            \
+            hipLaunchKernelGGL((k_spmv<GG, double2, double2>), dim3(nb), dim3(256), 0, stream(), \
+                               a->m, a->d_rowptr, a->d_cols, a->d_vals, (const double2 *)x, e); \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_spmv<GG, float2, float2>), dim3(nb), dim3(256), 0, stream(),   \
+                               a->m, a->d_rowptr, a->d_cols, a->d_vals32, (const float2 *)x, e); \
+        break;
+    switch (G) {
+        EDX_SPMV_CASE(4)
+        EDX_SPMV_CASE(8)
+        EDX_SPMV_CASE(16)
+        EDX_SPMV_CASE(32)
+        EDX_SPMV_CASE(64)
+        default: return set_error(EDX_ERR_ARG, "unsupported lanes per row %d", G);
+    }
+#undef EDX_SPMV_CASE
+    EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
+int to_c64(int64_t n, const double2 *in, float2 *out, const int *run) {
+    int64_t g = (n + 255) / 256;
+    if (g > (int64_t)num_cus() * 16) g = (int64_t)num_cus() * 16;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_to_c64_vec, dim3((unsigned)g), dim3(256), 0, stream(), n, in, out, run);
+    EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
+int spmm_launch(const edx_csr *a, int precision, int nrhs, const void *X, double2 *Y,
+                const double2 *Z) {
+    if (a->m == 0 || nrhs == 0) return EDX_OK;
+    const int G = lanes_for(a) > 16 ? 16 : lanes_for(a);
+    constexpr int NR = 8;
+    int64_t blocks = (a->m * G + 255) / 256;
+    int64_t cap = (int64_t)num_cus() * 8;
+    if (blocks > cap) blocks = cap;
+    dim3 grid((unsigned)blocks, (unsigned)((nrhs + NR - 1) / NR));
+#define EDX_SPMM_CASE(GG)                                                                       \
+    case GG:                                                                                    \
+        if (precision == EDX_C128)                                                              \
+            hipLaunchKernelGGL((k_spmm<GG, NR, double2, double2>), grid, dim3(256), 0, stream(),   \
+                               a->m, a->n, a->d_rowptr, a->d_cols, a->d_vals, (const double2 *)X, \
+                               nrhs, Y, Z);                                                     \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_spmm<GG, NR, float2, float2>), grid, dim3(256), 0, stream(),     \
+                               a->m, a->n, a->d_rowptr, a->d_cols, a->d_vals32, (const float2 *)X, \
+                               nrhs, Y, Z);                                                     \
+        break;
+    switch (G) {
+        EDX_SPMM_CASE(4)
+        EDX_SPMM_CASE(8)
+        EDX_SPMM_CASE(16)
+        default: return set_error(EDX_ERR_ARG, "unsupported lanes %d", G);
+    }
+#undef EDX_SPMM_CASE
+    EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
+}  // namespace edx
+
+extern "C" {
+
+int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y) {
+    EDX_CHECK(a && d_x && d_y, EDX_ERR_ARG, "null argument");
+    EDX_CHECK(precision == EDX_C128 || (precision == EDX_C64 && a->d_vals32), EDX_ERR_ARG,
+              "precision %d not available on this matrix", precision);
+    SpmvEpi e{};
+    e.y = (double2 *)d_y;
+    e.z = (const double2 *)d_y;   // y += A x
+    const void *x = d_x;
+    float2 *x32 = nullptr;
+    if (precision == EDX_C64) {
+        x32 = (float2 *)edx::dmalloc((a->n ? a->n : 1) * sizeof(float2), "x c64");
+        EDX_CHECK(x32, EDX_ERR_NOMEM, "%s", edx_last_error());
+        int rc = edx::to_c64(a->n, (const double2 *)d_x, x32, nullptr);
+        if (rc) {
+            edx::dfree(x32);
+            return rc;
+        }
+        x = x32;
+    }
+    int rc = edx::spmv_launch(a, precision, x, e, 0, nullptr);
+    hipError_t s = hipStreamSynchronize(edx::stream());
+    edx::dfree(x32);
+    if (rc) return rc;
+    EDX_HIP(s);
+    return EDX_OK;
+}
+
+int edx_spmm(const edx_csr *a, int precision, int nrhs, const void *d_X, void *d_Y) {
+    EDX_CHECK(a && d_X && d_Y && nrhs >= 0, EDX_ERR_ARG, "bad argument");
+    EDX_CHECK(precision == EDX_C128 || (precision == EDX_C64 && a->d_vals32), EDX_ERR_ARG,
+              "precision %d not available on this matrix", precision);
+    const void *X = d_X;
+    float2 *x32 = nullptr;
+    if (precision == EDX_C64) {
+        int64_t len = a->n * (int64_t)nrhs;
+        x32 = (float2 *)edx::dmalloc((len ? len : 1) * sizeof(float2), "X c64");
+        EDX_CHECK(x32, EDX_ERR_NOMEM, "%s", edx_last_error());
+        int rc = edx::to_c64(len, (const double2 *)d_X, x32, nullptr);
+        if (rc) {
+            edx::dfree(x32);
+            return rc;
+        }
+        X = x32;
+    }
+    int rc = edx::spmm_launch(a, precision, nrhs, X, (double2 *)d_Y, (const double2 *)d_Y);
+    hipError_t s = hipStreamSynchronize(edx::stream());
+    edx::dfree(x32);
+    if (rc) return rc;
+    EDX_HIP(s);
+    return EDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""File-driven orchestration around the drop-in solvers.
+
+Counterparts of ``ed_siam_fort`` / ``xas_siam_fort`` / ``rixs_siam_fort``
+(edrixs/solvers.py:1819-2704) for :class:`edrixs_amd.siam.SiamModel`: write the
+term, fock and config files, call ``fedrixs`` (ours by default), read the
+results back, and post-process poles into spectra (isotropic XAS averages the
+three dipole components, edrixs/solvers.py:2453-2470).
+"""
+import contextlib
+import os
+
+import numpy as np
+
+from . import iostream, spectra
+
+
+@contextlib.contextmanager
+def _cwd(path):
+    old = os.getcwd()
+    os.makedirs(path, exist_ok=True)
+    os.chdir(path)
+    try:
+        yield
+    finally:
+        os.chdir(old)
+
+
+def _fsolver(fsolver):
+    if fsolver is None:
+        from . import fedrixs as fsolver
+    return fsolver
+
+
+def ed_siam(model, workdir=".", neval=3, nvector=3, maxiter=500, eigval_tol=1e-10,
+            ed_solver=1, min_ndim=100, fsolver=None):
+    """Ground state(s): eigvals.dat, denmat.dat and eigvec.N (edrixs/solvers.py:2123-2148)."""
+    fs = _fsolver(fsolver)
+    with _cwd(workdir):
+        iostream.write_terms(model.hop_i, "hopping_i.in")
+        iostream.write_terms(model.coul_i, "coulomb_i.in")
+        iostream.write_terms(model.hop_n, "hopping_n.in")
+        iostream.write_terms(model.coul_n, "coulomb_n.in")
+        iostream.write_config(ed_solver=ed_solver, num_val_orbs=model.norb, neval=neval,
+                              nvector=nvector, ncv=neval + 2, idump=True, maxiter=maxiter,
+                              min_ndim=min_ndim, eigval_tol=eigval_tol)
+        iostream.write_fock_dec_by_N(model.norb, model.nocc, "fock_i.in")
+        fs.ed_fsolver(0, 0, 1)
+        data = np.loadtxt("eigvals.dat", ndmin=2)
+        evals = data[:neval, 1]
+        d = np.loadtxt("denmat.dat", ndmin=2)
+        shape = (nvector, model.norb, model.norb)
+        denmat = d[:, 3].reshape(shape) + 1j * d[:, 4].reshape(shape)
+    return evals, denmat
+
+
+def xas_siam(model, ominc, workdir=".", gamma_c=0.5, num_gs=1, nkryl=300, temperature=1.0,
+             fsolver=None, write_fock=True):
+    """Isotropic XAS over the three dipole components (edrixs/solvers.py:2268-2475)."""
+    fs = _fsolver(fsolver)
+    om = np.asarray(ominc, dtype=float)
+    gam = np.broadcast_to(np.asarray(gamma_c, dtype=float), om.shape)
+    xas = np.zeros(len(om))
+    poles = []
+    with _cwd(workdir):
+        iostream.write_config(num_val_orbs=model.norb, num_core_orbs=model.ncore,
+                              num_gs=num_gs, nkryl=nkryl)
+        if write_fock:
+            iostream.write_fock_dec_by_N(model.norb, model.nocc, "fock_i.in")
+            iostream.write_fock_dec_by_N(model.norb, model.nocc + 1, "fock_n.in")
+        iostream.write_terms(model.hop_n, "hopping_n.in")
+        iostream.write_terms(model.coul_n, "coulomb_n.in")
+        for k in range(3):
+            iostream.write_terms(model.trans_i[k], "transop_xas.in")
+            fs.xas_fsolver(0, 0, 1)
+            p = iostream.read_poles_from_file(["xas_poles.%d" % (i + 1) for i in range(num_gs)])
+            xas += spectra.get_spectra_from_poles(p, om, gam, temperature)
+            poles.append(p)
+    return xas / 3.0, spectra.merge_pole_dicts(poles)
+
+
+def rixs_siam(model, ominc, eloss, pol_vectors, workdir=".", gamma_c=0.5, gamma_f=0.1,
+              num_gs=1, nkryl=300, linsys_max=1000, linsys_tol=1e-10, temperature=1.0,
+              fsolver=None, write_fock=True):
+    """RIXS for each incident energy and (e_in, e_out) polarization pair
+    (edrixs/solvers.py:2478-2704).  Returns rixs[iom, ieloss, ipol] and poles."""
+    fs = _fsolver(fsolver)
+    om = np.asarray(ominc, dtype=float)
+    el = np.asarray(eloss, dtype=float)
+    gc = np.broadcast_to(np.asarray(gamma_c, dtype=float), om.shape)
+    gf = np.broadcast_to(np.asarray(gamma_f, dtype=float), el.shape)
+    rixs = np.zeros((len(om), len(el), len(pol_vectors)))
+    poles = []
+    with _cwd(workdir):
+        if write_fock:
+            iostream.write_fock_dec_by_N(model.norb, model.nocc, "fock_i.in")
+            iostream.write_fock_dec_by_N(model.norb, model.nocc + 1, "fock_n.in")
+            iostream.write_fock_dec_by_N(model.norb, model.nocc, "fock_f.in")
+        iostream.write_terms(model.hop_i, "hopping_i.in")
+        iostream.write_terms(model.coul_i, "coulomb_i.in")
+        iostream.write_terms(model.hop_n, "hopping_n.in")
+        iostream.write_terms(model.coul_n, "coulomb_n.in")
+        for iom, omega in enumerate(om):
+            iostream.write_config(num_val_orbs=model.norb, num_core_orbs=model.ncore,
+                                  omega_in=omega, gamma_in=gc[iom], num_gs=num_gs, nkryl=nkryl,
+                                  linsys_max=linsys_max, linsys_tol=linsys_tol)
+            per_om = []
+            for ip, (ei, ef) in enumerate(pol_vectors):
+                ti, _ = model.pol_trans(ei)
+                _, tf = model.pol_trans(ef)
+                iostream.write_terms(ti, "transop_rixs_i.in")
+                iostream.write_terms(tf, "transop_rixs_f.in")
+                fs.rixs_fsolver(0, 0, 1)
+                p = iostream.read_poles_from_file(["rixs_poles.%d" % (i + 1)
+                                                   for i in range(num_gs)])
+                rixs[iom, :, ip] = spectra.get_spectra_from_poles(p, el, gf, temperature)
+                per_om.append(p)
+            poles.append(per_om)
+    return rixs, poles

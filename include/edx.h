@@ -1,0 +1,161 @@
+/*
+ * edx.h -- C ABI of libedx, the MI355X (gfx950) exact-diagonalization core.
+ *
+ * Every entry point returns 0 on success and a nonzero edx_status code on
+ * failure; edx_last_error() then holds a message for the calling thread.
+ * Pointers named d_* are device pointers (HIP global memory, e.g. the
+ * data_ptr() of a torch tensor on the current device); everything else is
+ * host memory.  Complex numbers are (re, im) pairs of double (complex128) or
+ * float (complex64), interleaved.  All indices crossing the ABI are 0-based.
+ *
+ * The drop-in layer (edx_*_fsolver) reproduces the f2py module the reference
+ * builds from src/pyapi.f90: same three integer arguments, same input files in
+ * the current directory, same output files.  The lower-level entry points
+ * replace the Fortran routines the drivers call, and are what the parity
+ * tests drive.
+ */
+#ifndef EDX_H
+#define EDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum edx_status {
+    EDX_OK = 0,
+    EDX_ERR_ARG = 1,        /* invalid argument / shape */
+    EDX_ERR_HIP = 2,        /* HIP runtime error (message has the HIP string) */
+    EDX_ERR_IO = 3,         /* missing or malformed input file */
+    EDX_ERR_NOMEM = 4,      /* device allocation failed / over budget */
+    EDX_ERR_NUMERIC = 5,    /* eigensolver / breakdown failure */
+    EDX_ERR_CAPACITY = 6    /* a row produced more candidates than supported */
+};
+
+/* value storage of a CSR matrix */
+enum edx_precision { EDX_C128 = 0, EDX_C64 = 1 };
+
+/* Term records, layout-compatible with T_tensor2 / T_tensor4
+ * (src/m_types.f90:7-20): 1-based spin-orbital indices, complex value. */
+typedef struct { int32_t ind1, ind2; double re, im; } edx_tensor2;
+typedef struct { int32_t ind1, ind2, ind3, ind4; double re, im; } edx_tensor4;
+
+typedef struct edx_basis edx_basis;   /* sorted Fock words resident in HBM */
+typedef struct edx_csr edx_csr;       /* CSR matrix resident in HBM */
+
+/* ------------------------------------------------------------- runtime */
+
+const char *edx_last_error(void);
+/* Select the HIP device used by this thread's calls (default 0). */
+int edx_init(int device);
+/* Device-memory bytes currently held by library objects. */
+int edx_mem_in_use(int64_t *bytes);
+
+/* --------------------------------------------------------- Fock basis */
+
+/* All words of nbits bits with popcount nocc in ascending order, enumerated on
+ * the device.  Replaces edrixs/fock_basis.py:398-451 (write_fock_dec_by_N) as the
+ * source of fock_*.in contents. */
+int edx_basis_combinations(int nbits, int nocc, edx_basis **out);
+/* Upload a sorted basis as read from fock_*.in (src/io.f90:174-199, 308-361).
+ * When it is the complete popcount set it is looked up by combinatorial rank,
+ * otherwise by binary search (src/utils.f90:2-40). */
+int edx_basis_from_words(const int64_t *words, int64_t n, int nbits, edx_basis **out);
+int edx_basis_size(const edx_basis *b, int64_t *n, int *is_complete);
+int edx_basis_download(const edx_basis *b, int64_t *words);
+/* Index of each word (or -1): replaces binary_search (src/utils.f90:2-40). */
+int edx_basis_lookup(const edx_basis *b, const int64_t *words, int64_t n, int64_t *index);
+void edx_basis_free(edx_basis *b);
+
+/* ------------------------------------------------------------ CSR build */
+
+/* H = sum hop + sum coul on one space; rows/columns = basis
+ * (src/ham.f90:2-349 build_ham_i).  ham_sign must be +1 or -1; with -1 the
+ * result is omega*I - H, omega added last on the diagonal (src/ham.f90:156-176). */
+int edx_build_ham_i(const edx_basis *fock, const edx_tensor2 *hop, int64_t nhopp,
+                    const edx_tensor4 *coul, int64_t ncoul, double omega_re,
+                    double omega_im, double ham_sign, int precision, edx_csr **out);
+/* Core-hole space H_n, rows (k, v) -> k*dim(fock_n)+v (src/ham.f90:352-711). */
+int edx_build_ham_n(const edx_basis *fock_n, int num_val_orbs, int num_core_orbs,
+                    const edx_tensor2 *hop, int64_t nhopp, const edx_tensor4 *coul,
+                    int64_t ncoul, double omega_re, double omega_im, double ham_sign,
+                    int precision, edx_csr **out);
+/* Absorption operator: rows = core-hole space over fock_n, columns = fock_i
+ * (src/ham.f90:714-912 build_transop_i). */
+int edx_build_transop_i(const edx_basis *fock_n, const edx_basis *fock_i, int num_val_orbs,
+                        int num_core_orbs, const edx_tensor2 *trans, int64_t ntran,
+                        int precision, edx_csr **out);
+/* Emission operator: rows = fock_f with full core, columns = core-hole space
+ * over fock_n (src/ham.f90:915-1115 build_transop_f). */
+int edx_build_transop_f(const edx_basis *fock_f, const edx_basis *fock_n, int num_val_orbs,
+                        int num_core_orbs, const edx_tensor2 *trans, int64_t ntran,
+                        int precision, edx_csr **out);
+int edx_csr_info(const edx_csr *a, int64_t *m, int64_t *n, int64_t *nnz, int *precision);
+/* Copy to host: rowptr[m+1], cols[nnz], vals[2*nnz] complex128 (any may be NULL). */
+int edx_csr_download(const edx_csr *a, int64_t *rowptr, int32_t *cols, void *vals);
+/* Copy the stored values of one precision: 2*nnz doubles (EDX_C128) or floats (EDX_C64). */
+int edx_csr_download_values(const edx_csr *a, int precision, void *vals);
+/* Add a complex64 value copy (rounded from complex128) for the mixed path. */
+int edx_csr_add_c64(edx_csr *a);
+void edx_csr_free(edx_csr *a);
+
+/* ------------------------------------------------------------------ SpMV */
+
+/* d_y += A d_x, complex128 vectors (src/spmv.f90:73-103 matvec_csr).
+ * precision selects the stored values used: EDX_C128 (FP64 accumulate) or
+ * EDX_C64 (complex64 values and x, FP32 accumulation, result added in FP64). */
+int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y);
+/* Multi-RHS: d_Y[:, k] += A d_X[:, k] for k < nrhs, column-major, leading
+ * dimensions m and n.  Same precision rules as edx_spmv. */
+int edx_spmm(const edx_csr *a, int precision, int nrhs, const void *d_X, void *d_Y);
+
+/* --------------------------------------------------------------- solvers */
+
+/* Lowest nev eigenpairs of a Hermitian CSR (replaces diag_ham_lanczos,
+ * src/lanczos.f90:6-167).  FP64 Lanczos with full reorthogonalization and a
+ * seeded start vector; degenerate copies are recovered by deflated restarts
+ * when deflate != 0.  evals[nev]; d_evecs is n x nvec complex128 column-major
+ * (may be NULL when nvec == 0).  niter_out: Krylov steps used (all restarts). */
+int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                     uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
+                     int *niter_out);
+/* Krylov tridiagonalization from an unnormalised start vector (replaces
+ * build_krylov_mp, src/lanczos.f90:210-303): alpha[maxn], beta[maxn]
+ * (beta_1..beta_neff as written to the pole files), norm = <v|v>. */
+int edx_build_krylov(const edx_csr *h, int precision, const void *d_v0, int maxn, int *neff,
+                     double *alpha, double *beta, double *norm);
+/* MINRES for A x = b (replaces pminres_csr, src/linsys.f90:3-128); stops when
+ * the recurrence residual estimate <= tol or after maxit steps.  d_x receives
+ * x; info[0] = iterations, final_res[0] = ||A x - b||. */
+int edx_minres(const edx_csr *a, int precision, const void *d_b, void *d_x, double tol,
+               int maxit, int *iters, double *final_res);
+
+/* Continued-fraction spectra on the device (edrixs/plot_spectrum.py:9-61):
+ * spec[w] += weight * (-1/pi) Im[norm / (z - a_1 - b_1^2/(z - a_2 - ...))],
+ * z = omega[w] + i*gamma[w] + e_gs; host arrays in and out. */
+int edx_cf_spectrum(int neff, const double *alpha, const double *beta, double norm,
+                    double e_gs, double weight, int nw, const double *omega,
+                    const double *gamma, double *spec);
+
+/* Density matrix <v|c+_i c_j|v> of a device vector over fock
+ * (src/ed_driver.f90:219-248); denmat is norb*norb complex128 row-major. */
+int edx_denmat(const edx_basis *fock, int num_val_orbs, const void *d_vec, double *denmat);
+
+/* ----------------------------------------------------- drop-in drivers */
+
+/* The four f2py entry points of edrixs.fedrixs (src/pyapi.f90:1-56, 58-118,
+ * 120-182, 184-240): read config.in and the term / fock files in the current
+ * directory, run on the device, write eigvals.dat, denmat.dat, eigvec.N,
+ * xas_poles.N, rixs_poles.N, opavg.dat with the reference formats.  comm,
+ * my_id and num_procs have the reference meaning; ranks other than 0 return
+ * at once (one GPU computes for the communicator). */
+int edx_ed_fsolver(int comm, int my_id, int num_procs);
+int edx_xas_fsolver(int comm, int my_id, int num_procs);
+int edx_rixs_fsolver(int comm, int my_id, int num_procs);
+int edx_opavg_fsolver(int comm, int my_id, int num_procs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDX_H */
