@@ -62,8 +62,11 @@ _SIGS = {
     "edx_csr_free": (None, [_P]),
     "edx_spmv": (_I, [_P, _I, _P, _P]),
     "edx_spmm": (_I, [_P, _I, _I, _P, _P]),
+    "edx_profile": (_I, [_I]),
+    "edx_profile_read": (_I, [_I, ctypes.POINTER(_I64), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "edx_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,
                               ctypes.POINTER(_I)]),
+    "edx_lanczos_refine": (_I, [_P, _I, _I, _I, _D, _I, _P, _I, _P, _P, ctypes.POINTER(_I)]),
     "edx_build_krylov": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
                               ctypes.POINTER(_D)]),
     "edx_minres": (_I, [_P, _I, _P, _P, _D, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),

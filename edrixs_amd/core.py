@@ -255,6 +255,31 @@ def lanczos_eigs(H, nev, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None
     return ev, vecs[:nvec], it.value
 
 
+def lanczos_refine(H, start, nev, maxiter=200, tol=1e-10, nvec=None, precision="c128"):
+    """Lanczos started from the sum of the rows of ``start`` (k x n device tensor)."""
+    T = torch()
+    nvec = nev if nvec is None else nvec
+    st = start.contiguous()
+    ev = np.zeros(nev)
+    vecs = T.zeros((max(nvec, 1), H.n), dtype=T.complex128, device=device())
+    it = ctypes.c_int()
+    _lib.call("edx_lanczos_refine", H._h, _prec(precision), nev, maxiter, float(tol),
+              st.shape[0], _dptr(st), nvec, _ptr(ev), _dptr(vecs), ctypes.byref(it))
+    return ev, vecs[:nvec], it.value
+
+
+def ground_state(H, nev=1, precision="c128", maxiter=500, tol=1e-10, seed=1, deflate=False,
+                 refine_maxiter=200):
+    """Lowest eigenpairs.  With precision 'c64' the Lanczos products use complex64
+    values and the result is finished by an FP64 Lanczos started from the
+    complex64 eigenvectors; returns (evals, evecs, iterations)."""
+    if _prec(precision) == EDX_C128:
+        return lanczos_eigs(H, nev, maxiter, tol, seed, deflate, nev, "c128")
+    ev, vecs, it = lanczos_eigs(H, nev, maxiter, max(tol, 1e-6), seed, deflate, nev, "c64")
+    ev2, vecs2, it2 = lanczos_refine(H, vecs, nev, refine_maxiter, tol, nev, "c128")
+    return ev2, vecs2, it + it2
+
+
 def build_krylov(H, v0, maxn, precision="c128"):
     """alpha_1..neff, beta_1..neff (pole-file convention) and norm = <v0|v0>."""
     _check_vec(v0, H.n)
@@ -302,3 +327,18 @@ def mem_in_use():
     v = ctypes.c_int64()
     _lib.call("edx_mem_in_use", ctypes.byref(v))
     return v.value
+
+
+def profile(enable=True):
+    """Start (clearing) or stop recording SpMV/SpMM launch timings."""
+    _lib.call("edx_profile", int(bool(enable)))
+
+
+def profile_read(precision="c128"):
+    """(launches, kernel milliseconds, algorithmic bytes) recorded for one value type."""
+    n = ctypes.c_int64()
+    ms = ctypes.c_double()
+    by = ctypes.c_double()
+    _lib.call("edx_profile_read", _prec(precision), ctypes.byref(n), ctypes.byref(ms),
+              ctypes.byref(by))
+    return n.value, ms.value, by.value

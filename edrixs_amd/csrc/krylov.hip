@@ -314,9 +314,22 @@ int edx_minres(const edx_csr *a, int precision, const void *d_b, void *d_x, doub
     return EDX_OK;
 }
 
-int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, double tol,
-                     uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
-                     int *niter_out) {
+}  // extern "C"
+
+namespace {
+
+__global__ void k_sum_columns(int64_t n, const double2 *S, int64_t ld, int k, double2 *out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double2 s = make_double2(0, 0);
+        for (int c = 0; c < k; ++c) s = cadd(s, S[(int64_t)c * ld + i]);
+        out[i] = s;
+    }
+}
+
+int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double tol, uint64_t seed,
+                 int deflate, int nvec, double *evals, void *d_evecs, int *niter_out,
+                 const double2 *start, int nstart) {
     EDX_TRY(check_square(h, precision));
     EDX_CHECK(nev >= 1 && maxiter >= 1 && evals && nvec >= 0 && nvec <= nev &&
                   (nvec == 0 || d_evecs),
@@ -343,8 +356,11 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
     int nlock = 0, iters = 0;
     for (int round = 0; round < max_rounds; ++round) {
         double2 *v0 = L.basis + (int64_t)nlock * n;
-        hipLaunchKernelGGL(k_random_real, dim3(nbv), dim3(VB), 0, s, n,
-                           seed + 0x9e37ull * (uint64_t)round, v0);
+        if (round == 0 && start)
+            hipLaunchKernelGGL(k_sum_columns, dim3(nbv), dim3(VB), 0, s, n, start, n, nstart, v0);
+        else
+            hipLaunchKernelGGL(k_random_real, dim3(nbv), dim3(VB), 0, s, n,
+                               seed + 0x9e37ull * (uint64_t)round, v0);
         EDX_LAUNCH_CHECK();
         EDX_TRY(orthogonalize(L, nlock, v0, nullptr));
         EDX_TRY(orthogonalize(L, nlock, v0, nullptr));
@@ -402,6 +418,25 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
     EDX_HIP(hipStreamSynchronize(s));
     if (niter_out) *niter_out = iters;
     return EDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                     uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
+                     int *niter_out) {
+    return lanczos_impl(h, precision, nev, maxiter, tol, seed, deflate, nvec, evals, d_evecs,
+                        niter_out, nullptr, 0);
+}
+
+int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                       int nstart, const void *d_start, int nvec, double *evals, void *d_evecs,
+                       int *niter_out) {
+    EDX_CHECK(d_start && nstart >= 1, EDX_ERR_ARG, "refinement needs start vectors");
+    return lanczos_impl(h, precision, nev, maxiter, tol, 0, 0, nvec, evals, d_evecs, niter_out,
+                        (const double2 *)d_start, nstart);
 }
 
 }  // extern "C"

@@ -200,6 +200,37 @@ int lanes_for(const edx_csr *a) {
 
 namespace edx {
 
+// ---------------------------------------------------------------- profiling
+struct ProfRec {
+    int kind;
+    double bytes;
+};
+static bool g_prof = false;
+static std::vector<hipEvent_t> g_ev;      // pairs: 2k start, 2k+1 stop
+static std::vector<ProfRec> g_rec;
+constexpr size_t PROF_CAP = 16384;
+
+static hipEvent_t prof_start(int kind, double bytes) {
+    if (!g_prof || g_rec.size() >= PROF_CAP) return nullptr;
+    if (g_ev.size() < 2 * PROF_CAP) return nullptr;
+    g_rec.push_back(ProfRec{kind, bytes});
+    hipEvent_t e = g_ev[2 * (g_rec.size() - 1)];
+    (void)hipEventRecord(e, stream());
+    return e;
+}
+
+static void prof_stop(hipEvent_t started) {
+    if (!started) return;
+    (void)hipEventRecord(g_ev[2 * (g_rec.size() - 1) + 1], stream());
+}
+
+double spmv_bytes(const edx_csr *a, int precision, int nrhs, bool z, bool dvec) {
+    const double sv = precision == EDX_C64 ? 8.0 : 16.0, sx = sv;
+    const double vec = (double)a->n * sx + (double)a->m * 16.0 * (1.0 + (z ? 1.0 : 0.0) +
+                                                                  (dvec ? 1.0 : 0.0));
+    return (double)a->nnz * (sv + 4.0) + (double)(a->m + 1) * 8.0 + nrhs * vec;
+}
+
 int spmv_blocks(const edx_csr *a, int G) {
     int64_t groups_needed = a->m;
     int64_t blocks = (groups_needed * G + 255) / 256;
@@ -218,6 +249,7 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     int nb = spmv_blocks(a, G);
     if (nblocks_out) *nblocks_out = nb;
     if (a->m == 0) return EDX_OK;
+    hipEvent_t pe = prof_start(precision, spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
 #define EDX_SPMV_CASE(GG)                                                                     \
     case GG:                                                                                  \
         if (precision == EDX_C128)                                                            \
@@ -237,6 +269,7 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     }
 #undef EDX_SPMV_CASE
     EDX_LAUNCH_CHECK();
+    prof_stop(pe);
     return EDX_OK;
 }
 
@@ -258,6 +291,7 @@ int spmm_launch(const edx_csr *a, int precision, int nrhs, const void *X, double
     int64_t cap = (int64_t)num_cus() * 8;
     if (blocks > cap) blocks = cap;
     dim3 grid((unsigned)blocks, (unsigned)((nrhs + NR - 1) / NR));
+    hipEvent_t pe = prof_start(precision, spmv_bytes(a, precision, nrhs, Z != nullptr, false));
 #define EDX_SPMM_CASE(GG)                                                                       \
     case GG:                                                                                    \
         if (precision == EDX_C128)                                                              \
@@ -277,12 +311,46 @@ int spmm_launch(const edx_csr *a, int precision, int nrhs, const void *X, double
     }
 #undef EDX_SPMM_CASE
     EDX_LAUNCH_CHECK();
+    prof_stop(pe);
     return EDX_OK;
 }
 
 }  // namespace edx
 
 extern "C" {
+
+int edx_profile(int enable) {
+    if (enable) {
+        if (edx::g_ev.size() < 2 * edx::PROF_CAP) {
+            for (size_t i = edx::g_ev.size(); i < 2 * edx::PROF_CAP; ++i) {
+                hipEvent_t e;
+                EDX_HIP(hipEventCreate(&e));
+                edx::g_ev.push_back(e);
+            }
+        }
+        edx::g_rec.clear();
+    }
+    edx::g_prof = enable != 0;
+    return EDX_OK;
+}
+
+int edx_profile_read(int kind, int64_t *launches, double *total_ms, double *total_bytes) {
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    int64_t n = 0;
+    double ms = 0.0, by = 0.0;
+    for (size_t i = 0; i < edx::g_rec.size(); ++i) {
+        if (edx::g_rec[i].kind != kind) continue;
+        float t = 0.f;
+        EDX_HIP(hipEventElapsedTime(&t, edx::g_ev[2 * i], edx::g_ev[2 * i + 1]));
+        ++n;
+        ms += t;
+        by += edx::g_rec[i].bytes;
+    }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    if (total_bytes) *total_bytes = by;
+    return EDX_OK;
+}
 
 int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y) {
     EDX_CHECK(a && d_x && d_y, EDX_ERR_ARG, "null argument");

@@ -110,6 +110,15 @@ int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y);
  * dimensions m and n.  Same precision rules as edx_spmv. */
 int edx_spmm(const edx_csr *a, int precision, int nrhs, const void *d_X, void *d_Y);
 
+/* Live profiling of SpMV/SpMM launches: while enabled, every launch is
+ * bracketed by HIP events on its stream and its algorithmic HBM bytes
+ *   nnz*(s_v + s_i) + (m+1)*s_p + n*s_x + m*s_y*(reads + writes of y-like vectors)
+ * are recorded.  edx_profile(1) clears and starts, edx_profile(0) stops;
+ * edx_profile_read synchronises and returns the launch count, summed kernel
+ * milliseconds and summed bytes (kind 0: complex128 values, 1: complex64). */
+int edx_profile(int enable);
+int edx_profile_read(int kind, int64_t *launches, double *total_ms, double *total_bytes);
+
 /* --------------------------------------------------------------- solvers */
 
 /* Lowest nev eigenpairs of a Hermitian CSR (replaces diag_ham_lanczos,
@@ -120,6 +129,13 @@ int edx_spmm(const edx_csr *a, int precision, int nrhs, const void *d_X, void *d
 int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, double tol,
                      uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
                      int *niter_out);
+/* Same solver started from the normalised sum of nstart given vectors
+ * (d_start, n x nstart column-major) instead of a random vector, without
+ * deflated restarts: the FP64 refinement that finishes a ground state found
+ * with complex64 products (no counterpart in the reference). */
+int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                       int nstart, const void *d_start, int nvec, double *evals, void *d_evecs,
+                       int *niter_out);
 /* Krylov tridiagonalization from an unnormalised start vector (replaces
  * build_krylov_mp, src/lanczos.f90:210-303): alpha[maxn], beta[maxn]
  * (beta_1..beta_neff as written to the pole files), norm = <v|v>. */
