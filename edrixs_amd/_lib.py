@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libedx.so")
 
 EDX_C128 = 0
 EDX_C64 = 1
+EDX_C64D = 2
 
 _STATUS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "I/O error",
            4: "out of device memory", 5: "numerical failure", 6: "capacity exceeded"}
@@ -59,6 +60,11 @@ _SIGS = {
     "edx_csr_download": (_I, [_P, _P, _P, _P]),
     "edx_csr_download_values": (_I, [_P, _I, _P]),
     "edx_csr_add_c64": (_I, [_P]),
+    "edx_csr_prepare": (_I, [_P, _I]),
+    "edx_build_krylov_batch": (_I, [_P, _I, _I, _P, _I, _P, _P, _P, _P]),
+    "edx_build_krylov_reorth": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
+                                     ctypes.POINTER(_D)]),
+    "edx_minres_shifted_batch": (_I, [_P, _I, _I, _P, _P, _P, _D, _I, _P, _P]),
     "edx_csr_free": (None, [_P]),
     "edx_spmv": (_I, [_P, _I, _P, _P]),
     "edx_spmm": (_I, [_P, _I, _I, _P, _P]),

@@ -25,15 +25,16 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EDX_C128, EDX_C64
+from ._lib import EDX_C128, EDX_C64, EDX_C64D
 
 T2 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("re", "<f8"), ("im", "<f8")])
 T4 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("ind3", "<i4"), ("ind4", "<i4"),
                ("re", "<f8"), ("im", "<f8")])
 
 PRECISIONS = {"c128": EDX_C128, "fp64": EDX_C128, "complex128": EDX_C128,
-              "c64": EDX_C64, "mixed": EDX_C64, "complex64": EDX_C64,
-              EDX_C128: EDX_C128, EDX_C64: EDX_C64}
+              "c64": EDX_C64, "complex64": EDX_C64,
+              "c64d": EDX_C64D, "mixed": EDX_C64D,
+              EDX_C128: EDX_C128, EDX_C64: EDX_C64, EDX_C64D: EDX_C64D}
 
 
 def _prec(p):
@@ -166,6 +167,11 @@ class CSRMatrix:
         _lib.call("edx_csr_download_values", self._h, p, _ptr(out))
         return out
 
+    def prepare(self, precision):
+        """Make the value storage a precision needs resident on the device."""
+        _lib.call("edx_csr_prepare", self._h, _prec(precision))
+        return self
+
     def add_c64(self):
         _lib.call("edx_csr_add_c64", self._h)
         self.has_c64 = True
@@ -275,7 +281,7 @@ def ground_state(H, nev=1, precision="c128", maxiter=500, tol=1e-10, seed=1, def
     complex64 eigenvectors; returns (evals, evecs, iterations)."""
     if _prec(precision) == EDX_C128:
         return lanczos_eigs(H, nev, maxiter, tol, seed, deflate, nev, "c128")
-    ev, vecs, it = lanczos_eigs(H, nev, maxiter, max(tol, 1e-6), seed, deflate, nev, "c64")
+    ev, vecs, it = lanczos_eigs(H, nev, maxiter, max(tol, 1e-6), seed, deflate, nev, precision)
     ev2, vecs2, it2 = lanczos_refine(H, vecs, nev, refine_maxiter, tol, nev, "c128")
     return ev2, vecs2, it + it2
 
@@ -291,6 +297,50 @@ def build_krylov(H, v0, maxn, precision="c128"):
               _ptr(alpha), _ptr(beta), ctypes.byref(norm))
     n = neff.value
     return alpha[:n].copy(), beta[:n].copy(), norm.value
+
+
+def build_krylov_reorth(H, v0, maxn, precision="c128"):
+    """Krylov tridiagonalization with full reorthogonalization (exact projection)."""
+    _check_vec(v0, H.n)
+    alpha = np.zeros(maxn)
+    beta = np.zeros(maxn)
+    neff = ctypes.c_int()
+    norm = ctypes.c_double()
+    _lib.call("edx_build_krylov_reorth", H._h, _prec(precision), _dptr(v0), maxn,
+              ctypes.byref(neff), _ptr(alpha), _ptr(beta), ctypes.byref(norm))
+    n = neff.value
+    return alpha[:n].copy(), beta[:n].copy(), norm.value
+
+
+def build_krylov_batch(H, V0, maxn, precision="c128"):
+    """K independent Krylov recurrences in lockstep; V0 is a (K, n) device tensor.
+
+    Returns lists (per RHS) of alpha_1..neff, beta_1..neff and norms."""
+    V0 = V0.contiguous()
+    K = V0.shape[0]
+    assert V0.shape[1] == H.n and V0.dtype == torch().complex128
+    alpha = np.zeros((K, maxn))
+    beta = np.zeros((K, maxn))
+    neff = np.zeros(K, dtype=np.int32)
+    norm = np.zeros(K)
+    _lib.call("edx_build_krylov_batch", H._h, _prec(precision), K, _dptr(V0), maxn, _ptr(neff),
+              _ptr(alpha), _ptr(beta), _ptr(norm))
+    return ([alpha[k, :neff[k]].copy() for k in range(K)],
+            [beta[k, :neff[k]].copy() for k in range(K)], norm)
+
+
+def minres_shifted_batch(H, sigma, B, tol=1e-10, maxit=500, precision="c128"):
+    """Solve (sigma_k - H) x_k = b_k for the rows b_k of B (K, n) in one recurrence."""
+    T = torch()
+    B = B.contiguous()
+    K = B.shape[0]
+    sig = np.ascontiguousarray(np.asarray(sigma, dtype=np.complex128).reshape(K))
+    X = T.zeros_like(B)
+    it = np.zeros(K, dtype=np.int32)
+    res = np.zeros(K)
+    _lib.call("edx_minres_shifted_batch", H._h, _prec(precision), K, _ptr(sig.view(np.float64)),
+              _dptr(B), _dptr(X), float(tol), int(maxit), _ptr(it), _ptr(res))
+    return X, it, res
 
 
 def minres(A, b, tol=1e-10, maxit=500, precision="c128"):

@@ -1,0 +1,935 @@
+// Multi-RHS solvers over one CSR (part of edx_kernels.hip).
+//
+// K right-hand sides are stored interleaved, X[r*K + k], so one pass over the
+// matrix (values, column indices, row pointers) serves all K products and every
+// gathered x line carries K useful values.  The SpMM epilogue is fused and
+// per-RHS:
+//     y_k[r] = c0_k (H x_k)[r] + c1_k z_k[r] + c2_k x_k[r]
+// (the c2 term applies the shift of sigma_k I - H without a shifted matrix),
+// with optional per-RHS dot partials sum_r conj(d_k[r]) y_k[r].
+//
+// Two SpMM schedules:
+//   * K < 8: lanes split a row's nonzeros (G per row, from the mean row
+//     length), up to 4 RHS per pass;
+//   * K >= 8: 16-lane groups own rows and lanes own right-hand sides (lane l
+//     handles RHS l, l+16, ...), so the row's values and column indices are
+//     loaded once for all K and the gathers X[c*K + k] are contiguous.
+// Element-wise kernels use blocks of 64 row-slots x min(K,16) RHS-slots
+// (RHS fastest, so consecutive threads touch consecutive elements); per-RHS
+// reductions run over a fixed row partition, independent of K.
+//
+// On top of it:
+//   * batched Krylov (build_krylov_mp, src/lanczos.f90:210-303, per RHS), used
+//     for the three XAS components and the RIXS final-state spectra;
+//   * batched MINRES for (sigma_k I - H_n) x_k = b_k (pminres_csr,
+//     src/linsys.f90:3-128, per RHS) -- the RIXS intermediate solves for many
+//     incident energies and polarizations in one recurrence.
+//
+// Value formats (edx_precision): C128 exact complex128; C64 complex64 values,
+// complex64 x, FP32 accumulation; C64D complex64 off-diagonal values with the
+// exact complex128 diagonal kept apart, complex128 x and FP64 accumulation.
+
+namespace {
+
+constexpr int BKT = 4;      // right-hand sides per pass of the row-split SpMM
+constexpr int RL_MAXK = 128;  // right-hand sides per pass of the RHS-parallel SpMM
+
+struct BEpi {
+    double2 *y;            // interleaved m x K
+    const double2 *z;      // interleaved or nullptr
+    const double2 *c0;     // per RHS [K] or nullptr (1)
+    const double2 *c1;     // per RHS [K] or nullptr (1)
+    const double2 *c2;     // per RHS [K] or nullptr (no x term)
+    const double2 *dvec;   // interleaved or nullptr
+    double2 *partials;     // [nblocks * K]
+};
+
+template <class VT>
+__device__ __forceinline__ VT ld_val(const VT *p);
+template <>
+__device__ __forceinline__ double2 ld_val<double2>(const double2 *p) {
+    return nt_load2(p);
+}
+template <>
+__device__ __forceinline__ float2 ld_val<float2>(const float2 *p) {
+    return nt_load2(p);
+}
+
+__device__ __forceinline__ double2 epilogue(const BEpi &e, int k, int64_t o, double2 hx,
+                                            const double2 *X64) {
+    double2 out = e.c0 ? cmul(e.c0[k], hx) : hx;
+    if (e.z) out = cadd(out, e.c1 ? cmul(e.c1[k], e.z[o]) : e.z[o]);
+    if (e.c2) out = cadd(out, cmul(e.c2[k], X64[o]));
+    return out;
+}
+
+// Row-split schedule: G lanes per row, KT right-hand sides per pass (blockIdx.y).
+template <int G, int KT, class VT, class XT, class ACC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ cols,
+                                                 const VT *__restrict__ vals,
+                                                 const double2 *__restrict__ diag,
+                                                 const XT *__restrict__ X,
+                                                 const double2 *__restrict__ X64, BEpi e) {
+    const int lig = threadIdx.x & (G - 1);
+    const int64_t group = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+    const int64_t ngroups = (int64_t)gridDim.x * (256 / G);
+    const int k0 = blockIdx.y * KT;
+    double2 dot[KT];
+#pragma unroll
+    for (int q = 0; q < KT; ++q) dot[q] = make_double2(0, 0);
+    for (int64_t r = group; r < m; r += ngroups) {
+        const int64_t b = rowptr[r], end = rowptr[r + 1];
+        ACC sr[KT], si[KT];
+#pragma unroll
+        for (int q = 0; q < KT; ++q) sr[q] = si[q] = (ACC)0;
+        for (int64_t j = b + lig; j < end; j += G) {
+            const VT a = ld_val<VT>(vals + j);
+            const int64_t c = nt_load(cols + j);
+            const XT *xr = X + c * K + k0;
+#pragma unroll
+            for (int q = 0; q < KT; ++q) {
+                if (k0 + q < K) {
+                    const XT xv = xr[q];
+                    sr[q] += (ACC)a.x * (ACC)xv.x - (ACC)a.y * (ACC)xv.y;
+                    si[q] += (ACC)a.x * (ACC)xv.y + (ACC)a.y * (ACC)xv.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KT; ++q) {
+#pragma unroll
+            for (int off = G / 2; off > 0; off >>= 1) {
+                sr[q] += __shfl_xor(sr[q], off, G);
+                si[q] += __shfl_xor(si[q], off, G);
+            }
+        }
+        if (lig == 0) {
+            double2 dg = make_double2(0, 0);
+            if (SPLIT) dg = diag[r];
+#pragma unroll
+            for (int q = 0; q < KT; ++q) {
+                const int k = k0 + q;
+                if (k >= K) break;
+                const int64_t o = r * K + k;
+                double2 hx = make_double2((double)sr[q], (double)si[q]);
+                if (SPLIT) hx = cadd(hx, cmul(dg, X64[o]));
+                const double2 out = epilogue(e, k, o, hx, X64);
+                e.y[o] = out;
+                if (e.dvec) dot[q] = cadd(dot[q], cconjmul(e.dvec[o], out));
+            }
+        }
+    }
+    if (e.dvec) {
+#pragma unroll
+        for (int q = 0; q < KT; ++q) {
+            double2 s = block_sum_256(dot[q]);
+            if (threadIdx.x == 0 && k0 + q < K) e.partials[(int64_t)blockIdx.x * K + k0 + q] = s;
+        }
+    }
+}
+
+// RHS-parallel schedule: 16 groups of 16 lanes per block, one row per group,
+// lane l owns right-hand sides k0 + l + 16 q (q < NPL), k0 = blockIdx.y * 128.
+template <int NPL, class VT, class XT, class ACC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ cols,
+                                                 const VT *__restrict__ vals,
+                                                 const double2 *__restrict__ diag,
+                                                 const XT *__restrict__ X,
+                                                 const double2 *__restrict__ X64, BEpi e) {
+    __shared__ double2 red[16 * 16 * NPL];
+    const int l = threadIdx.x & 15;
+    const int grp = threadIdx.x >> 4;
+    const int k0 = blockIdx.y * RL_MAXK;
+    const int64_t ngroups = (int64_t)gridDim.x * 16;
+    double2 dot[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) dot[q] = make_double2(0, 0);
+    for (int64_t r = (int64_t)blockIdx.x * 16 + grp; r < m; r += ngroups) {
+        const int64_t b = rowptr[r], end = rowptr[r + 1];
+        ACC sr[NPL], si[NPL];
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) sr[q] = si[q] = (ACC)0;
+        int64_t j = b;
+        for (; j + 3 < end; j += 4) {
+            VT a[4];
+            int64_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = ld_val<VT>(vals + j + u);
+                c[u] = nt_load(cols + j + u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                for (int q = 0; q < NPL; ++q) {
+                    const int k = k0 + l + 16 * q;
+                    if (k < K) {
+                        const XT xv = X[c[u] * K + k];
+                        sr[q] += (ACC)a[u].x * (ACC)xv.x - (ACC)a[u].y * (ACC)xv.y;
+                        si[q] += (ACC)a[u].x * (ACC)xv.y + (ACC)a[u].y * (ACC)xv.x;
+                    }
+                }
+            }
+        }
+        for (; j < end; ++j) {
+            const VT a = ld_val<VT>(vals + j);
+            const int64_t c = nt_load(cols + j);
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) {
+                const int k = k0 + l + 16 * q;
+                if (k < K) {
+                    const XT xv = X[c * K + k];
+                    sr[q] += (ACC)a.x * (ACC)xv.x - (ACC)a.y * (ACC)xv.y;
+                    si[q] += (ACC)a.x * (ACC)xv.y + (ACC)a.y * (ACC)xv.x;
+                }
+            }
+        }
+        double2 dg = make_double2(0, 0);
+        if (SPLIT) dg = diag[r];
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+            const int k = k0 + l + 16 * q;
+            if (k < K) {
+                const int64_t o = r * K + k;
+                double2 hx = make_double2((double)sr[q], (double)si[q]);
+                if (SPLIT) hx = cadd(hx, cmul(dg, X64[o]));
+                const double2 out = epilogue(e, k, o, hx, X64);
+                e.y[o] = out;
+                if (e.dvec) dot[q] = cadd(dot[q], cconjmul(e.dvec[o], out));
+            }
+        }
+    }
+    if (e.dvec) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) red[(grp * NPL + q) * 16 + l] = dot[q];
+        __syncthreads();
+        for (int t = threadIdx.x; t < 16 * NPL; t += 256) {
+            const int q = t / 16, ll = t % 16;
+            const int k = k0 + ll + 16 * q;
+            if (k >= K) continue;
+            double2 s = make_double2(0, 0);
+            for (int g = 0; g < 16; ++g) s = cadd(s, red[(g * NPL + q) * 16 + ll]);
+            e.partials[(int64_t)blockIdx.x * K + k] = s;
+        }
+    }
+}
+
+int bspmm_blocks(int64_t m) {
+    // fixed by the row count only
+    int64_t blocks = (m * 16 + 255) / 256;
+    int64_t cap = (int64_t)edx::num_cus() * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    return (int)blocks;
+}
+
+int pick_lanes(const edx_csr *a) {
+    if (a->spmm_lanes) return a->spmm_lanes;
+    double avg = a->m ? (double)a->nnz / (double)a->m : 0.0;
+    if (avg < 6) return 4;
+    if (avg < 20) return 8;
+    if (avg < 80) return 16;
+    return 32;
+}
+
+// ------------------------------------------------ element-wise layout helpers
+//
+// grid (eb, ceil(K/16)), block 64 * KTE threads, KTE = min(K, 16);
+// thread -> (row slot s = tid / KTE, rhs k = blockIdx.y*16 + tid % KTE).
+
+constexpr int ESLOTS = 64;
+
+int elem_grid_x(int64_t n) {
+    int64_t g = (n + ESLOTS - 1) / ESLOTS;
+    int64_t cap = (int64_t)edx::num_cus() * 8;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+struct ELaunch {
+    dim3 grid, block;
+};
+
+ELaunch elaunch(int64_t n, int K) {
+    const int kte = K < 16 ? K : 16;
+    return ELaunch{dim3((unsigned)elem_grid_x(n), (unsigned)((K + 15) / 16)), dim3(ESLOTS * kte)};
+}
+
+#define ELEM_SETUP                                                     \
+    const int kte = blockDim.x / ESLOTS;                               \
+    const int kk = threadIdx.x % kte, slot = threadIdx.x / kte;        \
+    const int k = blockIdx.y * 16 + kk;                                \
+    const bool kv = k < K;                                             \
+    const int64_t rstride = (int64_t)gridDim.x * ESLOTS;               \
+    const int64_t rstart = (int64_t)blockIdx.x * ESLOTS + slot;
+
+// per-(block, k) sum over the 64 slots in slot order -> partials[blockIdx.x*K + k]
+__device__ __forceinline__ void elem_reduce(double2 acc, int K, double2 *partials) {
+    __shared__ double2 red[ESLOTS * 16];
+    const int kte = blockDim.x / ESLOTS;
+    const int kk = threadIdx.x % kte, slot = threadIdx.x / kte;
+    const int k = blockIdx.y * 16 + kk;
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (slot == 0 && k < K) {
+        double2 s = make_double2(0, 0);
+        for (int q = 0; q < ESLOTS; ++q) s = cadd(s, red[q * kte + kk]);
+        partials[(int64_t)blockIdx.x * K + k] = s;
+    }
+}
+
+// partials of sum_r conj(a) b per RHS
+__global__ void k_bdot(int64_t n, int K, const double2 *a, const double2 *b, double2 *partials) {
+    ELEM_SETUP
+    double2 s = make_double2(0, 0);
+    if (kv)
+        for (int64_t r = rstart; r < n; r += rstride) s = cadd(s, cconjmul(a[r * K + k], b[r * K + k]));
+    elem_reduce(s, K, partials);
+}
+
+// out[k] = sum_b partials[b*K + k]  (one block per RHS, fixed order)
+__global__ __launch_bounds__(256) void k_breduce(const double2 *partials, int nb, int K, double2 *out) {
+    const int k = blockIdx.x;
+    __shared__ double sx[256], sy[256];
+    double2 s = make_double2(0, 0);
+    for (int b = threadIdx.x; b < nb; b += 256) s = cadd(s, partials[(int64_t)b * K + k]);
+    sx[threadIdx.x] = s.x;
+    sy[threadIdx.x] = s.y;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sx[threadIdx.x] += sx[threadIdx.x + off];
+            sy[threadIdx.x] += sy[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = make_double2(sx[0], sy[0]);
+}
+
+__global__ void k_il_to_c64(int64_t n, int K, const double2 *in, float2 *out) {
+    ELEM_SETUP
+    if (!kv) return;
+    for (int64_t r = rstart; r < n; r += rstride)
+        out[r * K + k] = make_float2((float)in[r * K + k].x, (float)in[r * K + k].y);
+}
+
+// column-major (K columns of length n) <-> interleaved (n rows of K)
+__global__ void k_cm_to_il(int64_t n, int K, const double2 *cm, double2 *il) {
+    ELEM_SETUP
+    if (!kv) return;
+    for (int64_t r = rstart; r < n; r += rstride) il[r * K + k] = cm[(int64_t)k * n + r];
+}
+
+__global__ void k_il_to_cm(int64_t n, int K, const double2 *il, double2 *cm) {
+    ELEM_SETUP
+    if (!kv) return;
+    for (int64_t r = rstart; r < n; r += rstride) cm[(int64_t)k * n + r] = il[r * K + k];
+}
+
+// a -= b
+__global__ void k_bsub(int64_t n, int K, double2 *a, const double2 *b) {
+    ELEM_SETUP
+    if (!kv) return;
+    for (int64_t r = rstart; r < n; r += rstride) a[r * K + k] = csub(a[r * K + k], b[r * K + k]);
+}
+
+// v_k /= sqrt(norm_k)
+__global__ void k_bdiv_sqrt_norm(int64_t n, int K, double2 *v, const double *norm) {
+    ELEM_SETUP
+    if (!kv) return;
+    const double d = sqrt(norm[k]);
+    if (!(d > 0)) return;
+    for (int64_t r = rstart; r < n; r += rstride) {
+        const double2 t = v[r * K + k];
+        v[r * K + k] = make_double2(t.x / d, t.y / d);
+    }
+}
+
+// ------------------------------------------------------- batched Krylov
+
+struct BKrylov {
+    double *alpha;     // [maxn * K], alpha[(i-1)*K + k]
+    double *beta;      // [(maxn+1) * K]
+    double *norm;      // [K]
+    int *neff;         // [K]
+    int *run;          // [(maxn+1) * K]
+    double2 *c1;       // [K]  -beta_{i-1}
+    double2 *sums;     // [K]
+};
+
+__global__ void k_bk_init(BKrylov s, int K, const double2 *sums) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    s.norm[k] = sums[k].x;
+    s.c1[k] = make_double2(-0.0, -0.0);
+    s.neff[k] = 0;
+    s.run[k] = sums[k].x > 0.0 ? 1 : 0;
+}
+
+__global__ void k_bk_alpha(BKrylov s, int K, int i, const double2 *sums) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (!s.run[(i - 1) * K + k]) return;
+    s.alpha[(int64_t)(i - 1) * K + k] = sums[k].x;
+    s.neff[k] = i;
+}
+
+// y -= alpha_k x ; partial |y|^2
+__global__ void k_baxpy_norm(int64_t n, int K, double2 *y, const double2 *x, const double *alpha_i,
+                             double2 *partials) {
+    ELEM_SETUP
+    double2 s = make_double2(0, 0);
+    if (kv) {
+        const double al = alpha_i[k];
+        for (int64_t r = rstart; r < n; r += rstride) {
+            const int64_t o = r * K + k;
+            const double2 v = make_double2(y[o].x - al * x[o].x, y[o].y - al * x[o].y);
+            y[o] = v;
+            s.x += v.x * v.x + v.y * v.y;
+        }
+    }
+    elem_reduce(s, K, partials);
+}
+
+__global__ void k_bk_beta(BKrylov s, int K, int i, const double2 *sums) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (!s.run[(i - 1) * K + k]) {
+        s.run[i * K + k] = 0;
+        return;
+    }
+    const double b = sqrt(sums[k].x);
+    s.beta[(int64_t)i * K + k] = b;
+    s.c1[k] = make_double2(-b, -0.0);
+    s.run[i * K + k] = fabs(b) < 1e-10 ? 0 : 1;   // src/lanczos.f90:290-293
+}
+
+__global__ void k_bdiv_beta(int64_t n, int K, double2 *v, const double *beta_i, const int *run_i) {
+    ELEM_SETUP
+    if (!kv || !run_i[k]) return;
+    const double d = beta_i[k];
+    for (int64_t r = rstart; r < n; r += rstride) {
+        const double2 t = v[r * K + k];
+        v[r * K + k] = make_double2(t.x / d, t.y / d);
+    }
+}
+
+// ------------------------------------------------------- batched MINRES
+
+struct BMinres {
+    double *beta0;   // [K]
+    double2 *sigma0, *sigma1, *gamma0, *gamma1, *eta, *res;
+    double2 *alpha, *rho2, *rho3, *inv_rho1, *xcoef;
+    int *run;            // [(maxit+1) * K]
+    int *iters;          // [K]
+    double tol;
+    int maxit;
+};
+
+__global__ void k_bm_init(BMinres s, int K, const double2 *sums) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const double b0 = sqrt(sums[k].x);
+    s.beta0[k] = b0;
+    s.sigma0[k] = s.sigma1[k] = make_double2(0, 0);
+    s.gamma0[k] = s.gamma1[k] = make_double2(1, 0);
+    s.eta[k] = make_double2(b0, 0);
+    s.res[k] = make_double2(b0, 0);
+    s.iters[k] = 0;
+    s.run[k] = (b0 > s.tol && 0 < s.maxit) ? 1 : 0;
+}
+
+__global__ void k_bm_divbeta(int64_t n, int K, double2 *v, const double *beta0, const int *run_i) {
+    ELEM_SETUP
+    if (!kv || !run_i[k]) return;
+    const double d = beta0[k];
+    for (int64_t r = rstart; r < n; r += rstride) {
+        const double2 t = v[r * K + k];
+        v[r * K + k] = make_double2(t.x / d, t.y / d);
+    }
+}
+
+__global__ void k_bm_alpha(BMinres s, int K, int i, const double2 *sums) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K || !s.run[i * K + k]) return;
+    s.alpha[k] = sums[k];
+}
+
+// vp = tmp - alpha vc - beta0 vp ; partial |vp|^2   (src/linsys.f90:87)
+__global__ void k_bm_vupdate(int64_t n, int K, const double2 *tmp, const double2 *vc, double2 *vp,
+                             BMinres s, int i, double2 *partials) {
+    ELEM_SETUP
+    double2 acc = make_double2(0, 0);
+    if (kv && s.run[(int64_t)i * K + k]) {
+        const double2 al = s.alpha[k];
+        const double b0 = s.beta0[k];
+        for (int64_t r = rstart; r < n; r += rstride) {
+            const int64_t o = r * K + k;
+            const double2 v = csub(csub(tmp[o], cmul(al, vc[o])), cscale(vp[o], b0));
+            vp[o] = v;
+            acc.x += v.x * v.x + v.y * v.y;
+        }
+    }
+    elem_reduce(acc, K, partials);
+}
+
+__global__ void k_bm_scalars(BMinres s, int K, int i, const double2 *sums) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (!s.run[i * K + k]) {
+        s.run[(i + 1) * K + k] = 0;
+        return;
+    }
+    const double beta1 = sqrt(sums[k].x);
+    const double beta0 = s.beta0[k];
+    const double2 alpha = s.alpha[k];
+    const double2 g0 = s.gamma0[k], g1 = s.gamma1[k], sg0 = s.sigma0[k], sg1 = s.sigma1[k];
+    const double2 delta = csub(cmul(g0, alpha), cscale(cmul(g1, sg0), beta0));
+    const double2 rho1 = csqrt_dev(cadd(cmul(delta, delta), make_double2(beta1 * beta1, 0.0)));
+    const double2 rho2 = cadd(cmul(sg0, alpha), cscale(cmul(g1, g0), beta0));
+    const double2 rho3 = cscale(sg1, beta0);
+    s.gamma1[k] = g0;
+    s.sigma1[k] = sg0;
+    const double2 ng0 = cdiv(delta, rho1), ns0 = cdiv(make_double2(beta1, 0.0), rho1);
+    s.gamma0[k] = ng0;
+    s.sigma0[k] = ns0;
+    s.rho2[k] = rho2;
+    s.rho3[k] = rho3;
+    s.inv_rho1[k] = cdiv(make_double2(1.0, 0.0), rho1);
+    s.xcoef[k] = cmul(ng0, s.eta[k]);
+    const double as0 = hypot(ns0.x, ns0.y);
+    s.res[k] = cscale(s.res[k], as0);
+    const double2 eta = s.eta[k];
+    s.eta[k] = make_double2(-(ns0.x * eta.x - ns0.y * eta.y), -(ns0.x * eta.y + ns0.y * eta.x));
+    s.beta0[k] = beta1;
+    s.iters[k] = i + 1;
+    s.run[(i + 1) * K + k] = (hypot(s.res[k].x, s.res[k].y) > s.tol && i + 1 < s.maxit) ? 1 : 0;
+}
+
+// w_prev = (v_curr - rho3 w_prev - rho2 w_curr) / rho1 ; x += gamma0 eta w_prev
+__global__ void k_bm_wx(int64_t n, int K, const double2 *vc, double2 *wp, const double2 *wc,
+                        double2 *x, BMinres s, int i) {
+    ELEM_SETUP
+    if (!kv || !s.run[(int64_t)i * K + k]) return;
+    const double2 r3 = s.rho3[k], r2 = s.rho2[k], ir1 = s.inv_rho1[k], xc = s.xcoef[k];
+    for (int64_t r = rstart; r < n; r += rstride) {
+        const int64_t o = r * K + k;
+        const double2 w = cmul(csub(csub(vc[o], cmul(r3, wp[o])), cmul(r2, wc[o])), ir1);
+        wp[o] = w;
+        x[o] = cadd(x[o], cmul(xc, w));
+    }
+}
+
+int elem_blocks(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    int64_t cap = (int64_t)edx::num_cus() * 16;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+// the split (diagonal + complex64 off-diagonal) form of a square CSR
+__global__ void k_split_count(int64_t m, const int64_t *rowptr, const int32_t *cols, int64_t *cnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c = 0;
+        for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) c += (cols[j] != r);
+        cnt[r] = c;
+    }
+}
+
+__global__ void k_split_fill(int64_t m, const int64_t *rowptr, const int32_t *cols, const double2 *vals,
+                             const int64_t *orow, int32_t *ocols, float2 *ovals, double2 *diag) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = orow[r];
+        double2 d = make_double2(0, 0);
+        for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            if (cols[j] == r) {
+                d = vals[j];
+            } else {
+                ocols[o] = cols[j];
+                ovals[o] = make_float2((float)vals[j].x, (float)vals[j].y);
+                ++o;
+            }
+        }
+        diag[r] = d;
+    }
+}
+
+}  // namespace
+
+namespace edx {
+
+// Y = epilogue(H X) for K interleaved RHS.  For C64, X32 holds the complex64
+// copy of X (interleaved); X64 is always the complex128 X (split diagonal and
+// shift terms read it).
+int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
+          const BEpi &e, int *nblocks_out) {
+    const int nb = bspmm_blocks(a->m);
+    if (nblocks_out) *nblocks_out = nb;
+    if (a->m == 0 || K == 0) return EDX_OK;
+    hipEvent_t pe = prof_start(precision, spmv_bytes_k(a, precision, K, e.z != nullptr,
+                                                       e.dvec != nullptr));
+    if (K >= 8) {
+        const int kp = K < RL_MAXK ? K : RL_MAXK;
+        const int npl = kp <= 16 ? 1 : (kp <= 32 ? 2 : (kp <= 64 ? 4 : 8));
+        dim3 grid((unsigned)nb, (unsigned)((K + RL_MAXK - 1) / RL_MAXK));
+#define EDX_RL(NP)                                                                                 \
+    case NP:                                                                                       \
+        if (precision == EDX_C128)                                                                 \
+            hipLaunchKernelGGL((k_spmm_rl<NP, double2, double2, double, false>), grid, dim3(256), 0, \
+                               stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals, nullptr, X64, \
+                               X64, e);                                                             \
+        else if (precision == EDX_C64)                                                             \
+            hipLaunchKernelGGL((k_spmm_rl<NP, float2, float2, float, false>), grid, dim3(256), 0,   \
+                               stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals32, nullptr,    \
+                               X32, X64, e);                                                        \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_spmm_rl<NP, float2, double2, double, true>), grid, dim3(256), 0,  \
+                               stream(), a->m, K, a->d_off_rowptr, a->d_off_cols, a->d_off_vals32, \
+                               a->d_diag, X64, X64, e);                                            \
+        break;
+        switch (npl) {
+            EDX_RL(1)
+            EDX_RL(2)
+            EDX_RL(4)
+            EDX_RL(8)
+            default: return set_error(EDX_ERR_ARG, "bad rhs count %d", K);
+        }
+#undef EDX_RL
+    } else {
+        const int G = pick_lanes(a);
+        dim3 grid((unsigned)nb, (unsigned)((K + BKT - 1) / BKT));
+#define EDX_BSPMM(GG)                                                                              \
+    case GG:                                                                                       \
+        if (precision == EDX_C128)                                                                 \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, double2, double2, double, false>), grid, dim3(256), \
+                               0, stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals, nullptr,   \
+                               X64, X64, e);                                                        \
+        else if (precision == EDX_C64)                                                             \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, float2, float2, float, false>), grid, dim3(256),  \
+                               0, stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals32, nullptr, \
+                               X32, X64, e);                                                        \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, float2, double2, double, true>), grid, dim3(256), \
+                               0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols, a->d_off_vals32, \
+                               a->d_diag, X64, X64, e);                                            \
+        break;
+        switch (G) {
+            EDX_BSPMM(4)
+            EDX_BSPMM(8)
+            EDX_BSPMM(16)
+            EDX_BSPMM(32)
+            default: return set_error(EDX_ERR_ARG, "unsupported lanes %d", G);
+        }
+#undef EDX_BSPMM
+    }
+    EDX_LAUNCH_CHECK();
+    prof_stop(pe);
+    return EDX_OK;
+}
+
+int bspmm_single(const edx_csr *a, const double2 *x, const SpmvEpi &se, int *nblocks_out) {
+    BEpi e{};
+    e.y = se.y;
+    e.z = se.z;
+    e.c0 = se.coef;
+    e.c1 = se.coef ? se.coef + 1 : nullptr;
+    e.dvec = se.dvec;
+    e.partials = se.partials;
+    return bspmm(a, EDX_C64D, 1, x, nullptr, e, nblocks_out);
+}
+
+int csr_add_split(edx_csr *a) {
+    if (a->d_diag) return EDX_OK;
+    EDX_CHECK(a->m == a->n, EDX_ERR_ARG, "split form needs a square matrix");
+    const int64_t m = a->m;
+    int64_t *orow = (int64_t *)dmalloc((m + 1) * sizeof(int64_t), "split rowptr");
+    double2 *diag = (double2 *)dmalloc((m ? m : 1) * sizeof(double2), "split diagonal");
+    if (!orow || !diag) {
+        dfree(orow);
+        dfree(diag);
+        return EDX_ERR_NOMEM;
+    }
+    const int g = elem_blocks(m);
+    EDX_HIP(hipMemsetAsync(orow + m, 0, sizeof(int64_t), stream()));
+    hipLaunchKernelGGL(k_split_count, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols, orow);
+    EDX_LAUNCH_CHECK();
+    EDX_TRY(exclusive_scan(orow, m + 1));
+    int64_t onnz = 0;
+    EDX_HIP(hipMemcpy(&onnz, orow + m, sizeof(int64_t), hipMemcpyDeviceToHost));
+    int32_t *oc = (int32_t *)dmalloc((onnz ? onnz : 1) * sizeof(int32_t), "split columns");
+    float2 *ov = (float2 *)dmalloc((onnz ? onnz : 1) * sizeof(float2), "split values");
+    if (!oc || !ov) {
+        dfree(orow);
+        dfree(diag);
+        dfree(oc);
+        dfree(ov);
+        return EDX_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_split_fill, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
+                       a->d_vals, orow, oc, ov, diag);
+    EDX_LAUNCH_CHECK();
+    EDX_HIP(hipStreamSynchronize(stream()));
+    a->d_off_rowptr = orow;
+    a->d_off_cols = oc;
+    a->d_off_vals32 = ov;
+    a->d_diag = diag;
+    a->off_nnz = onnz;
+    return EDX_OK;
+}
+
+int ensure_precision(edx_csr *a, int precision) {
+    if (precision == EDX_C128) return EDX_OK;
+    if (precision == EDX_C64) return edx_csr_add_c64(a);
+    if (precision == EDX_C64D) return csr_add_split(a);
+    return set_error(EDX_ERR_ARG, "unknown precision %d", precision);
+}
+
+}  // namespace edx
+
+namespace {
+
+struct Dbuf {
+    void *p = nullptr;
+    ~Dbuf() { edx::dfree(p); }
+    int alloc(size_t bytes, const char *what) {
+        p = edx::dmalloc(bytes ? bytes : 16, what);
+        return p ? EDX_OK : EDX_ERR_NOMEM;
+    }
+    template <class T>
+    T *as() const {
+        return (T *)p;
+    }
+};
+
+#define ELAUNCH(kernel, n, K, ...)                                                        \
+    do {                                                                                  \
+        const ELaunch el_ = elaunch((n), (K));                                            \
+        hipLaunchKernelGGL(kernel, el_.grid, el_.block, 0, edx::stream(), __VA_ARGS__); \
+    } while (0)
+
+int bkrylov_run(const edx_csr *h, int precision, int K, const double2 *V0cm, int maxn, int *neff,
+                double *alpha, double *beta, double *norm) {
+    const int64_t n = h->n;
+    hipStream_t s = edx::stream();
+    Dbuf vb, x32, part, sc, flags;
+    EDX_TRY(vb.alloc((size_t)2 * n * K * sizeof(double2), "batched krylov vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc((size_t)n * K * sizeof(float2), "batched x32"));
+    const int nbs = bspmm_blocks(n), nbe = elem_grid_x(n);
+    const int nbp = std::max(nbs, nbe);
+    EDX_TRY(part.alloc((size_t)nbp * K * sizeof(double2) + 64, "partials"));
+    const size_t nsc = (size_t)maxn * K + (size_t)(maxn + 1) * K + K;   // alpha, beta, norm
+    EDX_TRY(sc.alloc(nsc * sizeof(double) + (size_t)2 * K * sizeof(double2), "batched scalars"));
+    EDX_TRY(flags.alloc(((size_t)(maxn + 1) * K + K) * sizeof(int), "batched flags"));
+    BKrylov st;
+    st.alpha = sc.as<double>();
+    st.beta = st.alpha + (size_t)maxn * K;
+    st.norm = st.beta + (size_t)(maxn + 1) * K;
+    st.c1 = (double2 *)(st.norm + K);
+    st.sums = st.c1 + K;
+    st.run = flags.as<int>();
+    st.neff = st.run + (size_t)(maxn + 1) * K;
+    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + (size_t)n * K};
+    EDX_HIP(hipMemsetAsync(vb.p, 0, (size_t)2 * n * K * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(sc.p, 0, nsc * sizeof(double), s));
+    EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(maxn + 1) * K + K) * sizeof(int), s));
+    const int kb = (K + 63) / 64;
+    ELAUNCH(k_cm_to_il, n, K, n, K, V0cm, v[0]);
+    ELAUNCH(k_bdot, n, K, n, K, v[0], v[0], part.as<double2>());
+    hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, st.sums);
+    hipLaunchKernelGGL(k_bk_init, dim3(kb), dim3(64), 0, s, st, K, st.sums);
+    ELAUNCH(k_bdiv_sqrt_norm, n, K, n, K, v[0], st.norm);
+    EDX_LAUNCH_CHECK();
+    int curr = 0, prev = 1, checked = 0;
+    for (int i = 1; i <= maxn; ++i) {
+        if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[curr], x32.as<float2>());
+        BEpi e{};
+        e.y = v[prev];
+        e.z = v[prev];
+        e.c1 = st.c1;
+        e.dvec = v[curr];
+        e.partials = part.as<double2>();
+        int nb = 0;
+        EDX_TRY(edx::bspmm(h, precision, K, v[curr], x32.as<float2>(), e, &nb));
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nb, K, st.sums);
+        hipLaunchKernelGGL(k_bk_alpha, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
+        ELAUNCH(k_baxpy_norm, n, K, n, K, v[prev], v[curr], st.alpha + (size_t)(i - 1) * K,
+                part.as<double2>());
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, st.sums);
+        hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
+        ELAUNCH(k_bdiv_beta, n, K, n, K, v[prev], st.beta + (size_t)i * K, st.run + (size_t)i * K);
+        EDX_LAUNCH_CHECK();
+        std::swap(curr, prev);
+        if (i - checked >= 32 && i < maxn) {
+            std::vector<int> r(K);
+            EDX_HIP(hipMemcpy(r.data(), st.run + (size_t)i * K, K * sizeof(int), hipMemcpyDeviceToHost));
+            checked = i;
+            bool any = false;
+            for (int k = 0; k < K; ++k) any |= r[k] != 0;
+            if (!any) break;
+        }
+    }
+    std::vector<double> ha((size_t)maxn * K), hb((size_t)(maxn + 1) * K), hn(K);
+    std::vector<int> hneff(K);
+    EDX_HIP(hipMemcpy(ha.data(), st.alpha, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(hipMemcpy(hb.data(), st.beta, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(hipMemcpy(hn.data(), st.norm, K * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(hipMemcpy(hneff.data(), st.neff, K * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k) {
+        neff[k] = hneff[k];
+        norm[k] = hn[k];
+        for (int i = 0; i < maxn; ++i) {
+            alpha[(size_t)k * maxn + i] = ha[(size_t)i * K + k];
+            beta[(size_t)k * maxn + i] = hb[(size_t)(i + 1) * K + k];
+        }
+    }
+    return EDX_OK;
+}
+
+int bminres_run(const edx_csr *h, int precision, int K, const double2 *sigma, const double2 *Bcm,
+                double2 *Xcm, double tol, int maxit, int *iters, double *final_res) {
+    const int64_t n = h->n;
+    hipStream_t s = edx::stream();
+    Dbuf vb, x32, part, sc, flags;
+    const size_t nk = (size_t)n * K;
+    EDX_TRY(vb.alloc(6 * nk * sizeof(double2), "batched minres vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(nk * sizeof(float2), "batched x32"));
+    const int nbs = bspmm_blocks(n), nbe = elem_grid_x(n);
+    EDX_TRY(part.alloc((size_t)std::max(nbs, nbe) * K * sizeof(double2) + 64, "partials"));
+    const int nvec_scalars = 14;
+    EDX_TRY(sc.alloc((size_t)K * (nvec_scalars * sizeof(double2) + sizeof(double)), "minres scalars"));
+    EDX_TRY(flags.alloc(((size_t)(maxit + 2) * K + K) * sizeof(int), "minres flags"));
+    double2 *S2 = sc.as<double2>();
+    BMinres st;
+    st.sigma0 = S2;
+    st.sigma1 = S2 + K;
+    st.gamma0 = S2 + 2 * K;
+    st.gamma1 = S2 + 3 * K;
+    st.eta = S2 + 4 * K;
+    st.res = S2 + 5 * K;
+    st.alpha = S2 + 6 * K;
+    st.rho2 = S2 + 7 * K;
+    st.rho3 = S2 + 8 * K;
+    st.inv_rho1 = S2 + 9 * K;
+    st.xcoef = S2 + 10 * K;
+    double2 *sums = S2 + 11 * K;
+    double2 *neg1 = S2 + 12 * K;     // -1 for every RHS (c0 of sigma - H)
+    double2 *sig = S2 + 13 * K;      // shifts
+    st.beta0 = (double *)(S2 + 14 * K);
+    st.run = flags.as<int>();
+    st.iters = st.run + (size_t)(maxit + 2) * K;
+    st.tol = tol;
+    st.maxit = maxit;
+    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + nk};
+    double2 *w[2] = {vb.as<double2>() + 2 * nk, vb.as<double2>() + 3 * nk};
+    double2 *tmp = vb.as<double2>() + 4 * nk, *x = vb.as<double2>() + 5 * nk, *b = tmp;
+    EDX_HIP(hipMemsetAsync(vb.p, 0, 6 * nk * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(maxit + 2) * K + K) * sizeof(int), s));
+    {
+        std::vector<double2> hneg(K, make_double2(-1.0, 0.0));
+        EDX_HIP(hipMemcpy(neg1, hneg.data(), K * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(hipMemcpy(sig, sigma, K * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    const int kb = (K + 63) / 64;
+    ELAUNCH(k_cm_to_il, n, K, n, K, Bcm, v[0]);
+    ELAUNCH(k_bdot, n, K, n, K, v[0], v[0], part.as<double2>());
+    hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, sums);
+    hipLaunchKernelGGL(k_bm_init, dim3(kb), dim3(64), 0, s, st, K, sums);
+    EDX_LAUNCH_CHECK();
+    int curr = 0, prev = 1, checked = 0;
+    for (int i = 0; i < maxit; ++i) {
+        const int *run_i = st.run + (size_t)i * K;
+        ELAUNCH(k_bm_divbeta, n, K, n, K, v[curr], st.beta0, run_i);
+        if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[curr], x32.as<float2>());
+        BEpi e{};   // tmp = sigma_k v - H v ; partial <v|tmp>
+        e.y = tmp;
+        e.c0 = neg1;
+        e.c2 = sig;
+        e.dvec = v[curr];
+        e.partials = part.as<double2>();
+        int nb = 0;
+        EDX_TRY(edx::bspmm(h, precision, K, v[curr], x32.as<float2>(), e, &nb));
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nb, K, sums);
+        hipLaunchKernelGGL(k_bm_alpha, dim3(kb), dim3(64), 0, s, st, K, i, sums);
+        ELAUNCH(k_bm_vupdate, n, K, n, K, tmp, v[curr], v[prev], st, i, part.as<double2>());
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, sums);
+        hipLaunchKernelGGL(k_bm_scalars, dim3(kb), dim3(64), 0, s, st, K, i, sums);
+        ELAUNCH(k_bm_wx, n, K, n, K, v[curr], w[prev], w[curr], x, st, i);
+        EDX_LAUNCH_CHECK();
+        std::swap(curr, prev);
+        if (i + 1 - checked >= 16) {
+            std::vector<int> r(K);
+            EDX_HIP(hipMemcpy(r.data(), st.run + (size_t)(i + 1) * K, K * sizeof(int),
+                              hipMemcpyDeviceToHost));
+            checked = i + 1;
+            bool any = false;
+            for (int k = 0; k < K; ++k) any |= r[k] != 0;
+            if (!any) break;
+        }
+    }
+    // true residuals |(sigma_k - H) x_k - b_k|
+    ELAUNCH(k_cm_to_il, n, K, n, K, Bcm, b);
+    if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, x, x32.as<float2>());
+    BEpi e{};
+    e.y = v[0];
+    e.c0 = neg1;
+    e.c2 = sig;
+    EDX_TRY(edx::bspmm(h, precision, K, x, x32.as<float2>(), e, nullptr));
+    ELAUNCH(k_bsub, n, K, n, K, v[0], b);
+    ELAUNCH(k_bdot, n, K, n, K, v[0], v[0], part.as<double2>());
+    hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, sums);
+    ELAUNCH(k_il_to_cm, n, K, n, K, x, Xcm);
+    EDX_LAUNCH_CHECK();
+    std::vector<double2> hs(K);
+    EDX_HIP(hipMemcpy(hs.data(), sums, K * sizeof(double2), hipMemcpyDeviceToHost));
+    std::vector<int> hit(K);
+    EDX_HIP(hipMemcpy(hit.data(), st.iters, K * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k) {
+        if (iters) iters[k] = hit[k];
+        if (final_res) final_res[k] = sqrt(fabs(hs[k].x));
+    }
+    return EDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_csr_prepare(edx_csr *a, int precision) {
+    EDX_CHECK(a, EDX_ERR_ARG, "null matrix");
+    return edx::ensure_precision(a, precision);
+}
+
+int edx_build_krylov_batch(const edx_csr *h, int precision, int nrhs, const void *d_V0, int maxn,
+                           int *neff, double *alpha, double *beta, double *norm) {
+    EDX_CHECK(h && d_V0 && neff && alpha && beta && norm && nrhs >= 1 && maxn >= 1, EDX_ERR_ARG,
+              "bad argument");
+    EDX_CHECK(nrhs <= 4096, EDX_ERR_ARG, "at most 4096 right-hand sides per call");
+    EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix must be square");
+    EDX_TRY(edx::ensure_precision(const_cast<edx_csr *>(h), precision));
+    int rc = bkrylov_run(h, precision, nrhs, (const double2 *)d_V0, maxn, neff, alpha, beta, norm);
+    if (rc) return rc;
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    return EDX_OK;
+}
+
+int edx_minres_shifted_batch(const edx_csr *h, int precision, int nrhs, const double *sigma,
+                             const void *d_B, void *d_X, double tol, int maxit, int *iters,
+                             double *final_res) {
+    EDX_CHECK(h && sigma && d_B && d_X && nrhs >= 1 && maxit >= 0, EDX_ERR_ARG, "bad argument");
+    EDX_CHECK(nrhs <= 4096, EDX_ERR_ARG, "at most 4096 right-hand sides per call");
+    EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix must be square");
+    EDX_TRY(edx::ensure_precision(const_cast<edx_csr *>(h), precision));
+    int rc = bminres_run(h, precision, nrhs, (const double2 *)sigma, (const double2 *)d_B,
+                         (double2 *)d_X, tol, maxit, iters, final_res);
+    if (rc) return rc;
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    return EDX_OK;
+}
+
+}  // extern "C"

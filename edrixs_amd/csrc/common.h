@@ -66,6 +66,13 @@ struct edx_csr {
     float2 *d_vals32;
     int spmv_lanes;      // lanes per row chosen for SpMV (0 = not tuned)
     int spmv_lanes32;
+    int spmm_lanes;
+    // split form (EDX_C64D): exact diagonal + complex64 off-diagonal CSR
+    double2 *d_diag;
+    int64_t *d_off_rowptr;
+    int32_t *d_off_cols;
+    float2 *d_off_vals32;
+    int64_t off_nnz;
 };
 
 // Lookup descriptor for a column space basis.

@@ -399,7 +399,13 @@ int build_csr(BuildParams p, int64_t ncols, int precision, edx_csr **out) {
         edx::dfree(vals);
         return rc;
     }
-    edx_csr *a = new edx_csr{p.M, ncols, nnz, rowptr, cols, vals, nullptr, 0, 0};
+    edx_csr *a = new edx_csr{};
+    a->m = p.M;
+    a->n = ncols;
+    a->nnz = nnz;
+    a->d_rowptr = rowptr;
+    a->d_cols = cols;
+    a->d_vals = vals;
     if (precision == EDX_C64) {
         rc = edx_csr_add_c64(a);
         if (rc) {
@@ -567,6 +573,10 @@ void edx_csr_free(edx_csr *a) {
     edx::dfree(a->d_cols);
     edx::dfree(a->d_vals);
     edx::dfree(a->d_vals32);
+    edx::dfree(a->d_diag);
+    edx::dfree(a->d_off_rowptr);
+    edx::dfree(a->d_off_cols);
+    edx::dfree(a->d_off_vals32);
     delete a;
 }
 

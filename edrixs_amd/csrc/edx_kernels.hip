@@ -13,4 +13,5 @@
 #include "spmv.hip"
 #include "blas.hip"
 #include "krylov.hip"
+#include "batch.hip"
 #include "observables.hip"

@@ -39,9 +39,7 @@ int check_square(const edx_csr *h, int precision) {
     EDX_CHECK(h, EDX_ERR_ARG, "null matrix");
     EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix is %lld x %lld, not square", (long long)h->m,
               (long long)h->n);
-    EDX_CHECK(precision == EDX_C128 || precision == EDX_C64, EDX_ERR_ARG, "bad precision");
-    if (precision == EDX_C64 && !h->d_vals32) EDX_TRY(edx_csr_add_c64(const_cast<edx_csr *>(h)));
-    return EDX_OK;
+    return edx_csr_prepare(const_cast<edx_csr *>(h), precision);
 }
 
 // y = c0*H x + c1*z with optional dot partials; converts x for the C64 path.
@@ -236,6 +234,52 @@ int orthogonalize(Lanczos &L, int k, double2 *w, std::vector<double2> *hk) {
 
 // One Lanczos run started from basis column `first` (already normalised and
 // orthogonal to the locked columns [0, first)).  Returns Ritz pairs of the run.
+// One Lanczos step from basis column first+j: w = H v_j - alpha v_j - beta v_{j-1},
+// two classical Gram-Schmidt passes against columns [0, first+j], alpha
+// corrected by the v_j component; appends alpha_j and beta_{j+1} = |w|.
+int lanczos_one(Lanczos &L, int first, int j, std::vector<double> &al, std::vector<double> &be) {
+    hipStream_t s = edx::stream();
+    double2 *vj = L.basis + (int64_t)(first + j) * L.ld;
+    SpmvEpi e{};
+    e.y = L.w;
+    EDX_TRY(apply(L.h, L.precision, vj, L.x32, e, nullptr, nullptr));
+    double2 a;
+    EDX_TRY(host_dot(L, vj, L.w, &a));
+    double alpha = a.x;
+    const double2 *vprev = j > 0 ? L.basis + (int64_t)(first + j - 1) * L.ld : nullptr;
+    hipLaunchKernelGGL(k_axpby_sub, dim3(L.nbv), dim3(VB), 0, s, L.n, L.w, vj, alpha, vprev,
+                       j > 0 ? be[j - 1] : 0.0);
+    EDX_LAUNCH_CHECK();
+    std::vector<double2> h1;
+    EDX_TRY(orthogonalize(L, first + j + 1, L.w, &h1));
+    EDX_TRY(orthogonalize(L, first + j + 1, L.w, nullptr));
+    alpha += h1[first + j].x;
+    double2 nb2;
+    EDX_TRY(host_dot(L, L.w, L.w, &nb2));
+    al.push_back(alpha);
+    be.push_back(sqrt(fabs(nb2.x)));
+    return EDX_OK;
+}
+
+// maxn fully reorthogonalized steps from column first (already normalised);
+// stops early when beta < 1e-10 as src/lanczos.f90:290-293 does.
+int lanczos_steps(Lanczos &L, int first, int maxn, std::vector<double> &al,
+                  std::vector<double> &be, int &iters) {
+    const int maxm = std::min(maxn, L.cap_cols - first - 1);
+    for (int j = 0; j < maxm; ++j) {
+        EDX_TRY(lanczos_one(L, first, j, al, be));
+        ++iters;
+        const double beta = be.back();
+        if (beta < 1e-10) break;
+        if (j + 1 < maxm) {
+            hipLaunchKernelGGL(k_copy_div, dim3(L.nbv), dim3(VB), 0, edx::stream(), L.n, L.w, beta,
+                               L.basis + (int64_t)(first + j + 1) * L.ld);
+            EDX_LAUNCH_CHECK();
+        }
+    }
+    return EDX_OK;
+}
+
 int lanczos_run(Lanczos &L, int first, int nev, int maxiter, double tol, std::vector<double> &ritz,
                 std::vector<double> &S, int &m_out, int &iters) {
     std::vector<double> al, be;   // be[j] couples v_j and v_{j+1}
@@ -243,26 +287,8 @@ int lanczos_run(Lanczos &L, int first, int nev, int maxiter, double tol, std::ve
     int m = 0;
     hipStream_t s = edx::stream();
     for (int j = 0; j < maxm; ++j) {
-        double2 *vj = L.basis + (int64_t)(first + j) * L.ld;
-        SpmvEpi e{};
-        e.y = L.w;
-        EDX_TRY(apply(L.h, L.precision, vj, L.x32, e, nullptr, nullptr));
-        double2 a;
-        EDX_TRY(host_dot(L, vj, L.w, &a));
-        double alpha = a.x;
-        const double2 *vprev = j > 0 ? L.basis + (int64_t)(first + j - 1) * L.ld : nullptr;
-        hipLaunchKernelGGL(k_axpby_sub, dim3(L.nbv), dim3(VB), 0, s, L.n, L.w, vj, alpha, vprev,
-                           j > 0 ? be[j - 1] : 0.0);
-        EDX_LAUNCH_CHECK();
-        std::vector<double2> h1;
-        EDX_TRY(orthogonalize(L, first + j + 1, L.w, &h1));
-        EDX_TRY(orthogonalize(L, first + j + 1, L.w, nullptr));
-        alpha += h1[first + j].x;
-        double2 nb2;
-        EDX_TRY(host_dot(L, L.w, L.w, &nb2));
-        const double beta = sqrt(fabs(nb2.x));
-        al.push_back(alpha);
-        be.push_back(beta);
+        EDX_TRY(lanczos_one(L, first, j, al, be));
+        const double beta = be.back();
         m = j + 1;
         ++iters;
         bool last = (beta < 1e-10) || (m == maxm);
@@ -429,6 +455,41 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
                      int *niter_out) {
     return lanczos_impl(h, precision, nev, maxiter, tol, seed, deflate, nvec, evals, d_evecs,
                         niter_out, nullptr, 0);
+}
+
+int edx_build_krylov_reorth(const edx_csr *h, int precision, const void *d_v0, int maxn,
+                            int *neff, double *alpha, double *beta, double *norm) {
+    EDX_TRY(check_square(h, precision));
+    EDX_CHECK(d_v0 && neff && alpha && beta && norm && maxn >= 1, EDX_ERR_ARG, "bad argument");
+    const int64_t n = h->n;
+    if (maxn > n) maxn = (int)n;
+    const int cap_cols = maxn + 2;
+    DevBuf basis, x32, w, part, hv;
+    EDX_TRY(basis.alloc((size_t)n * cap_cols * sizeof(double2), "krylov basis"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "krylov x32"));
+    EDX_TRY(w.alloc(n * sizeof(double2), "krylov w"));
+    const int nbv = vec_blocks(n);
+    EDX_TRY(part.alloc(((size_t)cap_cols * 64 + nbv + 64) * sizeof(double2), "partials"));
+    EDX_TRY(hv.alloc((size_t)cap_cols * sizeof(double2), "coefficients"));
+    Lanczos L{h, precision, n, basis.as<double2>(), n, cap_cols, x32.as<float2>(), w.as<double2>(),
+              part.as<double2>(), hv.as<double2>(), nbv};
+    hipStream_t s = edx::stream();
+    EDX_HIP(hipMemcpyAsync(L.basis, d_v0, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    double2 nn;
+    EDX_TRY(host_dot(L, L.basis, L.basis, &nn));
+    *norm = nn.x;
+    EDX_CHECK(nn.x > 0, EDX_ERR_NUMERIC, "zero start vector");
+    hipLaunchKernelGGL(k_scale_real, dim3(nbv), dim3(VB), 0, s, n, L.basis, 1.0 / sqrt(nn.x));
+    std::vector<double> al, be;
+    int iters = 0;
+    EDX_TRY(lanczos_steps(L, 0, maxn, al, be, iters));
+    *neff = (int)al.size();
+    for (int i = 0; i < *neff; ++i) {
+        alpha[i] = al[i];
+        beta[i] = be[i];
+    }
+    EDX_HIP(hipStreamSynchronize(s));
+    return EDX_OK;
 }
 
 int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, double tol,

@@ -224,11 +224,20 @@ static void prof_stop(hipEvent_t started) {
     (void)hipEventRecord(g_ev[2 * (g_rec.size() - 1) + 1], stream());
 }
 
+// Algorithmic HBM bytes of one product with K right-hand sides: every stored
+// value, column index and row pointer once, x once, y written once, plus the
+// addend z and the dot vector d when the epilogue reads them.
+double spmv_bytes_k(const edx_csr *a, int precision, int K, bool z, bool dvec) {
+    const double yv = (double)a->m * 16.0 * (1.0 + (z ? 1.0 : 0.0) + (dvec ? 1.0 : 0.0));
+    if (precision == EDX_C64D)
+        return (double)a->off_nnz * 12.0 + (double)(a->m + 1) * 8.0 + (double)a->m * 16.0 +
+               K * ((double)a->n * 16.0 + yv);
+    const double sv = precision == EDX_C64 ? 8.0 : 16.0;
+    return (double)a->nnz * (sv + 4.0) + (double)(a->m + 1) * 8.0 + K * ((double)a->n * sv + yv);
+}
+
 double spmv_bytes(const edx_csr *a, int precision, int nrhs, bool z, bool dvec) {
-    const double sv = precision == EDX_C64 ? 8.0 : 16.0, sx = sv;
-    const double vec = (double)a->n * sx + (double)a->m * 16.0 * (1.0 + (z ? 1.0 : 0.0) +
-                                                                  (dvec ? 1.0 : 0.0));
-    return (double)a->nnz * (sv + 4.0) + (double)(a->m + 1) * 8.0 + nrhs * vec;
+    return spmv_bytes_k(a, precision, nrhs, z, dvec);
 }
 
 int spmv_blocks(const edx_csr *a, int G) {
@@ -242,8 +251,11 @@ int spmv_blocks(const edx_csr *a, int G) {
 
 // Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
 // complex64 (already converted) for C64.  Returns the block count used.
+int bspmm_single(const edx_csr *a, const double2 *x, const SpmvEpi &e, int *nblocks_out);
+
 int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e, int lanes,
                 int *nblocks_out) {
+    if (precision == EDX_C64D) return bspmm_single(a, (const double2 *)x, e, nblocks_out);
     int G = lanes ? lanes : (precision == EDX_C64 ? (a->spmv_lanes32 ? a->spmv_lanes32 : lanes_for(a))
                                                   : (a->spmv_lanes ? a->spmv_lanes : lanes_for(a)));
     int nb = spmv_blocks(a, G);
@@ -354,8 +366,9 @@ int edx_profile_read(int kind, int64_t *launches, double *total_ms, double *tota
 
 int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y) {
     EDX_CHECK(a && d_x && d_y, EDX_ERR_ARG, "null argument");
-    EDX_CHECK(precision == EDX_C128 || (precision == EDX_C64 && a->d_vals32), EDX_ERR_ARG,
-              "precision %d not available on this matrix", precision);
+    EDX_CHECK(precision == EDX_C128 || (precision == EDX_C64 && a->d_vals32) ||
+                  (precision == EDX_C64D && a->d_diag),
+              EDX_ERR_ARG, "precision %d not prepared on this matrix", precision);
     SpmvEpi e{};
     e.y = (double2 *)d_y;
     e.z = (const double2 *)d_y;   // y += A x

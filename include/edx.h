@@ -33,8 +33,12 @@ enum edx_status {
     EDX_ERR_CAPACITY = 6    /* a row produced more candidates than supported */
 };
 
-/* value storage of a CSR matrix */
-enum edx_precision { EDX_C128 = 0, EDX_C64 = 1 };
+/* value storage / arithmetic of products with a CSR matrix:
+ *   EDX_C128  exact complex128 values, complex128 vectors, FP64 accumulation;
+ *   EDX_C64   complex64 values and vectors, FP32 accumulation;
+ *   EDX_C64D  complex64 off-diagonal values, exact complex128 diagonal kept
+ *             apart, complex128 vectors, FP64 accumulation. */
+enum edx_precision { EDX_C128 = 0, EDX_C64 = 1, EDX_C64D = 2 };
 
 /* Term records, layout-compatible with T_tensor2 / T_tensor4
  * (src/m_types.f90:7-20): 1-based spin-orbital indices, complex value. */
@@ -98,6 +102,8 @@ int edx_csr_download(const edx_csr *a, int64_t *rowptr, int32_t *cols, void *val
 int edx_csr_download_values(const edx_csr *a, int precision, void *vals);
 /* Add a complex64 value copy (rounded from complex128) for the mixed path. */
 int edx_csr_add_c64(edx_csr *a);
+/* Make the storage a precision needs resident (EDX_C64 copy, EDX_C64D split). */
+int edx_csr_prepare(edx_csr *a, int precision);
 void edx_csr_free(edx_csr *a);
 
 /* ------------------------------------------------------------------ SpMV */
@@ -141,11 +147,34 @@ int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, do
  * (beta_1..beta_neff as written to the pole files), norm = <v|v>. */
 int edx_build_krylov(const edx_csr *h, int precision, const void *d_v0, int maxn, int *neff,
                      double *alpha, double *beta, double *norm);
+/* The same tridiagonalization with every new vector reorthogonalized against
+ * all previous ones (two classical Gram-Schmidt passes): the exact Krylov
+ * projection, against which the rounding of the plain recurrence is measured
+ * (no counterpart in the reference; keeps maxn+1 vectors of length n). */
+int edx_build_krylov_reorth(const edx_csr *h, int precision, const void *d_v0, int maxn,
+                            int *neff, double *alpha, double *beta, double *norm);
 /* MINRES for A x = b (replaces pminres_csr, src/linsys.f90:3-128); stops when
  * the recurrence residual estimate <= tol or after maxit steps.  d_x receives
  * x; info[0] = iterations, final_res[0] = ||A x - b||. */
 int edx_minres(const edx_csr *a, int precision, const void *d_b, void *d_x, double tol,
                int maxit, int *iters, double *final_res);
+
+/* Batched Krylov: nrhs independent build_krylov_mp recurrences in lockstep
+ * over one matrix (interleaved multi-RHS products).  d_V0 is n x nrhs
+ * column-major; alpha and beta are nrhs x maxn row-major (beta_1..beta_neff per
+ * RHS as in the pole files), neff[nrhs], norm[nrhs].  Each RHS gets the same
+ * numbers it would get alone. */
+int edx_build_krylov_batch(const edx_csr *h, int precision, int nrhs, const void *d_V0, int maxn,
+                           int *neff, double *alpha, double *beta, double *norm);
+/* Batched MINRES for (sigma_k I - H) x_k = b_k, k < nrhs, with the recurrence
+ * of pminres_csr (src/linsys.f90:3-128) per RHS; sigma is nrhs complex
+ * (re, im) pairs, d_B and d_X are n x nrhs column-major.  Per RHS: iters[k],
+ * final_res[k] = |(sigma_k - H) x_k - b_k|.  This replaces the reference's
+ * one-fsolver-call-per-(omega_in, polarization) loop with one shifted solve of
+ * the unshifted H_n (edrixs/solvers.py:2651-2693, src/rixs_driver.f90:130-150). */
+int edx_minres_shifted_batch(const edx_csr *h, int precision, int nrhs, const double *sigma,
+                             const void *d_B, void *d_X, double tol, int maxit, int *iters,
+                             double *final_res);
 
 /* Continued-fraction spectra on the device (edrixs/plot_spectrum.py:9-61):
  * spec[w] += weight * (-1/pi) Im[norm / (z - a_1 - b_1^2/(z - a_2 - ...))],

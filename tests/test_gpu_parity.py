@@ -394,3 +394,102 @@ def test_missing_input_raises(core, tmp_path):
             fedrixs.ed_fsolver(0, 0, 1)
     finally:
         os.chdir(cwd)
+
+
+# ------------------------------------------------------ batched forms
+
+def test_krylov_batch_matches_single_and_is_order_independent(core):
+    import torch
+    m, fn, bn, A, R = _siam_hn(core)
+    rng = np.random.default_rng(12)
+    V0 = rng.normal(size=(3, A.n)) + 1j * rng.normal(size=(3, A.n))
+    Vd = _to_dev(core, V0)
+    al, be, nr = core.build_krylov_batch(A, Vd, 80)
+    for k in range(3):
+        a1, b1, n1 = core.build_krylov(A, Vd[k].contiguous(), 80)
+        ra, rb, rn = o.build_krylov(R, V0[k], 80)
+        np.testing.assert_allclose(al[k][:20], ra[:20], rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(al[k][:20], a1[:20], rtol=1e-11, atol=1e-11)
+        assert abs(nr[k] - rn) <= 1e-12 * rn
+        # alone (batch of one) vs in the batch: identical numbers
+        ak, bk, nk = core.build_krylov_batch(A, Vd[k:k + 1].contiguous(), 80)
+        assert np.array_equal(ak[0], al[k]) and np.array_equal(bk[0], be[k]) and nk[0] == nr[k]
+    # reversed order gives the same per-RHS numbers
+    alr, ber, nrr = core.build_krylov_batch(A, torch.flip(Vd, [0]).contiguous(), 80)
+    for k in range(3):
+        assert np.array_equal(alr[2 - k], al[k]) and np.array_equal(ber[2 - k], be[k])
+
+
+def test_minres_shifted_batch_matches_oracle(core):
+    m = H.small_siam(nbath=2, seed=4)
+    fn = H.fock_words(m.norb, m.nocc + 1)
+    bn = core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    rng = np.random.default_rng(9)
+    sig = np.array([-20.0 + 0.5j, -15.0 + 0.5j, -5.0 + 0.3j, 10.0 + 0.5j])
+    B = rng.normal(size=(4, Hn.n)) + 1j * rng.normal(size=(4, Hn.n))
+    X, it, res = core.minres_shifted_batch(Hn, sig, _to_dev(core, B), tol=1e-10, maxit=3000)
+    Xd = X.cpu().numpy()
+    for k in range(4):
+        S = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n, omega=sig[k], ham_sign=-1.0)
+        rx, rit, rres = o.pminres(S, B[k], 1e-10, 3000)
+        assert abs(int(it[k]) - rit) <= 3
+        np.testing.assert_allclose(Xd[k], rx, rtol=0, atol=1e-7 * np.abs(rx).max())
+        assert res[k] < 1e-8 * np.linalg.norm(B[k])
+
+
+def test_split_c64d_product_error_is_offdiagonal_rounding_only(core):
+    m, fn, bn, A, R = _siam_hn(core)
+    A.prepare("c64d")
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    y = A.matvec(_to_dev(core, x), precision="c64d").cpu().numpy()
+    # reference: exact diagonal + complex64-rounded off-diagonal values, FP64 arithmetic
+    D = R.todense()
+    dg = np.diag(np.diag(D))
+    off = (D - dg).astype(np.complex64).astype(np.complex128)
+    ref = (dg + off) @ x
+    np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    exact = D @ x
+    assert np.abs(y - exact).max() < 1e-6 * np.abs(exact).max()
+
+
+def test_krylov_reorth_is_exact_projection(core):
+    # with full reorthogonalization the fraction from dim steps equals the pole sum
+    m = H.small_siam(nbath=2, seed=8)
+    fi = H.fock_words(m.norb, m.nocc)
+    bi = core.Basis.from_words(fi, m.norb)
+    A = core.build_ham_i(bi, m.hop_i, m.coul_i)
+    Hd = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    phi = np.random.default_rng(4).normal(size=A.n) + 0j
+    a, b, nrm = core.build_krylov_reorth(A, _to_dev(core, phi), 400)
+    assert len(a) <= A.n
+    e, v = np.linalg.eigh(Hd)
+    w = np.abs(v.conj().T @ phi) ** 2
+    om = np.linspace(-30, 30, 601)
+    z = om + 0.3j
+    exact = -1 / np.pi * np.imag((w[None, :] / (z[:, None] - e[None, :])).sum(1))
+    np.testing.assert_allclose(o.cf_spectrum(a, b, nrm, 0.0, om, 0.3), exact, rtol=1e-9, atol=1e-12)
+
+
+def test_rhs_parallel_batches_match_single(core):
+    # K >= 8 runs the RHS-parallel SpMM schedule
+    m, fn, bn, A, R = _siam_hn(core)
+    rng = np.random.default_rng(21)
+    K = 10
+    V0 = rng.normal(size=(K, A.n)) + 1j * rng.normal(size=(K, A.n))
+    al, be, nr = core.build_krylov_batch(A, _to_dev(core, V0), 60)
+    for k in (0, 5, 9):
+        ra, rb, rn = o.build_krylov(R, V0[k], 60)
+        np.testing.assert_allclose(al[k][:20], ra[:20], rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(be[k][:20], rb[:20], rtol=1e-9, atol=1e-9)
+    for prec in ("c128", "c64d"):
+        sig = -12.0 + 0.5j + np.arange(9) * 1.5
+        B = rng.normal(size=(9, A.n)) + 0j
+        X, it, res = core.minres_shifted_batch(A, sig, _to_dev(core, B), tol=1e-9, maxit=3000,
+                                               precision=prec)
+        Xd = X.cpu().numpy()
+        Hd = R.todense()
+        for k in range(9):
+            r = sig[k] * Xd[k] - Hd @ Xd[k] - B[k]
+            assert np.linalg.norm(r) < 1e-7 * np.linalg.norm(B[k]), (prec, k)
