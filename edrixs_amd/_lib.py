@@ -65,6 +65,7 @@ _SIGS = {
     "edx_build_krylov_reorth": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
                                      ctypes.POINTER(_D)]),
     "edx_minres_shifted_batch": (_I, [_P, _I, _I, _P, _P, _P, _D, _I, _P, _P]),
+    "edx_shifted_fom_project": (_I, [_P, _I, _P, _I, _P, _I, _PP, _PP, _D, _I, _P, _P]),
     "edx_csr_free": (None, [_P]),
     "edx_spmv": (_I, [_P, _I, _P, _P]),
     "edx_spmm": (_I, [_P, _I, _I, _P, _P]),

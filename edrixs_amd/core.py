@@ -343,6 +343,29 @@ def minres_shifted_batch(H, sigma, B, tol=1e-10, maxit=500, precision="c128"):
     return X, it, res
 
 
+def shifted_fom(H, b, sigma, projections=(None,), tol=1e-8, maxit=2000, precision="c128"):
+    """Solve (sigma_k - H) x_k = b for every shift from one Lanczos recurrence.
+
+    ``projections`` lists CSR operators P (``None`` = identity); returns
+    ``(outs, iters, res)`` where ``outs[p]`` is a (K, rows(P)) complex128 device
+    tensor holding P x_k, and ``res`` the relative residual of each iterate."""
+    T = torch()
+    _check_vec(b, H.n)
+    sig = np.ascontiguousarray(np.atleast_1d(np.asarray(sigma, dtype=np.complex128)))
+    K = len(sig)
+    projs = list(projections)
+    outs = [T.zeros((K, P.m if P is not None else H.n), dtype=T.complex128, device=b.device)
+            for P in projs]
+    hp = (ctypes.c_void_p * len(projs))(*[P._h if P is not None else None for P in projs])
+    op = (ctypes.c_void_p * len(projs))(*[o.data_ptr() for o in outs])
+    it = np.zeros(K, dtype=np.int32)
+    res = np.zeros(K)
+    _lib.call("edx_shifted_fom_project", H._h, _prec(precision), _dptr(b), K,
+              _ptr(sig.view(np.float64)), len(projs), hp, op, float(tol), int(maxit),
+              _ptr(it), _ptr(res))
+    return outs, it, res
+
+
 def minres(A, b, tol=1e-10, maxit=500, precision="c128"):
     T = torch()
     _check_vec(b, A.n)

@@ -552,7 +552,7 @@ __global__ void k_split_fill(int64_t m, const int64_t *rowptr, const int32_t *co
                 d = vals[j];
             } else {
                 ocols[o] = cols[j];
-                ovals[o] = make_float2((float)vals[j].x, (float)vals[j].y);
+                ovals[o] = herm_round(rowptr, cols, vals, r, j);
                 ++o;
             }
         }
@@ -631,17 +631,6 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
     EDX_LAUNCH_CHECK();
     prof_stop(pe);
     return EDX_OK;
-}
-
-int bspmm_single(const edx_csr *a, const double2 *x, const SpmvEpi &se, int *nblocks_out) {
-    BEpi e{};
-    e.y = se.y;
-    e.z = se.z;
-    e.c0 = se.coef;
-    e.c1 = se.coef ? se.coef + 1 : nullptr;
-    e.dvec = se.dvec;
-    e.partials = se.partials;
-    return bspmm(a, EDX_C64D, 1, x, nullptr, e, nblocks_out);
 }
 
 int csr_add_split(edx_csr *a) {

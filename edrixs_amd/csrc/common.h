@@ -66,6 +66,7 @@ struct edx_csr {
     float2 *d_vals32;
     int spmv_lanes;      // lanes per row chosen for SpMV (0 = not tuned)
     int spmv_lanes32;
+    int spmv_lanes64d;
     int spmm_lanes;
     // split form (EDX_C64D): exact diagonal + complex64 off-diagonal CSR
     double2 *d_diag;

@@ -256,6 +256,52 @@ int exclusive_scan(int64_t *d_a, int64_t n) {
     return EDX_OK;
 }
 
+// Hermitian-consistent complex64 rounding of a square CSR: an off-diagonal
+// entry whose transposed partner equals its conjugate up to FP64 rounding is
+// stored as round((a_rc + conj(a_cr)) / 2).  The partner rounds the conjugate of
+// the same FP64 sum, so the stored matrix keeps a_rc == conj(a_cr) exactly (the
+// FP64 sums of the two entries come from different term orders and can differ
+// in the last bit, which independent rounding would turn into a 1-ulp float
+// asymmetry and a non-Hermitian operator).  A diagonal entry whose imaginary
+// part is rounding residue (<= 1e-13 of the real part) is stored real.  Other
+// entries round independently.
+__device__ __forceinline__ float2 herm_round(const int64_t *rowptr, const int32_t *cols,
+                                             const double2 *vals, int64_t r, int64_t j) {
+    const double2 a = vals[j];
+    const int64_t c = cols[j];
+    if (c == r) {
+        const float re = __double2float_rn(a.x);
+        return make_float2(re, fabs(a.y) <= 1e-13 * fabs(a.x) ? 0.0f : __double2float_rn(a.y));
+    } else {
+        int64_t lo = rowptr[c], hi = rowptr[c + 1] - 1;
+        while (lo <= hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            const int64_t cm = cols[mid];
+            if (cm < r) lo = mid + 1;
+            else if (cm > r) hi = mid - 1;
+            else {
+                const double2 b = vals[mid];
+                const double dx = a.x - b.x, dy = a.y + b.y;
+                const double sc = fabs(a.x) + fabs(a.y) + fabs(b.x) + fabs(b.y);
+                if (fabs(dx) + fabs(dy) <= 1e-13 * sc) {
+                    const double2 s = make_double2((a.x + b.x) * 0.5, (a.y - b.y) * 0.5);
+                    return make_float2(__double2float_rn(s.x), __double2float_rn(s.y));
+                }
+                break;
+            }
+        }
+    }
+    return make_float2(__double2float_rn(a.x), __double2float_rn(a.y));
+}
+
+__global__ void k_to_c64_herm(int64_t m, const int64_t *rowptr, const int32_t *cols,
+                              const double2 *vals, float2 *out) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j)
+            out[j] = herm_round(rowptr, cols, vals, r, j);
+}
+
 __global__ void k_to_c64(int64_t n, const double2 *in, float2 *out) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -558,8 +604,15 @@ int edx_csr_add_c64(edx_csr *a) {
     if (a->nnz) {
         int64_t g = (a->nnz + 255) / 256;
         if (g > (int64_t)edx::num_cus() * 32) g = (int64_t)edx::num_cus() * 32;
-        hipLaunchKernelGGL(k_to_c64, dim3((unsigned)g), dim3(256), 0, edx::stream(), a->nnz,
-                           a->d_vals, v);
+        if (a->m == a->n) {
+            int64_t gr = (a->m + 255) / 256;
+            if (gr > (int64_t)edx::num_cus() * 32) gr = (int64_t)edx::num_cus() * 32;
+            hipLaunchKernelGGL(k_to_c64_herm, dim3((unsigned)gr), dim3(256), 0, edx::stream(),
+                               a->m, a->d_rowptr, a->d_cols, a->d_vals, v);
+        } else {
+            hipLaunchKernelGGL(k_to_c64, dim3((unsigned)g), dim3(256), 0, edx::stream(), a->nnz,
+                               a->d_vals, v);
+        }
         EDX_LAUNCH_CHECK();
         EDX_HIP(hipStreamSynchronize(edx::stream()));
     }

@@ -2,6 +2,7 @@
 // constant-memory binomial table and the shared kernels need no relocatable
 // device code.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -14,4 +15,5 @@
 #include "blas.hip"
 #include "krylov.hip"
 #include "batch.hip"
+#include "multishift.hip"
 #include "observables.hip"

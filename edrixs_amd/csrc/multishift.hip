@@ -1,0 +1,312 @@
+// Multi-shift Galerkin (FOM) solver for the RIXS intermediate states (part of
+// edx_kernels.hip).
+//
+//   edx_shifted_fom_project  <- the per-incident-energy solve of the RIXS driver
+//                               (src/rixs_driver.f90:127-150: build omega - H_n,
+//                               pminres_csr) and the emission operator applied to
+//                               its solution (src/rixs_driver.f90:163-185)
+//
+// The Krylov space K_m(H, b) is the same for every shift sigma, so one Lanczos
+// recurrence on H (the build_krylov_mp recurrence, src/lanczos.f90:210-303)
+// serves every incident energy at once.  With H V_m = V_m T_m + beta_m v_{m+1}
+// e_m^T the Galerkin iterate of (sigma - H) x = b is
+//     x_m = V_m y_m,   (sigma - T_m) y_m = |b| e_1,
+// whose residual is exactly beta_m |e_m^T y_m| times v_{m+1}.  Writing
+// sigma - T_m = L_m U_m (L unit lower bidiagonal, U upper bidiagonal with
+// pivots eta_j and superdiagonal -beta_j) gives the short recurrences
+//     eta_1 = sigma - alpha_1,  eta_j = sigma - alpha_j - beta_{j-1}^2 / eta_{j-1}
+//     zeta_1 = |b|,             zeta_j = beta_{j-1} zeta_{j-1} / eta_{j-1}
+//     p_j = (v_j + beta_{j-1} p_{j-1}) / eta_j,   x_j = x_{j-1} + zeta_j p_j
+//     |r_j| = beta_j |zeta_j / eta_j|
+// (the direct Lanczos form, Saad, Iterative Methods for Sparse Linear Systems,
+// 2nd ed., sec. 6.7.1).  Im eta_j has the sign of Im sigma and
+// |Im eta_j| >= |Im sigma|, so a shift with a nonzero imaginary part never
+// meets a zero pivot.  Only projections P x_j are kept: q_j = P p_j obeys the
+// same recurrence with P v_j in place of v_j, so each shift carries two vectors
+// of P's row space (for the emission operator, the final-state space) and the
+// n-space solution is never stored.
+
+namespace {
+
+constexpr int MS_MAXK = 256;   // shifts per call (one block holds their scalars)
+
+struct MsShift {
+    double2 sigma;
+    double2 eta;       // pivot eta_i of the current step
+    double2 inv_eta;   // 1 / eta_i
+    double2 zeta;      // zeta_i
+    double res;        // residual estimate of the last finished iterate (absolute)
+    int active;        // still iterating
+    int iters;         // iterate the stored projection belongs to
+};
+
+struct MsCtl {
+    double tol_abs;
+    int nactive;
+};
+
+__device__ __forceinline__ double2 crecip(double2 a) {
+    const double d = a.x * a.x + a.y * a.y;
+    return make_double2(a.x / d, -a.y / d);
+}
+
+__device__ __forceinline__ double cabs2(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+
+// Step i (1-based) of the shift recurrences, after alpha_i is known.  A shift
+// whose iterate i-1 already meets the tolerance is frozen there.
+__global__ __launch_bounds__(256) void k_ms_scalars(int i, const double *alpha, const double *beta,
+                                                    const KrylovDev *S, MsShift *sh, int K,
+                                                    MsCtl *ctl, const int *run) {
+    if (!run[i - 1]) return;
+    __shared__ int cnt[256];
+    int c = 0;
+    const double tol = ctl->tol_abs;
+    for (int k = threadIdx.x; k < K; k += 256) {
+        MsShift s = sh[k];
+        const double a = alpha[i - 1];
+        if (i == 1) {
+            s.eta = make_double2(s.sigma.x - a, s.sigma.y);
+            s.zeta = make_double2(S->sqrt_norm, 0.0);
+            s.active = 1;
+            s.iters = 0;
+            s.res = S->sqrt_norm;
+        } else if (s.active) {
+            const double bp = beta[i - 1];
+            const double2 ratio = cmul(s.zeta, s.inv_eta);   // zeta_{i-1} / eta_{i-1}
+            s.res = bp * cabs2(ratio);
+            if (s.res <= tol) {
+                s.active = 0;
+            } else {
+                s.zeta = cscale(ratio, bp);
+                const double2 t = cscale(s.inv_eta, bp * bp);
+                s.eta = make_double2(s.sigma.x - a - t.x, s.sigma.y - t.y);
+            }
+        }
+        if (s.active) {
+            s.inv_eta = crecip(s.eta);
+            s.iters = i;
+            ++c;
+        }
+        sh[k] = s;
+    }
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) cnt[threadIdx.x] += cnt[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctl->nactive = cnt[0];
+}
+
+// q_k = (P v + beta_{i-1} q_k) / eta_k and out_k += zeta_k q_k for the active
+// shifts; P is a CSR with FP64 values (nullptr rowptr: the identity).
+// Q and out are mP x K column-major.
+__global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *prow,
+                                                   const int32_t *pcol, const double2 *pval,
+                                                   const double2 *v, const double *beta_prev,
+                                                   const MsShift *sh, int K, double2 *Q,
+                                                   double2 *out, const int *run, const MsCtl *ctl) {
+    if (!run[0] || ctl->nactive == 0) return;
+    __shared__ double2 s_ie[MS_MAXK], s_z[MS_MAXK];
+    __shared__ int s_act[MS_MAXK];
+    for (int k = threadIdx.x; k < K; k += 256) {
+        s_ie[k] = sh[k].inv_eta;
+        s_z[k] = sh[k].zeta;
+        s_act[k] = sh[k].active;
+    }
+    __syncthreads();
+    const double bp = *beta_prev;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        double2 w;
+        if (prow) {
+            w = make_double2(0, 0);
+            for (int64_t j = prow[r]; j < prow[r + 1]; ++j) w = cadd(w, cmul(pval[j], v[pcol[j]]));
+        } else {
+            w = v[r];
+        }
+        for (int k = 0; k < K; ++k) {
+            if (!s_act[k]) continue;
+            const int64_t o = (int64_t)k * mP + r;
+            const double2 qo = Q[o];
+            const double2 q = cmul(make_double2(w.x + bp * qo.x, w.y + bp * qo.y), s_ie[k]);
+            Q[o] = q;
+            out[o] = cadd(out[o], cmul(s_z[k], q));
+        }
+    }
+}
+
+// beta_i as in build_krylov_mp; the recurrence also stops once no shift is active
+__global__ __launch_bounds__(VB) void k_ms_beta(const double2 *partials, int nb, KrylovDev *st,
+                                                double *beta, int i, int *run, const MsCtl *ctl) {
+    if (!run[i - 1]) {
+        if (threadIdx.x == 0) run[i] = 0;
+        return;
+    }
+    double2 s = reduce_partials(partials, nb);
+    if (threadIdx.x == 0) {
+        const double b = sqrt(s.x);
+        beta[i] = b;
+        st->coef[1] = make_double2(-b, -0.0);
+        run[i] = (fabs(b) < 1e-10 || ctl->nactive == 0) ? 0 : 1;
+    }
+}
+
+// shifts still active when the recurrence ends keep iterate neff
+__global__ void k_ms_finish(const KrylovDev *S, const double *beta, MsShift *sh, int K) {
+    const int m = S->neff;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        MsShift s = sh[k];
+        if (s.active && m > 0) {
+            s.res = beta[m] * cabs2(cmul(s.zeta, s.inv_eta));
+            s.iters = m;
+        }
+        sh[k] = s;
+    }
+}
+
+int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, const double *sigma,
+                    int nproj, const edx_csr *const *proj, double2 *const *out, double tol,
+                    int maxit, int *iters, double *final_res) {
+    const int64_t n = h->n;
+    hipStream_t s = edx::stream();
+    for (int p = 0; p < nproj; ++p) {
+        const int64_t mp = proj[p] ? proj[p]->m : n;
+        if (mp && K) EDX_HIP(hipMemsetAsync(out[p], 0, (size_t)mp * K * sizeof(double2), s));
+    }
+    DevBuf vb, x32, part, st, ab, runf, shb, ctlb, qb;
+    EDX_TRY(vb.alloc(2 * (size_t)n * sizeof(double2), "fom lanczos vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc((size_t)n * sizeof(float2), "fom x32"));
+    const int nbs = edx::spmv_blocks(h, 64);
+    const int nbv = vec_blocks(n);
+    EDX_TRY(part.alloc((size_t)std::max(nbs, nbv) * 4 * sizeof(double2) + 64, "partials"));
+    EDX_TRY(st.alloc(sizeof(KrylovDev), "fom krylov state"));
+    EDX_TRY(ab.alloc((2 * (size_t)maxit + 2) * sizeof(double), "alpha beta"));
+    EDX_TRY(runf.alloc(((size_t)maxit + 2) * sizeof(int), "run flags"));
+    EDX_TRY(shb.alloc((size_t)K * sizeof(MsShift), "shift state"));
+    EDX_TRY(ctlb.alloc(sizeof(MsCtl), "shift control"));
+    size_t qtot = 0;
+    std::vector<size_t> qoff(nproj);
+    for (int p = 0; p < nproj; ++p) {
+        qoff[p] = qtot;
+        qtot += (size_t)(proj[p] ? proj[p]->m : n) * K;
+    }
+    EDX_TRY(qb.alloc((qtot ? qtot : 1) * sizeof(double2), "projected directions"));
+    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + n};
+    double *d_alpha = ab.as<double>(), *d_beta = ab.as<double>() + maxit + 1;
+    int *run = runf.as<int>();
+    KrylovDev *S = st.as<KrylovDev>();
+    MsShift *sh = shb.as<MsShift>();
+    MsCtl *ctl = ctlb.as<MsCtl>();
+
+    std::vector<MsShift> hs(K);
+    for (int k = 0; k < K; ++k) {
+        hs[k] = MsShift{};
+        hs[k].sigma = make_double2(sigma[2 * k], sigma[2 * k + 1]);
+    }
+    EDX_HIP(hipMemcpyAsync(sh, hs.data(), K * sizeof(MsShift), hipMemcpyHostToDevice, s));
+    EDX_HIP(hipMemsetAsync(qb.p, 0, (qtot ? qtot : 1) * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(vb.p, 0, 2 * (size_t)n * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(ab.p, 0, (2 * (size_t)maxit + 2) * sizeof(double), s));
+    EDX_HIP(hipMemsetAsync(run, 0, ((size_t)maxit + 2) * sizeof(int), s));
+    EDX_HIP(hipMemcpyAsync(v[0], b, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_krylov_init, dim3(1), dim3(1), 0, s, S, run);
+    hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, v[0], v[0], part.as<double2>(), nullptr);
+    hipLaunchKernelGGL(k_krylov_norm, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S);
+    EDX_LAUNCH_CHECK();
+    KrylovDev hk;
+    EDX_HIP(hipMemcpy(&hk, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
+    if (!(hk.sqrt_norm > 0.0)) {   // b = 0: every solution is 0
+        for (int k = 0; k < K; ++k) {
+            iters[k] = 0;
+            final_res[k] = 0.0;
+        }
+        return EDX_OK;
+    }
+    MsCtl hc{tol * hk.sqrt_norm, K};
+    EDX_HIP(hipMemcpyAsync(ctl, &hc, sizeof(MsCtl), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[0], &S->sqrt_norm, nullptr);
+    EDX_LAUNCH_CHECK();
+
+    int curr = 0, prev = 1, checked = 0;
+    for (int i = 1; i <= maxit; ++i) {
+        SpmvEpi e{};
+        e.y = v[prev];
+        e.z = v[prev];
+        e.coef = S->coef;
+        e.dvec = v[curr];
+        e.partials = part.as<double2>();
+        e.run = run + (i - 1);
+        int nb = 0;
+        EDX_TRY(apply(h, precision, v[curr], x32.as<float2>(), e, &nb, run + (i - 1)));
+        hipLaunchKernelGGL(k_krylov_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
+                           d_alpha, i, run);
+        hipLaunchKernelGGL(k_ms_scalars, dim3(1), dim3(256), 0, s, i, d_alpha, d_beta, S, sh, K,
+                           ctl, run);
+        for (int p = 0; p < nproj; ++p) {
+            const edx_csr *P = proj[p];
+            const int64_t mp = P ? P->m : n;
+            hipLaunchKernelGGL(k_ms_update, dim3(vec_blocks(mp)), dim3(256), 0, s, mp,
+                               P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
+                               P ? P->d_vals : nullptr, v[curr], d_beta + (i - 1), sh, K,
+                               qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl);
+        }
+        hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
+                           d_alpha + (i - 1), part.as<double2>(), run + (i - 1));
+        hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S, d_beta,
+                           i, run, ctl);
+        hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[prev], d_beta + i, run + i);
+        EDX_LAUNCH_CHECK();
+        std::swap(curr, prev);
+        if (i - checked >= 16 && i < maxit) {
+            int r = 1;
+            EDX_HIP(hipMemcpy(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
+            checked = i;
+            if (!r) break;
+        }
+    }
+    hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(256), 0, s, S, d_beta, sh, K);
+    EDX_LAUNCH_CHECK();
+    EDX_HIP(hipMemcpy(hs.data(), sh, K * sizeof(MsShift), hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k) {
+        iters[k] = hs[k].iters;
+        final_res[k] = hs[k].res / hk.sqrt_norm;
+    }
+    return EDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_shifted_fom_project(const edx_csr *h, int precision, const void *d_b, int nshift,
+                            const double *sigma, int nproj, const edx_csr *const *proj,
+                            void *const *d_out, double tol, int maxit, int *iters,
+                            double *final_res) {
+    EDX_TRY(check_square(h, precision));
+    EDX_CHECK(h->n > 0, EDX_ERR_ARG, "empty matrix");
+    EDX_CHECK(nshift >= 1 && nshift <= MS_MAXK, EDX_ERR_ARG, "nshift %d not in [1, %d]", nshift,
+              MS_MAXK);
+    EDX_CHECK(nproj >= 1 && proj && d_out, EDX_ERR_ARG, "need at least one projection");
+    EDX_CHECK(d_b && sigma && iters && final_res, EDX_ERR_ARG, "null argument");
+    EDX_CHECK(maxit >= 1, EDX_ERR_ARG, "maxit %d < 1", maxit);
+    EDX_CHECK(tol >= 0.0, EDX_ERR_ARG, "negative tolerance");
+    for (int k = 0; k < nshift; ++k)
+        EDX_CHECK(sigma[2 * k + 1] != 0.0 && std::isfinite(sigma[2 * k]) &&
+                      std::isfinite(sigma[2 * k + 1]),
+                  EDX_ERR_ARG, "shift %d = (%g, %g): needs a finite nonzero imaginary part", k,
+                  sigma[2 * k], sigma[2 * k + 1]);
+    for (int p = 0; p < nproj; ++p) {
+        EDX_CHECK(d_out[p], EDX_ERR_ARG, "null output %d", p);
+        if (proj[p])
+            EDX_CHECK(proj[p]->n == h->n, EDX_ERR_ARG,
+                      "projection %d has %lld columns, matrix has %lld rows", p,
+                      (long long)proj[p]->n, (long long)h->n);
+    }
+    std::vector<double2 *> outs(nproj);
+    for (int p = 0; p < nproj; ++p) outs[p] = (double2 *)d_out[p];
+    return shifted_fom_run(h, precision, (const double2 *)d_b, nshift, sigma, nproj, proj,
+                           outs.data(), tol, maxit, iters, final_res);
+}
+
+}  // extern "C"

@@ -70,13 +70,17 @@ __device__ __forceinline__ double2 block_sum_256(double2 v) {
     return r;
 }
 
-template <int G, class VT, class XT>
+// SPLIT: the split form (complex64 off-diagonal values in the CSR arrays, the
+// exact FP64 diagonal in diag, complex128 x, FP64 accumulation); the leader
+// adds diag[r] * x[r] to the off-diagonal sum.
+template <int G, class VT, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restrict__ rowptr,
                                               const int32_t *__restrict__ cols,
                                               const VT *__restrict__ vals,
-                                              const XT *__restrict__ x, SpmvEpi e) {
+                                              const XT *__restrict__ x,
+                                              const double2 *__restrict__ diag, SpmvEpi e) {
     if (e.run && *e.run == 0) return;
-    using acc_t = typename AccT<VT>::type;
+    using acc_t = ACC;
     const int lig = threadIdx.x & (G - 1);
     const int64_t group = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
     const int64_t ngroups = (int64_t)gridDim.x * (256 / G);
@@ -114,7 +118,9 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
             si += __shfl_xor(si, off, G);
         }
         if (lig == 0) {
-            double2 out = cmul(c0, make_double2((double)sr, (double)si));
+            double2 ax = make_double2((double)sr, (double)si);
+            if constexpr (SPLIT) ax = cadd(ax, cmul(diag[r], x[r]));
+            double2 out = cmul(c0, ax);
             if (e.z) out = cadd(out, cmul(c1, e.z[r]));
             e.y[r] = out;
             if (e.dvec) dot = cadd(dot, cconjmul(e.dvec[r], out));
@@ -187,13 +193,20 @@ __global__ void k_to_c64_vec(int64_t n, const double2 *in, float2 *out, const in
         out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
-int lanes_for(const edx_csr *a) {
-    double avg = a->m ? (double)a->nnz / (double)a->m : 0.0;
+int lanes_for_avg(double avg) {
     if (avg < 12) return 4;
     if (avg < 40) return 8;
     if (avg < 160) return 16;
     if (avg < 480) return 32;
     return 64;
+}
+
+int lanes_for(const edx_csr *a) {
+    return lanes_for_avg(a->m ? (double)a->nnz / (double)a->m : 0.0);
+}
+
+int lanes_for_split(const edx_csr *a) {
+    return lanes_for_avg(a->m ? (double)a->off_nnz / (double)a->m : 0.0);
 }
 
 }  // namespace
@@ -250,14 +263,18 @@ int spmv_blocks(const edx_csr *a, int G) {
 }
 
 // Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
-// complex64 (already converted) for C64.  Returns the block count used.
-int bspmm_single(const edx_csr *a, const double2 *x, const SpmvEpi &e, int *nblocks_out);
+// C64D and complex64 (already converted) for C64.  Returns the block count used.
 
 int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e, int lanes,
                 int *nblocks_out) {
-    if (precision == EDX_C64D) return bspmm_single(a, (const double2 *)x, e, nblocks_out);
-    int G = lanes ? lanes : (precision == EDX_C64 ? (a->spmv_lanes32 ? a->spmv_lanes32 : lanes_for(a))
-                                                  : (a->spmv_lanes ? a->spmv_lanes : lanes_for(a)));
+    if (precision == EDX_C64D && !a->d_diag)
+        return set_error(EDX_ERR_ARG, "split (c64d) form not prepared");
+    int G = lanes;
+    if (!G) {
+        if (precision == EDX_C64) G = a->spmv_lanes32 ? a->spmv_lanes32 : lanes_for(a);
+        else if (precision == EDX_C64D) G = a->spmv_lanes64d ? a->spmv_lanes64d : lanes_for_split(a);
+        else G = a->spmv_lanes ? a->spmv_lanes : lanes_for(a);
+    }
     int nb = spmv_blocks(a, G);
     if (nblocks_out) *nblocks_out = nb;
     if (a->m == 0) return EDX_OK;
@@ -265,11 +282,17 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
 #define EDX_SPMV_CASE(GG)                                                                     \
     case GG:                                                                                  \
         if (precision == EDX_C128)                                                            \
-            hipLaunchKernelGGL((k_spmv<GG, double2, double2>), dim3(nb), dim3(256), 0, stream(), \
-                               a->m, a->d_rowptr, a->d_cols, a->d_vals, (const double2 *)x, e); \
+            hipLaunchKernelGGL((k_spmv<GG, double2, double2, double, false>), dim3(nb), dim3(256), \
+                               0, stream(), a->m, a->d_rowptr, a->d_cols, a->d_vals,           \
+                               (const double2 *)x, nullptr, e);                                \
+        else if (precision == EDX_C64D)                                                       \
+            hipLaunchKernelGGL((k_spmv<GG, float2, double2, double, true>), dim3(nb), dim3(256),  \
+                               0, stream(), a->m, a->d_off_rowptr, a->d_off_cols,              \
+                               a->d_off_vals32, (const double2 *)x, a->d_diag, e);             \
         else                                                                                  \
-            hipLaunchKernelGGL((k_spmv<GG, float2, float2>), dim3(nb), dim3(256), 0, stream(),   \
-                               a->m, a->d_rowptr, a->d_cols, a->d_vals32, (const float2 *)x, e); \
+            hipLaunchKernelGGL((k_spmv<GG, float2, float2, float, false>), dim3(nb), dim3(256),   \
+                               0, stream(), a->m, a->d_rowptr, a->d_cols, a->d_vals32,         \
+                               (const float2 *)x, nullptr, e);                                 \
         break;
     switch (G) {
         EDX_SPMV_CASE(4)

@@ -172,6 +172,23 @@ int edx_build_krylov_batch(const edx_csr *h, int precision, int nrhs, const void
  * final_res[k] = |(sigma_k - H) x_k - b_k|.  This replaces the reference's
  * one-fsolver-call-per-(omega_in, polarization) loop with one shifted solve of
  * the unshifted H_n (edrixs/solvers.py:2651-2693, src/rixs_driver.f90:130-150). */
+/* Multi-shift Galerkin solve of (sigma_k I - H) x_k = b for nshift shifts from
+ * ONE Lanczos recurrence on H, keeping only projections: for each of nproj
+ * operators P_p (a CSR with H's row count as its column count, or NULL for the
+ * identity) d_out[p] receives P_p x_k for every k, rows(P_p) x nshift
+ * column-major complex128.  Shift k stops at the first iterate whose residual
+ * |b - (sigma_k - H) x_k| (the recurrence's own value beta_m |e_m^T y_m|) is
+ * <= tol |b|; iters[k] and final_res[k] (relative to |b|) describe that iterate.
+ * Every sigma_k needs a nonzero imaginary part (the core-hole width); nshift <=
+ * 256.  This replaces the reference's per-incident-energy rebuild of
+ * omega - H_n and pminres_csr solve (src/rixs_driver.f90:127-150,
+ * src/linsys.f90:3-128) followed by the emission operator applied to each
+ * solution (src/rixs_driver.f90:163-185): one recurrence serves all incident
+ * energies of a polarization (edrixs/solvers.py:2651-2693 loops over them). */
+int edx_shifted_fom_project(const edx_csr *h, int precision, const void *d_b, int nshift,
+                            const double *sigma, int nproj, const edx_csr *const *proj,
+                            void *const *d_out, double tol, int maxit, int *iters,
+                            double *final_res);
 int edx_minres_shifted_batch(const edx_csr *h, int precision, int nrhs, const double *sigma,
                              const void *d_B, void *d_X, double tol, int maxit, int *iters,
                              double *final_res);

@@ -57,3 +57,29 @@ def small_siam(nbath=2, seed=1, nocc=None):
 
 def ncomb(n, k):
     return math.comb(n, k)
+
+
+def herm_c64_values(rowptr, cols, vals, n):
+    """Expected complex64 copy of a square CSR: an off-diagonal entry whose
+    transposed partner is its conjugate to 1e-13 relative is rounded from the
+    FP64 Hermitian average (a_rc + conj(a_cr)) / 2, so the stored copy stays
+    exactly Hermitian; a diagonal entry whose imaginary part is rounding
+    residue (<= 1e-13 of the real part) becomes real; all other entries round
+    independently."""
+    rowptr = np.asarray(rowptr, np.int64)
+    cols = np.asarray(cols, np.int64)
+    vals = np.asarray(vals, np.complex128)
+    rows = np.repeat(np.arange(len(rowptr) - 1, dtype=np.int64), np.diff(rowptr))
+    keys = rows * n + cols                     # ascending: rows, then sorted columns
+    pk = cols * n + rows
+    pos = np.clip(np.searchsorted(keys, pk), 0, max(len(keys) - 1, 0))
+    found = (keys[pos] == pk) & (cols != rows) if len(keys) else np.zeros(0, bool)
+    a, b = vals, vals[pos]
+    close = (np.abs(a.real - b.real) + np.abs(a.imag + b.imag)
+             <= 1e-13 * (np.abs(a.real) + np.abs(a.imag) + np.abs(b.real) + np.abs(b.imag)))
+    use = found & close
+    out = vals.copy()
+    out[use] = (a.real[use] + b.real[use]) * 0.5 + 1j * ((a.imag[use] - b.imag[use]) * 0.5)
+    diag = (cols == rows) & (np.abs(vals.imag) <= 1e-13 * np.abs(vals.real))
+    out[diag] = vals.real[diag]
+    return out.astype(np.complex64)

@@ -150,7 +150,26 @@ def test_c64_copy_is_rounded_c128(core):
     assert A.has_c64
     v64 = A.values("c128")
     v32 = A.values("c64")
-    assert np.array_equal(v32, v64.astype(np.complex64))   # round-to-nearest of each part
+    rp, cols, _ = A.download()
+    assert np.array_equal(v32, H.herm_c64_values(rp, cols, v64, A.n))
+    # the copy is exactly Hermitian and each value is within one float ulp of FP64
+    d32 = sp_dense(rp, cols, v32, A.m, A.n)
+    assert np.array_equal(d32, d32.conj().T)
+    assert np.all(np.abs(v32 - v64) <= 1.2e-7 * np.abs(v64) + 1e-300)
+    # a rectangular operator keeps plain per-entry rounding
+    m = H.small_siam(nbath=2, seed=3)
+    bi = core.Basis.from_words(H.fock_words(m.norb, m.nocc), m.norb)
+    bn = core.Basis.from_words(H.fock_words(m.norb, m.nocc + 1), m.norb)
+    D = core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[0])
+    D.add_c64()
+    assert np.array_equal(D.values("c64"), D.values("c128").astype(np.complex64))
+
+
+def sp_dense(rp, cols, vals, m, n):
+    out = np.zeros((m, n), vals.dtype)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    out[rows, cols] = vals
+    return out
 
 
 # ------------------------------------------------------------------- SpMV
@@ -444,10 +463,14 @@ def test_split_c64d_product_error_is_offdiagonal_rounding_only(core):
     rng = np.random.default_rng(1)
     x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
     y = A.matvec(_to_dev(core, x), precision="c64d").cpu().numpy()
-    # reference: exact diagonal + complex64-rounded off-diagonal values, FP64 arithmetic
+    # reference: exact diagonal + complex64-rounded (Hermitian-consistent) off-diagonal
+    # values, FP64 arithmetic
     D = R.todense()
     dg = np.diag(np.diag(D))
-    off = (D - dg).astype(np.complex64).astype(np.complex128)
+    v32 = H.herm_c64_values(R.rowptr, R.cols, R.vals, R.ncols)
+    off = sp_dense(R.rowptr, R.cols, v32.astype(np.complex128), R.m, R.ncols)
+    off[np.arange(R.m), np.arange(R.m)] = 0
+    assert np.array_equal(off, off.conj().T)
     ref = (dg + off) @ x
     np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
     exact = D @ x
@@ -493,3 +516,79 @@ def test_rhs_parallel_batches_match_single(core):
         for k in range(9):
             r = sig[k] * Xd[k] - Hd @ Xd[k] - B[k]
             assert np.linalg.norm(r) < 1e-7 * np.linalg.norm(B[k]), (prec, k)
+
+
+# ------------------------------------------------------- multi-shift RIXS solve
+
+def _rixs_case(seed=3):
+    m = H.small_siam(nbath=2, seed=seed)
+    fi, fn = H.fock_words(m.norb, m.nocc), H.fock_words(m.norb, m.nocc + 1)
+    Ri = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    e, v = np.linalg.eigh(Ri)
+    Rn = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    Din = o.build_transop_i(fn, fi, m.norb, m.ncore, m.trans_i[0]).todense()
+    Dout = o.build_transop_f(fi, fn, m.norb, m.ncore, m.trans_f[1])
+    b = Din @ v[:, 0]
+    lo = np.linalg.eigvalsh(Rn.todense())[0]
+    sig = lo + np.linspace(-2.0, 10.0, 6) + 0.5j
+    return m, fi, fn, Rn, Dout, b, sig
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64d"])
+def test_shifted_fom_matches_dense_solve(core, precision):
+    m, fi, fn, Rn, Dout, b, sig = _rixs_case()
+    bi, bn = core.Basis.from_words(fi, m.norb), core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    Df = core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1])
+    tol = 1e-10 if precision == "c128" else 1e-8
+    (X, Y), it, res = core.shifted_fom(Hn, _to_dev(core, b), sig, projections=(None, Df),
+                                       tol=tol, maxit=4000, precision=precision)
+    X, Y = X.cpu().numpy(), Y.cpu().numpy()
+    A = Rn.todense()
+    Dd = Dout.todense()
+    nb = np.linalg.norm(b)
+    for k in range(len(sig)):
+        S = sig[k] * np.eye(A.shape[0]) - A
+        x = np.linalg.solve(S, b)
+        true_res = np.linalg.norm(b - S @ X[k]) / nb
+        assert res[k] <= tol and 0 < it[k] < 4000
+        if precision == "c128":
+            # the recurrence's residual tracks the true one (no reorthogonalization)
+            assert true_res < 10 * tol
+            assert np.linalg.norm(X[k] - x) < 1e-7 * np.linalg.norm(x)
+            assert np.linalg.norm(Y[k] - Dd @ x) < 1e-7 * np.linalg.norm(Dd @ x)
+            # the reference's per-shift MINRES reaches the same solution
+            R = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n, omega=sig[k], ham_sign=-1.0)
+            rx, _, _ = o.pminres(R, b, 1e-12 * nb, 6000)
+            assert np.linalg.norm(X[k] - rx) < 1e-7 * np.linalg.norm(rx)
+        else:
+            # complex64 off-diagonal values: operator rounding ~ 1e-7 relative
+            assert true_res < 1e-5
+            assert np.linalg.norm(Y[k] - Dd @ x) < 1e-5 * np.linalg.norm(Dd @ x)
+        # the projection of the returned iterate is the projection of x
+        assert np.linalg.norm(Y[k] - Dd @ X[k]) < 1e-12 * np.linalg.norm(Y[k]) + 1e-300
+
+
+def test_shifted_fom_edge_cases(core):
+    m, fi, fn, Rn, Dout, b, sig = _rixs_case()
+    bn = core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    # b = 0: zero solution, no iterations
+    (X,), it, res = core.shifted_fom(Hn, _to_dev(core, np.zeros(Hn.n)), sig[:2])
+    assert np.all(X.cpu().numpy() == 0) and list(it) == [0, 0] and list(res) == [0, 0]
+    # a real shift has no guaranteed nonzero pivots: rejected
+    with pytest.raises(Exception, match="imaginary"):
+        core.shifted_fom(Hn, _to_dev(core, b), [sig[0], 3.0 + 0j])
+    # maxit caps the iteration; the reported residual is that of the last iterate
+    (X,), it, res = core.shifted_fom(Hn, _to_dev(core, b), sig[:1], tol=0.0, maxit=7)
+    A = sig[0] * np.eye(Hn.n) - Rn.todense()
+    assert it[0] == 7
+    tr = np.linalg.norm(b - A @ X.cpu().numpy()[0]) / np.linalg.norm(b)
+    assert abs(tr - res[0]) < 1e-6 * tr
+    # an eigenvector start converges at once (Lanczos breakdown after one step)
+    ev = np.linalg.eigh(Rn.todense())[1][:, 5]
+    (X,), it, res = core.shifted_fom(Hn, _to_dev(core, ev), sig[:3], tol=1e-12)
+    w = np.linalg.eigvalsh(Rn.todense())[5]
+    for k in range(3):
+        assert it[k] == 1
+        np.testing.assert_allclose(X.cpu().numpy()[k], ev / (sig[k] - w), atol=1e-12)
