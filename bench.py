@@ -8,14 +8,16 @@ core-hole space 14,976,864 rows), all on the device:
   1. CSR build of H_i, H_n, the three dipole absorption operators and the
      emission operators of the RIXS polarizations;
   2. ground state (Lanczos; the mixed forms finish with FP64 refinement);
-  3. isotropic XAS: the three Krylov recurrences (nkryl steps) from D_k|gs>
-     run as one batched recurrence over H_n; continued fraction on nw points;
+  3. isotropic XAS: the three Krylov recurrences from D_k|gs> run as one
+     batched recurrence over H_n for at least nkryl steps and until each
+     continued fraction on nw points has converged (changes by less than
+     cf_tol over cf_chunk further steps);
   4. RIXS at n_in incident energies x 2 polarizations: the shifted systems
      (w_in + E_gs + i Gamma_c - H_n) x = D_in|gs> of one polarization solved
      together by one multi-shift Galerkin recurrence over the unshifted H_n
      that keeps only D_out x (relative residual rixs_tol), then the
-     final-state Krylov recurrences over H_i batched; continued fractions on
-     nw energy-loss points.
+     final-state Krylov recurrences over H_i batched and run to converged
+     continued fractions on the nw energy-loss points the same way.
 
 Spectra per step = 1 (isotropic XAS) + 2 n_in (RIXS columns).  Three forms
 are timed: fp64 (complex128), c64d (complex64 off-diagonal values with the
@@ -56,11 +58,21 @@ def parse():
     p.add_argument("--rixs-tol", type=float, default=1e-8,
                    help="relative residual of the RIXS intermediate-state solves")
     p.add_argument("--rixs-maxit", type=int, default=6000)
+    p.add_argument("--cf-tol", type=float, default=1e-8,
+                   help="Krylov recurrences stop once a further cf-chunk steps change "
+                        "their continued fraction by less than this (relative L2)")
+    p.add_argument("--cf-chunk", type=int, default=250)
+    p.add_argument("--nkryl-max", type=int, default=5000)
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--forms", default="c128,c64d,c64")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    p.add_argument("--no-exact", action="store_true", help="skip the reorthogonalized reference")
+    p.add_argument("--exact", action="store_true",
+                   help="also compare with the fully reorthogonalized FP64 projection")
     p.add_argument("--cpu-rows", type=int, default=40000)
+    p.add_argument("--probe", action="store_true",
+                   help="run only a short fixed set of SpMV/SpMM launches (for counter runs)")
+    p.add_argument("--probe-form", default="c64d")
+    p.add_argument("--launches", type=int, default=10)
     return p.parse_args()
 
 
@@ -106,13 +118,11 @@ class Workload:
         # isotropic XAS: three components in one batched recurrence
         t0 = time.perf_counter()
         Phi = torch.stack([D[k].matvec(gs) for k in range(3)]).contiguous()
-        al, be, nr = c.build_krylov_batch(Hn, Phi, a.nkryl, precision)
-        xas = np.zeros(a.nw)
-        xas_comp = []
-        for k in range(3):
-            s = c.cf_spectrum(al[k], be[k], nr[k], egs, self.omega, self.GAMMA_C)
-            xas_comp.append(s)
-            xas += s / 3.0
+        # at least nkryl steps, continued until each fraction has converged
+        xas_comp, xsteps, xchange = c.converged_spectra(
+            Hn, Phi, egs, self.omega, self.GAMMA_C, precision=precision, tol=a.cf_tol,
+            chunk=a.cf_chunk, min_steps=a.nkryl, max_steps=a.nkryl_max)
+        xas = sum(xas_comp) / 3.0
         torch.cuda.synchronize()
         T["xas"] = time.perf_counter() - t0
         if self.w_in is None:   # incident energies: 10 eV window starting 2 eV below the edge
@@ -138,13 +148,19 @@ class Workload:
         # (w_in, pol) order: q = i_w * npol + p
         Phf = torch.stack([proj[p][i] for i in range(len(self.w_in))
                            for p in range(npol)]).contiguous()
-        alf, bef, nrf = c.build_krylov_batch(Hi, Phf, a.nkryl, precision)
-        rixs = [c.cf_spectrum(alf[q], bef[q], nrf[q], egs, self.eloss, self.GAMMA_F)
-                for q in range(Phf.shape[0])]
+        # final-state recurrences run until each continued fraction has converged
+        # on the energy-loss grid (the narrow final-state width needs far more
+        # than the XAS step count)
+        rixs, ksteps, kchange = c.converged_spectra(
+            Hi, Phf, egs, self.eloss, self.GAMMA_F, precision=precision, tol=a.cf_tol,
+            chunk=a.cf_chunk, min_steps=a.nkryl, max_steps=a.nkryl_max)
         torch.cuda.synchronize()
         T["rixs_krylov"] = time.perf_counter() - t0
         info = dict(nnz_i=Hi.nnz, nnz_n=Hn.nnz, lanczos_iters=it,
                     rixs_iters=iters, rixs_res_rel_max=max(res),
+                    rixs_krylov_steps=[int(v) for v in ksteps],
+                    xas_krylov_steps=[int(v) for v in xsteps], xas_cf_change_max=float(max(xchange)),
+                    rixs_cf_change_max=float(max(kchange)),
                     phases_s=T, egs=egs)
         out = dict(egs=egs, xas=xas, xas_comp=xas_comp, rixs=rixs, info=info, gs=gs)
         del Hi, Hn, D, Din, Dout, Phi, proj, Phf, vecs
@@ -168,7 +184,8 @@ def timed(work, precision, steps, warmup):
             {k: round(v, 3) for k, v in out["info"]["phases_s"].items()}))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    prof = c.profile_read(precision)
+    prof = {k: c.profile_read(precision, k) for k in c.KERNELS}
+    prof["all"] = c.profile_read(precision)
     c.profile(False)
     return dt, out, prof
 
@@ -179,22 +196,28 @@ def rel(a, b):
 
 def exact_reference(work, gs, results):
     """Fully reorthogonalized FP64 Krylov projection for the z dipole component,
-    and the same recurrence with the mixed (c64d) products."""
+    taken to the step count the FP64 form converged at (bounded by the HBM that
+    keeping every basis vector needs), and the same recurrence with the mixed
+    (c64d) products; every form's converged z spectrum is compared with it."""
+    import torch
     c, m, a = work.core, work.model, work.args
     Hn = c.build_ham_n(work.bn, m.norb, m.ncore, m.hop_n, m.coul_n)
     Dz = c.build_transop_i(work.bn, work.bi, m.norb, m.ncore, m.trans_i[2])
     phi = Dz.matvec(gs)
     egs = results["c128"]["egs"]
+    free, _ = torch.cuda.mem_get_info()
+    steps = min(results["c128"]["info"]["xas_krylov_steps"][2],
+                int(0.5 * free / (16.0 * Hn.n)) - 2)
     t = time.perf_counter()
-    al, be, nr = c.build_krylov_reorth(Hn, phi, a.nkryl, "c128")
+    al, be, nr = c.build_krylov_reorth(Hn, phi, steps, "c128")
     ref = c.cf_spectrum(al, be, nr, egs, work.omega, work.GAMMA_C)
-    log("reorthogonalized FP64 reference: %.1f s" % (time.perf_counter() - t))
+    log("reorthogonalized FP64 reference (%d steps): %.1f s" % (steps, time.perf_counter() - t))
     Hn.prepare("c64d")
-    al2, be2, nr2 = c.build_krylov_reorth(Hn, phi, a.nkryl, "c64d")
+    al2, be2, nr2 = c.build_krylov_reorth(Hn, phi, steps, "c64d")
     mix = c.cf_spectrum(al2, be2, nr2, egs, work.omega, work.GAMMA_C)
-    out = {"component": "z", "c64d_reorth_vs_exact": rel(mix, ref)}
+    out = {"component": "z", "steps": steps, "c64d_reorth_vs_exact": rel(mix, ref)}
     for form, r in results.items():
-        out["%s_plain_vs_exact" % form] = rel(r["xas_comp"][2], ref)
+        out["%s_converged_vs_exact" % form] = rel(r["xas_comp"][2], ref)
     return out
 
 
@@ -231,11 +254,12 @@ def cpu_baseline(work, info, rows):
     build_i, build_n = tbi * mi, tbn * mn
     spmv_i, spmv_n = tmi * info["nnz_i"], tmn * info["nnz_n"]
     t_ed = build_i + info["lanczos_iters"] * spmv_i
-    t_xas = 3 * (build_n + a.nkryl * spmv_n)
+    t_xas = 3 * (build_n + np.mean(info["xas_krylov_steps"]) * spmv_n)
     # the reference runs one MINRES per (w_in, pol); its iteration count is taken
     # as that of the Galerkin iterate the device solve stopped at for that shift
     it = np.mean(info["rixs_iters"])
-    t_rixs_each = build_n + it * spmv_n + build_i + a.nkryl * spmv_i
+    kst = np.mean(info["rixs_krylov_steps"])
+    t_rixs_each = build_n + it * spmv_n + build_i + kst * spmv_i
     n_rixs = len(info["rixs_iters"])
     total = t_ed + t_xas + n_rixs * t_rixs_each
     nspec = 1 + n_rixs
@@ -248,6 +272,51 @@ def cpu_baseline(work, info, rows):
             "seconds_per_step": total}
 
 
+def probe(args):
+    """A short fixed workload for hardware-counter runs: the dominant SpMV/SpMM
+    launches of the step at full size (single-vector split SpMV over H_n, the
+    K=3 XAS batch over H_n, the K=16 final-state batch over H_i), each
+    ``--launches`` times, plus a plain device copy of 1 GiB as the calibration
+    kernel of known traffic."""
+    import torch
+    c = __import__("edrixs_amd.core", fromlist=["core"])
+    from edrixs_amd import siam
+    m = siam.SiamModel(nbath=args.nbath, seed=args.seed)
+    bi = c.Basis.combinations(m.norb, m.nocc)
+    bn = c.Basis.combinations(m.norb, m.nocc + 1)
+    Hi = c.build_ham_i(bi, m.hop_i, m.coul_i)
+    Hn = c.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    f = args.probe_form
+    Hi.prepare(f)
+    Hn.prepare(f)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    def rnd(*shape):
+        return torch.randn(*shape, dtype=torch.complex128, device="cuda", generator=g)
+    x, y = rnd(Hn.n), rnd(Hn.n)
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("probe: copy done")
+    for _ in range(args.launches):
+        Hn.spmv(x, y, precision=f)
+    torch.cuda.synchronize()
+    log("probe: %d k_spmv launches on H_n done" % args.launches)
+    kb = c.KrylovBatch(Hn, rnd(3, Hn.n), f, capacity=args.launches + 1)
+    kb.advance(args.launches)
+    kb.close()
+    torch.cuda.synchronize()
+    log("probe: %d K=3 steps on H_n done" % args.launches)
+    kb = c.KrylovBatch(Hi, rnd(16, Hi.n), f, capacity=args.launches + 1)
+    kb.advance(args.launches)
+    kb.close()
+    torch.cuda.synchronize()
+    log("probe: %d K=16 steps on H_i done" % args.launches)
+    print(json.dumps({"probe": f, "launches": args.launches, "nnz_H_n": Hn.nnz, "nnz_H_i": Hi.nnz,
+                      "rows_H_n": Hn.m, "rows_H_i": Hi.m}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -257,6 +326,9 @@ def main():
         sys.exit(2)
     import torch
     torch.zeros(1, device="cuda")
+    if args.probe:
+        probe(args)
+        return
     work = Workload(args)
     forms = args.forms.split(",")
     results, times, profs = {}, {}, {}
@@ -266,21 +338,30 @@ def main():
     nspec = 1 + 2 * args.n_in
     main_form = "c64d" if "c64d" in forms else forms[0]
     t_main = times[main_form]
-    n, ms, by = profs[main_form]
+    # the dominant kernel: the SpMV/SpMM variant with the most kernel time
+    kname = max(work.core.KERNELS, key=lambda k: profs[main_form][k][1])
+    n, ms, by = profs[main_form][kname]
     gbs = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     m = work.model
     forms_out = {}
     ref = results.get("c128")
     for f in forms:
         r = results[f]
-        pn, pms, pby = profs[f]
+        kern = {}
+        for k, (pn, pms, pby) in profs[f].items():
+            if pn:
+                kern[k] = {"launches": pn, "total_s": pms * 1e-3, "avg_us": pms / pn * 1e3,
+                           "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
         d = {"spectra_per_s": nspec * args.steps / times[f],
              "ms_per_step": times[f] / args.steps * 1e3,
              "phases_s": r["info"]["phases_s"],
-             "spmv_launches": pn, "spmv_avg_us": pms / max(pn, 1) * 1e3,
-             "spmv_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0,
+             "spmv_kernels": kern,
              "lanczos_iters": r["info"]["lanczos_iters"],
              "rixs_iters_max": max(r["info"]["rixs_iters"]),
+             "rixs_krylov_steps_max": max(r["info"]["rixs_krylov_steps"]),
+             "rixs_cf_change_max": r["info"]["rixs_cf_change_max"],
+             "xas_krylov_steps_max": max(r["info"]["xas_krylov_steps"]),
+             "xas_cf_change_max": r["info"]["xas_cf_change_max"],
              "rixs_rel_residual_max": r["info"]["rixs_res_rel_max"]}
         if ref is not None and f != "c128":
             d["ground_energy_abs_err_eV"] = abs(r["egs"] - ref["egs"])
@@ -305,9 +386,9 @@ def main():
         "config": {
             "workload": ("norb=24 N=12 SIAM (10 d + 14 bath, 2p core), end to end: CSR builds of "
                          "H_i (%d rows) and H_n (%d rows), ground state, isotropic XAS (3 batched "
-                         "x %d Krylov steps) and RIXS at %d incident energies x 2 polarizations "
+                         "Krylov recurrences, >= %d steps, to converged continued fractions) and RIXS at %d incident energies x 2 polarizations "
                          "(one multi-shift Galerkin recurrence over H_n per polarization, projected by "
-                         "D_out, + batched final-state Krylov over H_i), "
+                         "D_out, + batched final-state Krylov over H_i to converged continued fractions), "
                          "continued fractions on %d points"
                          % (work.bi.n, work.bn.n * m.ncore, args.nkryl, args.n_in, args.nw)),
             "form": main_form, "spectra_per_step": nspec,
@@ -317,14 +398,14 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_spmm_il (%s), all launches of the timed region" % main_form,
+            "kernel": "%s (%s form), every launch of the timed region" % (kname, main_form),
             "launches": n, "avg_us": ms / max(n, 1) * 1e3, "bytes_per_launch": by / max(n, 1),
         },
         "forms": forms_out,
     }
     if "c128" in results:
         line["speedup_vs_fp64"] = times["c128"] / t_main
-    if not args.no_exact and "c128" in results:
+    if args.exact and "c128" in results:
         line["accuracy_vs_exact_projection"] = exact_reference(work, results["c128"]["gs"], results)
     if not args.no_cpu:
         cb = cpu_baseline(work, results[main_form]["info"], args.cpu_rows)

@@ -65,12 +65,18 @@ _SIGS = {
     "edx_build_krylov_reorth": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
                                      ctypes.POINTER(_D)]),
     "edx_minres_shifted_batch": (_I, [_P, _I, _I, _P, _P, _P, _D, _I, _P, _P]),
+    "edx_krylov_batch_start": (_I, [_P, _I, _I, _P, _I, _PP]),
+    "edx_krylov_batch_advance": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "edx_krylov_batch_read": (_I, [_P, _I, _P, _P, _P, _P]),
+    "edx_krylov_batch_free": (None, [_P]),
     "edx_shifted_fom_project": (_I, [_P, _I, _P, _I, _P, _I, _PP, _PP, _D, _I, _P, _P]),
     "edx_csr_free": (None, [_P]),
+    "edx_csr_spmv_lanes": (_I, [_P, _I, _I, ctypes.POINTER(_I)]),
     "edx_spmv": (_I, [_P, _I, _P, _P]),
     "edx_spmm": (_I, [_P, _I, _I, _P, _P]),
     "edx_profile": (_I, [_I]),
-    "edx_profile_read": (_I, [_I, ctypes.POINTER(_I64), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "edx_profile_read": (_I, [_I, _I, ctypes.POINTER(_I64), ctypes.POINTER(_D),
+                              ctypes.POINTER(_D)]),
     "edx_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,
                               ctypes.POINTER(_I)]),
     "edx_lanczos_refine": (_I, [_P, _I, _I, _I, _D, _I, _P, _I, _P, _P, ctypes.POINTER(_I)]),

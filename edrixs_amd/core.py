@@ -133,7 +133,7 @@ class Basis:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _lib._lib is not None:
+        if h is not None and _lib is not None and _lib._lib is not None:
             _lib._lib.edx_basis_free(h)
             self._h = None
 
@@ -172,6 +172,12 @@ class CSRMatrix:
         _lib.call("edx_csr_prepare", self._h, _prec(precision))
         return self
 
+    def spmv_lanes(self, precision="c128", set=0):
+        """Lanes per row of the single-vector SpMV (set > 0 fixes it; 0 = rule)."""
+        out = ctypes.c_int()
+        _lib.call("edx_csr_spmv_lanes", self._h, _prec(precision), int(set), ctypes.byref(out))
+        return out.value
+
     def add_c64(self):
         _lib.call("edx_csr_add_c64", self._h)
         self.has_c64 = True
@@ -204,7 +210,7 @@ class CSRMatrix:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _lib._lib is not None:
+        if h is not None and _lib is not None and _lib._lib is not None:
             _lib._lib.edx_csr_free(h)
             self._h = None
 
@@ -329,6 +335,113 @@ def build_krylov_batch(H, V0, maxn, precision="c128"):
             [beta[k, :neff[k]].copy() for k in range(K)], norm)
 
 
+class KrylovBatch:
+    """Resumable batched Krylov recurrence (edx_krylov_batch_*) over H from the
+    rows of V0 (K, n); the device keeps vectors and the (alpha, beta) history."""
+
+    def __init__(self, H, V0, precision="c128", capacity=256):
+        V0 = V0.contiguous()
+        assert V0.dim() == 2 and V0.shape[1] == H.n and V0.dtype == torch().complex128
+        self.K = V0.shape[0]
+        self.H = H                      # the state reads H: keep it alive
+        self.done = 0
+        self._h = None
+        h = ctypes.c_void_p()
+        _lib.call("edx_krylov_batch_start", H._h, _prec(precision), self.K, _dptr(V0),
+                  int(capacity), ctypes.byref(h))
+        self._h = h
+
+    def advance(self, nsteps):
+        """Up to nsteps more steps; returns the total steps taken (fewer than asked
+        once every recurrence has broken down)."""
+        d = ctypes.c_int()
+        _lib.call("edx_krylov_batch_advance", self._h, int(nsteps), ctypes.byref(d))
+        self.done = d.value
+        return self.done
+
+    def coefficients(self, count=None):
+        """Per RHS: alpha_1..neff, beta_1..neff (pole-file convention) of the first
+        ``count`` steps, and the norms <v0|v0>."""
+        count = self.done if count is None else int(count)
+        K = self.K
+        alpha = np.zeros((K, max(count, 1)))
+        beta = np.zeros((K, max(count, 1)))
+        neff = np.zeros(K, dtype=np.int32)
+        norm = np.zeros(K)
+        _lib.call("edx_krylov_batch_read", self._h, count, _ptr(neff), _ptr(alpha), _ptr(beta),
+                  _ptr(norm))
+        return ([alpha[k, :neff[k]].copy() for k in range(K)],
+                [beta[k, :neff[k]].copy() for k in range(K)], norm)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None and _lib._lib is not None:
+            _lib._lib.edx_krylov_batch_free(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+# A block of K interleaved vectors larger than this runs as K single-vector
+# recurrences instead (each vector then stays in the 256 MB last-level cache).
+SPLIT_BLOCK_BYTES = 192 << 20
+
+
+def converged_spectra(H, V0, e_gs, grid, gamma, precision="c128", tol=1e-8, chunk=250,
+                      min_steps=250, max_steps=4000):
+    """Continued-fraction spectra of the K start vectors (rows of V0), each taken at
+    the first multiple of ``chunk`` steps (>= min_steps) where its relative L2
+    change since the previous multiple is <= tol, or where its recurrence broke
+    down.  Returns (spectra, steps, last_change); a spectrum that never met tol
+    by max_steps keeps its last value and reports its last change.  Fewer than
+    8 vectors whose interleaved block exceeds SPLIT_BLOCK_BYTES advance as
+    separate single-vector recurrences."""
+    V0 = V0.contiguous()
+    K = V0.shape[0]
+    if K < 8 and K * H.n * 16 > SPLIT_BLOCK_BYTES:
+        groups = [[k] for k in range(K)]
+    else:
+        groups = [list(range(K))]
+    cap = max(min_steps, chunk) + chunk
+    kbs = [KrylovBatch(H, V0[g].contiguous() if len(g) < K else V0, precision, capacity=cap)
+           for g in groups]
+    prev = [None] * K
+    out = [None] * K
+    steps = [0] * K
+    change = [np.inf] * K
+    m = 0
+    try:
+        while True:
+            target = min(max(m + chunk, min_steps), max_steps)
+            for g, kb in zip(groups, kbs):
+                if all(out[k] is not None for k in g):
+                    continue
+                got = kb.advance(target - kb.done)
+                al, be, nr = kb.coefficients(got)
+                for j, k in enumerate(g):
+                    if out[k] is not None:
+                        continue
+                    s = cf_spectrum(al[j], be[j], nr[j], e_gs, grid, gamma)
+                    ended = len(al[j]) < got or got < target
+                    if prev[k] is not None:
+                        den = np.linalg.norm(s)
+                        change[k] = float(np.linalg.norm(s - prev[k]) / den) if den > 0 else 0.0
+                    if ended or change[k] <= tol or target >= max_steps:
+                        out[k] = s
+                        steps[k] = len(al[j])
+                        if ended:
+                            change[k] = 0.0
+                    prev[k] = s
+            m = target
+            if all(o is not None for o in out):
+                break
+    finally:
+        for kb in kbs:
+            kb.close()
+    return out, steps, change
+
+
 def minres_shifted_batch(H, sigma, B, tol=1e-10, maxit=500, precision="c128"):
     """Solve (sigma_k - H) x_k = b_k for the rows b_k of B (K, n) in one recurrence."""
     T = torch()
@@ -407,11 +520,16 @@ def profile(enable=True):
     _lib.call("edx_profile", int(bool(enable)))
 
 
-def profile_read(precision="c128"):
-    """(launches, kernel milliseconds, algorithmic bytes) recorded for one value type."""
+KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3}
+
+
+def profile_read(precision="c128", kernel=None):
+    """(launches, kernel milliseconds, algorithmic bytes) recorded for one
+    precision and one SpMV/SpMM kernel (a name of KERNELS; None = all)."""
     n = ctypes.c_int64()
     ms = ctypes.c_double()
     by = ctypes.c_double()
-    _lib.call("edx_profile_read", _prec(precision), ctypes.byref(n), ctypes.byref(ms),
+    k = -1 if kernel is None else KERNELS[kernel]
+    _lib.call("edx_profile_read", _prec(precision), k, ctypes.byref(n), ctypes.byref(ms),
               ctypes.byref(by))
     return n.value, ms.value, by.value

@@ -569,11 +569,25 @@ namespace edx {
 // shift terms read it).
 int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
           const BEpi &e, int *nblocks_out) {
+    if (K == 1 && a->m >= TUNE_MIN_ROWS && !e.c2) {
+        // one right-hand side of a large matrix: the tuned single-vector kernel
+        // (its vector then fits the last-level cache, which an interleaved
+        // block of several does not)
+        SpmvEpi se{};
+        se.y = e.y;
+        se.z = e.z;
+        se.c0 = e.c0;
+        se.c1 = e.c1;
+        se.dvec = e.dvec;
+        se.partials = e.partials;
+        return spmv_launch(a, precision, precision == EDX_C64 ? (const void *)X32 : (const void *)X64,
+                           se, 0, nblocks_out);
+    }
     const int nb = bspmm_blocks(a->m);
     if (nblocks_out) *nblocks_out = nb;
     if (a->m == 0 || K == 0) return EDX_OK;
-    hipEvent_t pe = prof_start(precision, spmv_bytes_k(a, precision, K, e.z != nullptr,
-                                                       e.dvec != nullptr));
+    hipEvent_t pe = prof_start(precision, K >= 8 ? EDX_KERNEL_SPMM_RL : EDX_KERNEL_SPMM_IL,
+                               spmv_bytes_k(a, precision, K, e.z != nullptr, e.dvec != nullptr));
     if (K >= 8) {
         const int kp = K < RL_MAXK ? K : RL_MAXK;
         const int npl = kp <= 16 ? 1 : (kp <= 32 ? 2 : (kp <= 64 ? 4 : 8));
@@ -673,10 +687,11 @@ int csr_add_split(edx_csr *a) {
 }
 
 int ensure_precision(edx_csr *a, int precision) {
-    if (precision == EDX_C128) return EDX_OK;
-    if (precision == EDX_C64) return edx_csr_add_c64(a);
-    if (precision == EDX_C64D) return csr_add_split(a);
-    return set_error(EDX_ERR_ARG, "unknown precision %d", precision);
+    if (precision == EDX_C64) EDX_TRY(edx_csr_add_c64(a));
+    else if (precision == EDX_C64D) EDX_TRY(csr_add_split(a));
+    else if (precision != EDX_C128) return set_error(EDX_ERR_ARG, "unknown precision %d", precision);
+    if (a->m == a->n) EDX_TRY(tune_spmv(a, precision));
+    return EDX_OK;
 }
 
 }  // namespace edx
@@ -702,79 +717,169 @@ struct Dbuf {
         hipLaunchKernelGGL(kernel, el_.grid, el_.block, 0, edx::stream(), __VA_ARGS__); \
     } while (0)
 
-int bkrylov_run(const edx_csr *h, int precision, int K, const double2 *V0cm, int maxn, int *neff,
-                double *alpha, double *beta, double *norm) {
-    const int64_t n = h->n;
-    hipStream_t s = edx::stream();
+}  // namespace
+
+// A batched Krylov recurrence that can be advanced in pieces: the vectors, the
+// run flags and the (alpha, beta) history stay on the device between calls.
+// The scalar arrays grow geometrically (device-to-device copy) when a call
+// needs more steps than they hold.
+struct edx_krylov_batch {
+    const edx_csr *h = nullptr;
+    int precision = 0;
+    int K = 0;
+    int64_t n = 0;
+    int done = 0;        // recurrence steps taken
+    int cap = 0;         // steps the scalar arrays hold
+    bool live = true;    // some RHS has not broken down
+    int curr = 0, prev = 1;
     Dbuf vb, x32, part, sc, flags;
-    EDX_TRY(vb.alloc((size_t)2 * n * K * sizeof(double2), "batched krylov vectors"));
-    if (precision == EDX_C64) EDX_TRY(x32.alloc((size_t)n * K * sizeof(float2), "batched x32"));
-    const int nbs = bspmm_blocks(n), nbe = elem_grid_x(n);
-    const int nbp = std::max(nbs, nbe);
-    EDX_TRY(part.alloc((size_t)nbp * K * sizeof(double2) + 64, "partials"));
-    const size_t nsc = (size_t)maxn * K + (size_t)(maxn + 1) * K + K;   // alpha, beta, norm
+    BKrylov st{};
+};
+
+namespace {
+
+int kb_layout(edx_krylov_batch *q, int cap, const void *old_sc, const void *old_flags, int old_cap) {
+    const int K = q->K;
+    Dbuf sc, flags;
+    const size_t nsc = (size_t)cap * K + (size_t)(cap + 1) * K + K;   // alpha, beta, norm
     EDX_TRY(sc.alloc(nsc * sizeof(double) + (size_t)2 * K * sizeof(double2), "batched scalars"));
-    EDX_TRY(flags.alloc(((size_t)(maxn + 1) * K + K) * sizeof(int), "batched flags"));
+    EDX_TRY(flags.alloc(((size_t)(cap + 1) * K + K) * sizeof(int), "batched flags"));
+    hipStream_t s = edx::stream();
+    EDX_HIP(hipMemsetAsync(sc.p, 0, nsc * sizeof(double) + (size_t)2 * K * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(cap + 1) * K + K) * sizeof(int), s));
     BKrylov st;
     st.alpha = sc.as<double>();
-    st.beta = st.alpha + (size_t)maxn * K;
-    st.norm = st.beta + (size_t)(maxn + 1) * K;
+    st.beta = st.alpha + (size_t)cap * K;
+    st.norm = st.beta + (size_t)(cap + 1) * K;
     st.c1 = (double2 *)(st.norm + K);
     st.sums = st.c1 + K;
     st.run = flags.as<int>();
-    st.neff = st.run + (size_t)(maxn + 1) * K;
-    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + (size_t)n * K};
-    EDX_HIP(hipMemsetAsync(vb.p, 0, (size_t)2 * n * K * sizeof(double2), s));
-    EDX_HIP(hipMemsetAsync(sc.p, 0, nsc * sizeof(double), s));
-    EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(maxn + 1) * K + K) * sizeof(int), s));
+    st.neff = st.run + (size_t)(cap + 1) * K;
+    if (old_sc) {
+        const BKrylov &o = q->st;
+        EDX_HIP(hipMemcpyAsync(st.alpha, o.alpha, (size_t)old_cap * K * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+        EDX_HIP(hipMemcpyAsync(st.beta, o.beta, (size_t)(old_cap + 1) * K * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+        EDX_HIP(hipMemcpyAsync(st.norm, o.norm, K * sizeof(double) + 2 * K * sizeof(double2),
+                               hipMemcpyDeviceToDevice, s));
+        EDX_HIP(hipMemcpyAsync(st.run, o.run, (size_t)(old_cap + 1) * K * sizeof(int),
+                               hipMemcpyDeviceToDevice, s));
+        EDX_HIP(hipMemcpyAsync(st.neff, o.neff, K * sizeof(int), hipMemcpyDeviceToDevice, s));
+        (void)old_flags;
+    }
+    std::swap(q->sc.p, sc.p);         // the old buffers are freed with the locals
+    std::swap(q->flags.p, flags.p);
+    q->st = st;
+    q->cap = cap;
+    EDX_HIP(hipStreamSynchronize(s));
+    return EDX_OK;
+}
+
+int kb_start(edx_krylov_batch *q, const double2 *V0cm, int cap) {
+    const int64_t n = q->n;
+    const int K = q->K;
+    hipStream_t s = edx::stream();
+    EDX_TRY(q->vb.alloc((size_t)2 * n * K * sizeof(double2), "batched krylov vectors"));
+    if (q->precision == EDX_C64)
+        EDX_TRY(q->x32.alloc((size_t)n * K * sizeof(float2), "batched x32"));
+    const int nbs = bspmm_blocks(n), nbe = elem_grid_x(n);
+    EDX_TRY(q->part.alloc(std::max<size_t>((size_t)std::max(nbs, nbe) * K + 64,
+                                           (size_t)edx::spmv_partials_len(q->h)) * sizeof(double2),
+                          "partials"));
+    EDX_TRY(kb_layout(q, cap, nullptr, nullptr, 0));
+    const BKrylov &st = q->st;
+    double2 *v0 = q->vb.as<double2>();
+    EDX_HIP(hipMemsetAsync(q->vb.p, 0, (size_t)2 * n * K * sizeof(double2), s));
     const int kb = (K + 63) / 64;
-    ELAUNCH(k_cm_to_il, n, K, n, K, V0cm, v[0]);
-    ELAUNCH(k_bdot, n, K, n, K, v[0], v[0], part.as<double2>());
-    hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, st.sums);
+    ELAUNCH(k_cm_to_il, n, K, n, K, V0cm, v0);
+    ELAUNCH(k_bdot, n, K, n, K, v0, v0, q->part.as<double2>());
+    hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nbe, K, st.sums);
     hipLaunchKernelGGL(k_bk_init, dim3(kb), dim3(64), 0, s, st, K, st.sums);
-    ELAUNCH(k_bdiv_sqrt_norm, n, K, n, K, v[0], st.norm);
+    ELAUNCH(k_bdiv_sqrt_norm, n, K, n, K, v0, st.norm);
     EDX_LAUNCH_CHECK();
-    int curr = 0, prev = 1, checked = 0;
-    for (int i = 1; i <= maxn; ++i) {
-        if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[curr], x32.as<float2>());
+    q->done = 0;
+    q->curr = 0;
+    q->prev = 1;
+    q->live = true;
+    return EDX_OK;
+}
+
+// up to nsteps more steps of every live recurrence (build_krylov_mp,
+// src/lanczos.f90:266-298, K at a time)
+int kb_advance(edx_krylov_batch *q, int nsteps) {
+    if (!q->live || nsteps <= 0) return EDX_OK;
+    if (q->done + nsteps > q->cap) {
+        int nc = std::max(q->cap * 2, q->done + nsteps);
+        EDX_TRY(kb_layout(q, nc, q->sc.p, q->flags.p, q->cap));
+    }
+    const int64_t n = q->n;
+    const int K = q->K;
+    const int kb = (K + 63) / 64;
+    hipStream_t s = edx::stream();
+    const BKrylov &st = q->st;
+    double2 *v[2] = {q->vb.as<double2>(), q->vb.as<double2>() + (size_t)n * K};
+    const int nbe = elem_grid_x(n);
+    const int end = q->done + nsteps;
+    int checked = q->done;
+    for (int i = q->done + 1; i <= end; ++i) {
+        const int curr = q->curr, prev = q->prev;
+        if (q->precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[curr], q->x32.as<float2>());
         BEpi e{};
         e.y = v[prev];
         e.z = v[prev];
         e.c1 = st.c1;
         e.dvec = v[curr];
-        e.partials = part.as<double2>();
+        e.partials = q->part.as<double2>();
         int nb = 0;
-        EDX_TRY(edx::bspmm(h, precision, K, v[curr], x32.as<float2>(), e, &nb));
-        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nb, K, st.sums);
+        EDX_TRY(edx::bspmm(q->h, q->precision, K, v[curr], q->x32.as<float2>(), e, &nb));
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nb, K, st.sums);
         hipLaunchKernelGGL(k_bk_alpha, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
         ELAUNCH(k_baxpy_norm, n, K, n, K, v[prev], v[curr], st.alpha + (size_t)(i - 1) * K,
-                part.as<double2>());
-        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, st.sums);
+                q->part.as<double2>());
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nbe, K, st.sums);
         hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
         ELAUNCH(k_bdiv_beta, n, K, n, K, v[prev], st.beta + (size_t)i * K, st.run + (size_t)i * K);
         EDX_LAUNCH_CHECK();
-        std::swap(curr, prev);
-        if (i - checked >= 32 && i < maxn) {
+        std::swap(q->curr, q->prev);
+        q->done = i;
+        if (i - checked >= 32 || i == end) {
             std::vector<int> r(K);
             EDX_HIP(hipMemcpy(r.data(), st.run + (size_t)i * K, K * sizeof(int), hipMemcpyDeviceToHost));
             checked = i;
             bool any = false;
             for (int k = 0; k < K; ++k) any |= r[k] != 0;
-            if (!any) break;
+            if (!any) {
+                q->live = false;
+                break;
+            }
         }
     }
-    std::vector<double> ha((size_t)maxn * K), hb((size_t)(maxn + 1) * K), hn(K);
+    return EDX_OK;
+}
+
+// alpha / beta of the first count steps, nrhs x count row-major (beta_1..beta_count)
+int kb_read(const edx_krylov_batch *q, int count, int *neff, double *alpha, double *beta,
+            double *norm) {
+    const int K = q->K;
+    const int c = std::min(count, q->done);
+    std::vector<double> ha((size_t)std::max(c, 1) * K), hb((size_t)(c + 1) * K), hn(K);
     std::vector<int> hneff(K);
-    EDX_HIP(hipMemcpy(ha.data(), st.alpha, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
-    EDX_HIP(hipMemcpy(hb.data(), st.beta, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const BKrylov &st = q->st;
+    if (c > 0) {
+        EDX_HIP(hipMemcpy(ha.data(), st.alpha, (size_t)c * K * sizeof(double), hipMemcpyDeviceToHost));
+        EDX_HIP(hipMemcpy(hb.data(), st.beta, (size_t)(c + 1) * K * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    }
     EDX_HIP(hipMemcpy(hn.data(), st.norm, K * sizeof(double), hipMemcpyDeviceToHost));
     EDX_HIP(hipMemcpy(hneff.data(), st.neff, K * sizeof(int), hipMemcpyDeviceToHost));
     for (int k = 0; k < K; ++k) {
-        neff[k] = hneff[k];
-        norm[k] = hn[k];
-        for (int i = 0; i < maxn; ++i) {
-            alpha[(size_t)k * maxn + i] = ha[(size_t)i * K + k];
-            beta[(size_t)k * maxn + i] = hb[(size_t)(i + 1) * K + k];
+        if (neff) neff[k] = std::min(hneff[k], count);
+        if (norm) norm[k] = hn[k];
+        for (int i = 0; i < count; ++i) {
+            const bool have = i < c;
+            if (alpha) alpha[(size_t)k * count + i] = have ? ha[(size_t)i * K + k] : 0.0;
+            if (beta) beta[(size_t)k * count + i] = have ? hb[(size_t)(i + 1) * K + k] : 0.0;
         }
     }
     return EDX_OK;
@@ -900,12 +1005,52 @@ int edx_build_krylov_batch(const edx_csr *h, int precision, int nrhs, const void
               "bad argument");
     EDX_CHECK(nrhs <= 4096, EDX_ERR_ARG, "at most 4096 right-hand sides per call");
     EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix must be square");
-    EDX_TRY(edx::ensure_precision(const_cast<edx_csr *>(h), precision));
-    int rc = bkrylov_run(h, precision, nrhs, (const double2 *)d_V0, maxn, neff, alpha, beta, norm);
+    edx_krylov_batch *q = nullptr;
+    EDX_TRY(edx_krylov_batch_start(h, precision, nrhs, d_V0, maxn, &q));
+    int rc = kb_advance(q, maxn);
+    if (!rc) rc = kb_read(q, maxn, neff, alpha, beta, norm);
+    edx_krylov_batch_free(q);
     if (rc) return rc;
     EDX_HIP(hipStreamSynchronize(edx::stream()));
     return EDX_OK;
 }
+
+int edx_krylov_batch_start(const edx_csr *h, int precision, int nrhs, const void *d_V0,
+                           int capacity, edx_krylov_batch **out) {
+    EDX_CHECK(out, EDX_ERR_ARG, "null output handle");
+    *out = nullptr;
+    EDX_CHECK(h && d_V0 && nrhs >= 1, EDX_ERR_ARG, "bad argument");
+    EDX_CHECK(nrhs <= 4096, EDX_ERR_ARG, "at most 4096 right-hand sides per call");
+    EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix must be square");
+    EDX_TRY(edx::ensure_precision(const_cast<edx_csr *>(h), precision));
+    edx_krylov_batch *q = new edx_krylov_batch;
+    q->h = h;
+    q->precision = precision;
+    q->K = nrhs;
+    q->n = h->n;
+    int rc = kb_start(q, (const double2 *)d_V0, std::max(capacity, 16));
+    if (rc) {
+        delete q;
+        return rc;
+    }
+    *out = q;
+    return EDX_OK;
+}
+
+int edx_krylov_batch_advance(edx_krylov_batch *q, int nsteps, int *steps_done) {
+    EDX_CHECK(q && nsteps >= 0, EDX_ERR_ARG, "bad argument");
+    EDX_TRY(kb_advance(q, nsteps));
+    if (steps_done) *steps_done = q->done;
+    return EDX_OK;
+}
+
+int edx_krylov_batch_read(const edx_krylov_batch *q, int count, int *neff, double *alpha,
+                          double *beta, double *norm) {
+    EDX_CHECK(q && count >= 0, EDX_ERR_ARG, "bad argument");
+    return kb_read(q, count, neff, alpha, beta, norm);
+}
+
+void edx_krylov_batch_free(edx_krylov_batch *q) { delete q; }
 
 int edx_minres_shifted_batch(const edx_csr *h, int precision, int nrhs, const double *sigma,
                              const void *d_B, void *d_X, double tol, int maxit, int *iters,

@@ -12,6 +12,7 @@
 #include "basis.hip"
 #include "csr_build.hip"
 #include "spmv.hip"
+#include "autotune.hip"
 #include "blas.hip"
 #include "krylov.hip"
 #include "batch.hip"

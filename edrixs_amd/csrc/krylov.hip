@@ -58,9 +58,9 @@ int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int
     DevBuf vb, x32, part, st, ab, runf;
     EDX_TRY(vb.alloc(2 * n * sizeof(double2), "krylov vectors"));
     if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "krylov x32"));
-    int nbs = edx::spmv_blocks(h, 64);
     int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc((size_t)std::max(nbs, nbv) * 4 * sizeof(double2) + 64, "partials"));
+    EDX_TRY(part.alloc((size_t)std::max<int64_t>(edx::spmv_partials_len(h), nbv) * sizeof(double2),
+                       "partials"));
     EDX_TRY(st.alloc(sizeof(KrylovDev), "krylov state"));
     EDX_TRY(ab.alloc((2 * (size_t)maxn + 2) * sizeof(double), "alpha beta"));
     EDX_TRY(runf.alloc(((size_t)maxn + 2) * sizeof(int), "run flags"));
@@ -86,7 +86,8 @@ int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int
         SpmvEpi e{};
         e.y = v[prev];
         e.z = v[prev];
-        e.coef = S->coef;
+        e.c0 = &S->coef[0];
+        e.c1 = &S->coef[1];
         e.dvec = v[curr];
         e.partials = part.as<double2>();
         e.run = run + (i - 1);
@@ -125,9 +126,9 @@ int minres_run(const edx_csr *a, int precision, const double2 *b, double2 *x, do
     DevBuf vb, x32, part, st, runf;
     EDX_TRY(vb.alloc(5 * n * sizeof(double2), "minres vectors"));
     if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "minres x32"));
-    int nbs = edx::spmv_blocks(a, 64);
     int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc((size_t)std::max(nbs, nbv) * 4 * sizeof(double2) + 64, "partials"));
+    EDX_TRY(part.alloc((size_t)std::max<int64_t>(edx::spmv_partials_len(a), nbv) * sizeof(double2),
+                       "partials"));
     EDX_TRY(st.alloc(sizeof(MinresDev), "minres state"));
     const int nflags = (maxit > 0 ? maxit : 0) + 2;
     EDX_TRY(runf.alloc((size_t)nflags * sizeof(int), "run flags"));

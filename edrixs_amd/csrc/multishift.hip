@@ -177,9 +177,9 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
     DevBuf vb, x32, part, st, ab, runf, shb, ctlb, qb;
     EDX_TRY(vb.alloc(2 * (size_t)n * sizeof(double2), "fom lanczos vectors"));
     if (precision == EDX_C64) EDX_TRY(x32.alloc((size_t)n * sizeof(float2), "fom x32"));
-    const int nbs = edx::spmv_blocks(h, 64);
     const int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc((size_t)std::max(nbs, nbv) * 4 * sizeof(double2) + 64, "partials"));
+    EDX_TRY(part.alloc((size_t)std::max<int64_t>(edx::spmv_partials_len(h), nbv) * sizeof(double2),
+                       "partials"));
     EDX_TRY(st.alloc(sizeof(KrylovDev), "fom krylov state"));
     EDX_TRY(ab.alloc((2 * (size_t)maxit + 2) * sizeof(double), "alpha beta"));
     EDX_TRY(runf.alloc(((size_t)maxit + 2) * sizeof(int), "run flags"));
@@ -233,7 +233,8 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         SpmvEpi e{};
         e.y = v[prev];
         e.z = v[prev];
-        e.coef = S->coef;
+        e.c0 = &S->coef[0];
+        e.c1 = &S->coef[1];
         e.dvec = v[curr];
         e.partials = part.as<double2>();
         e.run = run + (i - 1);

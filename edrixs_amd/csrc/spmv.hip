@@ -42,7 +42,8 @@ struct AccT<float2> {
 struct SpmvEpi {
     double2 *y;
     const double2 *z;       // may alias y; nullptr -> no addend
-    const double2 *coef;    // coef[0] scales A x, coef[1] scales z; nullptr -> 1, 1
+    const double2 *c0;      // scales A x; nullptr -> 1
+    const double2 *c1;      // scales z; nullptr -> 1
     const double2 *dvec;    // nullptr -> no dot
     double2 *partials;      // one per block when dvec != nullptr
     const int *run;         // nullptr or flag: skip the launch when *run == 0
@@ -84,11 +85,8 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
     const int lig = threadIdx.x & (G - 1);
     const int64_t group = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
     const int64_t ngroups = (int64_t)gridDim.x * (256 / G);
-    double2 c0 = make_double2(1.0, 0.0), c1 = make_double2(1.0, 0.0);
-    if (e.coef) {
-        c0 = e.coef[0];
-        c1 = e.coef[1];
-    }
+    const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
+    const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
     double2 dot = make_double2(0.0, 0.0);
     for (int64_t r = group; r < m; r += ngroups) {
         const int64_t b = rowptr[r], end = rowptr[r + 1];
@@ -193,6 +191,27 @@ __global__ void k_to_c64_vec(int64_t n, const double2 *in, float2 *out, const in
         out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
+// Per-block dot partials of a launch with more than FOLD_N blocks are folded
+// to FOLD_N sums (block j adds the j-th contiguous chunk in a fixed order) so
+// the one-block scalar kernels that consume them stay short.
+constexpr int FOLD_N = 256;
+
+__global__ __launch_bounds__(256) void k_fold_partials(double2 *partials, int nb, const int *run) {
+    if (run && *run == 0) return;
+    const int chunk = (nb + FOLD_N - 1) / FOLD_N;
+    const int lo = blockIdx.x * chunk;
+    const int hi = min(nb, lo + chunk);
+    double2 s = make_double2(0, 0);
+    for (int i = lo + threadIdx.x; i < hi; i += 256) s = cadd(s, partials[i]);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) partials[nb + blockIdx.x] = s;
+}
+
+__global__ void k_move_folded(double2 *partials, int nb, const int *run) {
+    if (run && *run == 0) return;
+    if (threadIdx.x < FOLD_N) partials[threadIdx.x] = partials[nb + threadIdx.x];
+}
+
 int lanes_for_avg(double avg) {
     if (avg < 12) return 4;
     if (avg < 40) return 8;
@@ -215,7 +234,8 @@ namespace edx {
 
 // ---------------------------------------------------------------- profiling
 struct ProfRec {
-    int kind;
+    int kind;      // precision
+    int kernel;    // EDX_KERNEL_*
     double bytes;
 };
 static bool g_prof = false;
@@ -223,10 +243,10 @@ static std::vector<hipEvent_t> g_ev;      // pairs: 2k start, 2k+1 stop
 static std::vector<ProfRec> g_rec;
 constexpr size_t PROF_CAP = 16384;
 
-static hipEvent_t prof_start(int kind, double bytes) {
+static hipEvent_t prof_start(int kind, int kernel, double bytes) {
     if (!g_prof || g_rec.size() >= PROF_CAP) return nullptr;
     if (g_ev.size() < 2 * PROF_CAP) return nullptr;
-    g_rec.push_back(ProfRec{kind, bytes});
+    g_rec.push_back(ProfRec{kind, kernel, bytes});
     hipEvent_t e = g_ev[2 * (g_rec.size() - 1)];
     (void)hipEventRecord(e, stream());
     return e;
@@ -253,14 +273,19 @@ double spmv_bytes(const edx_csr *a, int precision, int nrhs, bool z, bool dvec) 
     return spmv_bytes_k(a, precision, nrhs, z, dvec);
 }
 
+// One lane group per row and no grid-stride cap: blocks are dispatched in row
+// order, so the rows in flight form one sliding window and their gathers of x
+// share cache lines (a capped grid-stride grid lets the blocks drift apart and
+// was 30-40 % slower on H_n at norb=24).
 int spmv_blocks(const edx_csr *a, int G) {
-    int64_t groups_needed = a->m;
-    int64_t blocks = (groups_needed * G + 255) / 256;
-    int64_t cap = (int64_t)num_cus() * 8;
-    if (blocks > cap) blocks = cap;
+    int64_t blocks = (a->m * G + 255) / 256;
     if (blocks < 1) blocks = 1;
     return (int)blocks;
 }
+
+// Length of a partials buffer that serves every single-vector SpMV launch of
+// the matrix: one entry per block of the widest launch plus the folded sums.
+int64_t spmv_partials_len(const edx_csr *a) { return (int64_t)spmv_blocks(a, 64) + FOLD_N + 64; }
 
 // Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
 // C64D and complex64 (already converted) for C64.  Returns the block count used.
@@ -278,7 +303,8 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     int nb = spmv_blocks(a, G);
     if (nblocks_out) *nblocks_out = nb;
     if (a->m == 0) return EDX_OK;
-    hipEvent_t pe = prof_start(precision, spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
+    hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
+                               spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
 #define EDX_SPMV_CASE(GG)                                                                     \
     case GG:                                                                                  \
         if (precision == EDX_C128)                                                            \
@@ -305,6 +331,13 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
 #undef EDX_SPMV_CASE
     EDX_LAUNCH_CHECK();
     prof_stop(pe);
+    if (e.dvec && nb > FOLD_N) {
+        hipLaunchKernelGGL(k_fold_partials, dim3(FOLD_N), dim3(256), 0, stream(), e.partials, nb,
+                           e.run);
+        hipLaunchKernelGGL(k_move_folded, dim3(1), dim3(FOLD_N), 0, stream(), e.partials, nb, e.run);
+        EDX_LAUNCH_CHECK();
+        if (nblocks_out) *nblocks_out = FOLD_N;
+    }
     return EDX_OK;
 }
 
@@ -326,7 +359,8 @@ int spmm_launch(const edx_csr *a, int precision, int nrhs, const void *X, double
     int64_t cap = (int64_t)num_cus() * 8;
     if (blocks > cap) blocks = cap;
     dim3 grid((unsigned)blocks, (unsigned)((nrhs + NR - 1) / NR));
-    hipEvent_t pe = prof_start(precision, spmv_bytes(a, precision, nrhs, Z != nullptr, false));
+    hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMM_CM,
+                               spmv_bytes(a, precision, nrhs, Z != nullptr, false));
 #define EDX_SPMM_CASE(GG)                                                                       \
     case GG:                                                                                    \
         if (precision == EDX_C128)                                                              \
@@ -369,12 +403,14 @@ int edx_profile(int enable) {
     return EDX_OK;
 }
 
-int edx_profile_read(int kind, int64_t *launches, double *total_ms, double *total_bytes) {
+int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
+                     double *total_bytes) {
     EDX_HIP(hipStreamSynchronize(edx::stream()));
     int64_t n = 0;
     double ms = 0.0, by = 0.0;
     for (size_t i = 0; i < edx::g_rec.size(); ++i) {
-        if (edx::g_rec[i].kind != kind) continue;
+        if (edx::g_rec[i].kind != precision) continue;
+        if (kernel >= 0 && edx::g_rec[i].kernel != kernel) continue;
         float t = 0.f;
         EDX_HIP(hipEventElapsedTime(&t, edx::g_ev[2 * i], edx::g_ev[2 * i + 1]));
         ++n;

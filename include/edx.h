@@ -116,14 +116,29 @@ int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y);
  * dimensions m and n.  Same precision rules as edx_spmv. */
 int edx_spmm(const edx_csr *a, int precision, int nrhs, const void *d_X, void *d_Y);
 
-/* Live profiling of SpMV/SpMM launches: while enabled, every launch is
- * bracketed by HIP events on its stream and its algorithmic HBM bytes
- *   nnz*(s_v + s_i) + (m+1)*s_p + n*s_x + m*s_y*(reads + writes of y-like vectors)
+/* Lanes per row of the single-vector SpMV of one precision: set = 4..64
+ * fixes it, set = 0 only reads it; *lanes = 0 means the mean-row-length rule
+ * (matrices under 131072 rows, or before the first edx_csr_prepare, which
+ * times the choices on larger matrices).  No reference counterpart. */
+int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes);
+
+/* Live profiling of SpMV/SpMM launches (no reference counterpart): while
+ * enabled, every launch is bracketed by HIP events on its stream and its
+ * algorithmic HBM bytes
+ *   nnz*(s_v + s_i) + (m+1)*s_p + K*(n*s_x + m*s_y*(reads + writes of y-like vectors))
  * are recorded.  edx_profile(1) clears and starts, edx_profile(0) stops;
  * edx_profile_read synchronises and returns the launch count, summed kernel
- * milliseconds and summed bytes (kind 0: complex128 values, 1: complex64). */
+ * milliseconds and summed bytes of the launches in one precision
+ * (edx_precision) and one kernel (edx_kernel, or -1 for all). */
+enum edx_kernel {
+    EDX_KERNEL_SPMV = 0,      /* k_spmv: one vector */
+    EDX_KERNEL_SPMM_IL = 1,   /* k_spmm_il: K < 8 interleaved right-hand sides */
+    EDX_KERNEL_SPMM_RL = 2,   /* k_spmm_rl: K >= 8, RHS-parallel lanes */
+    EDX_KERNEL_SPMM_CM = 3    /* k_spmm: column-major blocks (edx_spmm) */
+};
 int edx_profile(int enable);
-int edx_profile_read(int kind, int64_t *launches, double *total_ms, double *total_bytes);
+int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
+                     double *total_bytes);
 
 /* --------------------------------------------------------------- solvers */
 
@@ -172,6 +187,23 @@ int edx_build_krylov_batch(const edx_csr *h, int precision, int nrhs, const void
  * final_res[k] = |(sigma_k - H) x_k - b_k|.  This replaces the reference's
  * one-fsolver-call-per-(omega_in, polarization) loop with one shifted solve of
  * the unshifted H_n (edrixs/solvers.py:2651-2693, src/rixs_driver.f90:130-150). */
+/* The same batched recurrence, resumable: start normalizes the nrhs columns
+ * of d_V0 (n x nrhs column-major; copied, so the caller's buffer may go) and
+ * keeps vectors, flags and the (alpha, beta) history on the device; advance
+ * takes up to nsteps more steps of every recurrence that has not broken down
+ * (steps_done = total steps so far); read returns the first count steps in
+ * the layout of edx_build_krylov_batch (zeros past a recurrence's end).  A
+ * caller stops once the continued fractions it evaluates have converged,
+ * instead of fixing nkryl up front as build_krylov_mp does
+ * (src/lanczos.f90:266-298).  h must outlive the state. */
+typedef struct edx_krylov_batch edx_krylov_batch;
+int edx_krylov_batch_start(const edx_csr *h, int precision, int nrhs, const void *d_V0,
+                           int capacity, edx_krylov_batch **out);
+int edx_krylov_batch_advance(edx_krylov_batch *state, int nsteps, int *steps_done);
+int edx_krylov_batch_read(const edx_krylov_batch *state, int count, int *neff, double *alpha,
+                          double *beta, double *norm);
+void edx_krylov_batch_free(edx_krylov_batch *state);
+
 /* Multi-shift Galerkin solve of (sigma_k I - H) x_k = b for nshift shifts from
  * ONE Lanczos recurrence on H, keeping only projections: for each of nproj
  * operators P_p (a CSR with H's row count as its column count, or NULL for the

@@ -592,3 +592,58 @@ def test_shifted_fom_edge_cases(core):
     for k in range(3):
         assert it[k] == 1
         np.testing.assert_allclose(X.cpu().numpy()[k], ev / (sig[k] - w), atol=1e-12)
+
+
+# ------------------------------------------------- resumable Krylov, converged CF
+
+def test_krylov_batch_resumes_bit_exact(core):
+    m, fn, bn, A, R = _siam_hn(core)
+    rng = np.random.default_rng(31)
+    V0 = rng.normal(size=(3, A.n)) + 1j * rng.normal(size=(3, A.n))
+    Vd = _to_dev(core, V0)
+    al, be, nr = core.build_krylov_batch(A, Vd, 90)
+    kb = core.KrylovBatch(A, Vd, capacity=16)      # grows its history twice
+    assert kb.advance(20) == 20 and kb.advance(1) == 21 and kb.advance(69) == 90
+    a2, b2, n2 = kb.coefficients()
+    for k in range(3):
+        assert np.array_equal(a2[k], al[k]) and np.array_equal(b2[k], be[k]) and n2[k] == nr[k]
+    a3, b3, _ = kb.coefficients(50)
+    for k in range(3):
+        assert np.array_equal(a3[k], al[k][:50]) and np.array_equal(b3[k], be[k][:50])
+    kb.close()
+
+
+def test_converged_spectra_equal_pole_sum(core):
+    m = H.small_siam(nbath=2, seed=8)
+    fi = H.fock_words(m.norb, m.nocc)
+    A = core.build_ham_i(core.Basis.from_words(fi, m.norb), m.hop_i, m.coul_i)
+    Hd = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    rng = np.random.default_rng(5)
+    Phi = rng.normal(size=(2, A.n)) + 1j * rng.normal(size=(2, A.n))
+    e, v = np.linalg.eigh(Hd)
+    egs = e[0]
+    om = np.linspace(0.0, 10.0, 401)
+    gam = 0.2
+    for prec, tol, bound in (("c128", 1e-10, 1e-8), ("c64d", 1e-10, 1e-6)):
+        spec, steps, change = core.converged_spectra(A, _to_dev(core, Phi), egs, om, gam,
+                                                      precision=prec, tol=tol, chunk=40,
+                                                      min_steps=40, max_steps=4000)
+        for k in range(2):
+            w = np.abs(v.conj().T @ Phi[k]) ** 2
+            z = om + 1j * gam + egs
+            exact = -1 / np.pi * np.imag((w[None, :] / (z[:, None] - e[None, :])).sum(1))
+            assert change[k] <= tol and 0 < steps[k] < 4000
+            assert np.linalg.norm(spec[k] - exact) < bound * np.linalg.norm(exact), (prec, k)
+
+
+def test_converged_spectra_split_equals_block(core, monkeypatch):
+    # forcing the one-recurrence-per-vector schedule changes no number
+    m, fn, bn, A, R = _siam_hn(core)
+    rng = np.random.default_rng(17)
+    Phi = _to_dev(core, rng.normal(size=(3, A.n)) + 1j * rng.normal(size=(3, A.n)))
+    om = np.linspace(-20.0, 20.0, 201)
+    a = core.converged_spectra(A, Phi, 0.0, om, 0.5, tol=1e-9, chunk=30, min_steps=30)
+    monkeypatch.setattr(core, "SPLIT_BLOCK_BYTES", 0)
+    b = core.converged_spectra(A, Phi, 0.0, om, 0.5, tol=1e-9, chunk=30, min_steps=30)
+    for k in range(3):
+        assert np.array_equal(a[0][k], b[0][k]) and a[1][k] == b[1][k]
