@@ -64,11 +64,14 @@ def parse():
     p.add_argument("--cf-chunk", type=int, default=250)
     p.add_argument("--nkryl-max", type=int, default=5000)
     p.add_argument("--seed", type=int, default=2024)
-    p.add_argument("--forms", default="c128,c64d,c64")
+    p.add_argument("--forms", default="c64d,c128")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--exact", action="store_true",
                    help="also compare with the fully reorthogonalized FP64 projection")
     p.add_argument("--cpu-rows", type=int, default=40000)
+    p.add_argument("--plain-pairs", type=int, default=4,
+                   help="RIXS (w_in, pol) pairs the plain FP64 reference procedure is timed on "
+                        "(0: skip it; 2*n_in: all)")
     p.add_argument("--probe", action="store_true",
                    help="run only a short fixed set of SpMV/SpMM launches (for counter runs)")
     p.add_argument("--probe-form", default="c64d")
@@ -165,6 +168,76 @@ class Workload:
         out = dict(egs=egs, xas=xas, xas_comp=xas_comp, rixs=rixs, info=info, gs=gs)
         del Hi, Hn, D, Din, Dout, Phi, proj, Phf, vecs
         return out
+
+
+def plain_fp64(work, ref, pairs):
+    """The reference's procedure on the GPU in complex128, one (w_in, pol) at a
+    time as its driver runs (src/rixs_driver.f90:127-218): rebuild
+    w_in + E_gs + i Gamma_c - H_n, MINRES to the same relative residual, apply
+    D_out, then one final-state Krylov recurrence to a converged fraction; XAS
+    as three single-vector recurrences.  ``pairs`` of the 2 n_in RIXS pairs
+    (spread over the incident energies) are run and timed one by one; the RIXS
+    time is scaled to all pairs.  Their spectra are compared with the batched
+    multi-shift form's."""
+    import torch
+    c, m, a = work.core, work.model, work.args
+    T = {}
+    t0 = time.perf_counter()
+    Hi = c.build_ham_i(work.bi, m.hop_i, m.coul_i)
+    Hn = c.build_ham_n(work.bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    D = [c.build_transop_i(work.bn, work.bi, m.norb, m.ncore, m.trans_i[k]) for k in range(3)]
+    torch.cuda.synchronize()
+    T["build"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ev, vecs, it = c.ground_state(Hi, nev=1, precision="c128", maxiter=400, tol=1e-10)
+    egs = float(ev[0])
+    gs = vecs[0].contiguous()
+    torch.cuda.synchronize()
+    T["ground_state"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(3):
+        phi = D[k].matvec(gs).reshape(1, -1).contiguous()
+        c.converged_spectra(Hn, phi, egs, work.omega, work.GAMMA_C, precision="c128",
+                            tol=a.cf_tol, chunk=a.cf_chunk, min_steps=a.nkryl,
+                            max_steps=a.nkryl_max)
+    torch.cuda.synchronize()
+    T["xas"] = time.perf_counter() - t0
+    del Hn
+    npol = len(work.POLS)
+    allp = [(i, p) for i in range(len(work.w_in)) for p in range(npol)]
+    pick = sorted(set(int(round(v)) for v in np.linspace(0, len(allp) - 1, pairs)))
+    pair_s, iters, diffs = [], [], []
+    for q in pick:
+        i, p = allp[q]
+        t0 = time.perf_counter()
+        sig = work.w_in[i] + egs + 1j * work.GAMMA_C
+        A = c.build_ham_n(work.bn, m.norb, m.ncore, m.hop_n, m.coul_n, omega=sig, ham_sign=-1.0)
+        Din = c.build_transop_i(work.bn, work.bi, m.norb, m.ncore, work.pol_terms[p][0])
+        Dout = c.build_transop_f(work.bi, work.bn, m.norb, m.ncore, work.pol_terms[p][1])
+        b = Din.matvec(gs)
+        bn = float(torch.linalg.vector_norm(b))
+        x, nit, res = c.minres(A, b, tol=a.rixs_tol * bn, maxit=a.rixs_maxit)
+        del A
+        phi = Dout.matvec(x).reshape(1, -1).contiguous()
+        spec, _, _ = c.converged_spectra(Hi, phi, egs, work.eloss, work.GAMMA_F, precision="c128",
+                                         tol=a.cf_tol, chunk=a.cf_chunk, min_steps=a.nkryl,
+                                         max_steps=a.nkryl_max)
+        torch.cuda.synchronize()
+        pair_s.append(time.perf_counter() - t0)
+        iters.append(int(nit))
+        diffs.append(rel(spec[0], ref["rixs"][q]))
+        log("plain fp64 pair %d (w_in %.2f, pol %d): %.2f s, %d MINRES iterations" %
+            (q, work.w_in[i], p, pair_s[-1], nit))
+    t_rixs = float(np.mean(pair_s)) * len(allp)
+    total = T["build"] + T["ground_state"] + T["xas"] + t_rixs
+    nspec = 1 + len(allp)
+    return {"spectra_per_s": nspec / total, "seconds_per_step": total, "phases_s": T,
+            "rixs_pairs_timed": len(pick), "rixs_pairs_total": len(allp),
+            "rixs_pair_seconds": pair_s, "minres_iters": iters,
+            "rixs_rel_l2_vs_multishift_c128_max": max(diffs),
+            "note": ("builds, ground state and XAS timed in full; the RIXS time is the mean of "
+                     "the timed pairs times the number of pairs" if len(pick) < len(allp)
+                     else "every pair timed")}
 
 
 def timed(work, precision, steps, warmup):
@@ -272,12 +345,26 @@ def cpu_baseline(work, info, rows):
             "seconds_per_step": total}
 
 
+def measured_traffic(kernel, form, norb):
+    """HBM bytes per launch of a kernel as two rocprofv3 counter runs measured
+    it (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
+    from the summaries under profiles/; None when none covers this workload."""
+    import glob
+    for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        with open(fn) as f:
+            for line in f:
+                t = line.split()
+                if len(t) == 5 and t[0] == "traffic:" and t[1:4] == [kernel, form, "norb%d" % norb]:
+                    return float(t[4])
+    return None
+
+
 def probe(args):
-    """A short fixed workload for hardware-counter runs: the dominant SpMV/SpMM
-    launches of the step at full size (single-vector split SpMV over H_n, the
-    K=3 XAS batch over H_n, the K=16 final-state batch over H_i), each
-    ``--launches`` times, plus a plain device copy of 1 GiB as the calibration
-    kernel of known traffic."""
+    """A short fixed workload for hardware-counter runs: the SpMV/SpMM launches
+    of the step at full size (single-vector SpMV over H_n with the Krylov
+    epilogue, the K=3 block over H_n, the K=16 final-state block over H_i),
+    each ``--launches`` times, plus a plain device copy of 1 GiB as the
+    calibration kernel of known traffic."""
     import torch
     c = __import__("edrixs_amd.core", fromlist=["core"])
     from edrixs_amd import siam
@@ -292,17 +379,17 @@ def probe(args):
     g = torch.Generator(device="cuda").manual_seed(1)
     def rnd(*shape):
         return torch.randn(*shape, dtype=torch.complex128, device="cuda", generator=g)
-    x, y = rnd(Hn.n), rnd(Hn.n)
     src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
     log("probe: copy done")
-    for _ in range(args.launches):
-        Hn.spmv(x, y, precision=f)
+    kb = c.KrylovBatch(Hn, rnd(1, Hn.n), f, capacity=args.launches + 1)   # k_spmv, Krylov epilogue
+    kb.advance(args.launches)
+    kb.close()
     torch.cuda.synchronize()
-    log("probe: %d k_spmv launches on H_n done" % args.launches)
+    log("probe: %d k_spmv Krylov steps on H_n done" % args.launches)
     kb = c.KrylovBatch(Hn, rnd(3, Hn.n), f, capacity=args.launches + 1)
     kb.advance(args.launches)
     kb.close()
@@ -397,7 +484,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": measured_traffic(kname, main_form, m.norb),
             "kernel": "%s (%s form), every launch of the timed region" % (kname, main_form),
             "launches": n, "avg_us": ms / max(n, 1) * 1e3, "bytes_per_launch": by / max(n, 1),
         },
@@ -405,6 +493,10 @@ def main():
     }
     if "c128" in results:
         line["speedup_vs_fp64"] = times["c128"] / t_main
+        if args.plain_pairs > 0:
+            pl = plain_fp64(work, results["c128"], args.plain_pairs)
+            line["plain_fp64"] = pl
+            line["speedup_vs_plain_fp64"] = line["value"] / pl["spectra_per_s"]
     if args.exact and "c128" in results:
         line["accuracy_vs_exact_projection"] = exact_reference(work, results["c128"]["gs"], results)
     if not args.no_cpu:
