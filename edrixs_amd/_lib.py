@@ -69,6 +69,13 @@ _SIGS = {
     "edx_krylov_batch_advance": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "edx_krylov_batch_read": (_I, [_P, _I, _P, _P, _P, _P]),
     "edx_krylov_batch_free": (None, [_P]),
+    "edx_group_create": (_I, [_I, _I, _PP]),
+    "edx_group_add": (_I, [_P, _P, _P, ctypes.POINTER(_I)]),
+    "edx_group_remove": (_I, [_P, _I]),
+    "edx_group_advance": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "edx_group_read": (_I, [_P, _I, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(_D),
+                            ctypes.POINTER(_I)]),
+    "edx_group_free": (None, [_P]),
     "edx_shifted_fom_project": (_I, [_P, _I, _P, _I, _P, _I, _PP, _PP, _D, _I, _P, _P]),
     "edx_csr_free": (None, [_P]),
     "edx_csr_spmv_lanes": (_I, [_P, _I, _I, ctypes.POINTER(_I)]),

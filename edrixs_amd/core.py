@@ -383,6 +383,66 @@ class KrylovBatch:
         self.close()
 
 
+class KrylovGroup:
+    """Independent single-vector Krylov recurrences over different matrices,
+    advanced together (edx_group_*): one SpMV launch per step serves every
+    member.  Members are added and removed between advances; a member's
+    coefficients do not depend on the other members."""
+
+    def __init__(self, precision="c64d", max_steps=6000):
+        self.max_steps = int(max_steps)
+        self._h = None
+        self._mats = {}                 # member -> matrix (kept alive while a member)
+        h = ctypes.c_void_p()
+        _lib.call("edx_group_create", _prec(precision), self.max_steps, ctypes.byref(h))
+        self._h = h
+
+    def add(self, H, v0):
+        _check_vec(v0, H.n)
+        mid = ctypes.c_int()
+        _lib.call("edx_group_add", self._h, H._h, _dptr(v0), ctypes.byref(mid))
+        self._mats[mid.value] = H
+        return mid.value
+
+    def remove(self, member):
+        _lib.call("edx_group_remove", self._h, int(member))
+        self._mats.pop(member, None)
+
+    def advance(self, nsteps):
+        """nsteps steps of every running member; returns how many still run."""
+        live = ctypes.c_int()
+        _lib.call("edx_group_advance", self._h, int(nsteps), ctypes.byref(live))
+        return live.value
+
+    def coefficients(self, member):
+        """(alpha, beta, norm, running) of a member so far."""
+        steps = ctypes.c_int()
+        norm = ctypes.c_double()
+        run = ctypes.c_int()
+        _lib.call("edx_group_read", self._h, int(member), ctypes.byref(steps), None, None,
+                  ctypes.byref(norm), ctypes.byref(run))
+        a = np.zeros(max(steps.value, 1))
+        b = np.zeros(max(steps.value, 1))
+        _lib.call("edx_group_read", self._h, int(member), ctypes.byref(steps), _ptr(a), _ptr(b),
+                  ctypes.byref(norm), ctypes.byref(run))
+        n = steps.value
+        return a[:n].copy(), b[:n].copy(), norm.value, bool(run.value)
+
+    @property
+    def members(self):
+        return list(self._mats)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None and _lib._lib is not None:
+            _lib._lib.edx_group_free(h)
+        self._h = None
+        self._mats = {}
+
+    def __del__(self):
+        self.close()
+
+
 # A block of K interleaved vectors larger than this runs as K single-vector
 # recurrences instead (each vector then stays in the 256 MB last-level cache).
 SPLIT_BLOCK_BYTES = 192 << 20

@@ -17,4 +17,5 @@
 #include "krylov.hip"
 #include "batch.hip"
 #include "multishift.hip"
+#include "group.hip"
 #include "observables.hip"

@@ -1,0 +1,423 @@
+// Grouped Krylov recurrences across matrices (part of edx_kernels.hip).
+//
+// A serving loop holds many small, independent Krylov recurrences at once:
+// the XAS components and RIXS final states of every admitted request, each
+// over its own request's matrix, started and finished at different times.
+// Launched one by one, each of their products is a few microseconds of work
+// behind a launch; here every step of every live member is ONE SpMV launch,
+// one vector launch and three tiny reductions, whatever the number of members.
+//
+// Each member is a build_krylov_mp recurrence (src/lanczos.f90:266-298) on one
+// vector.  Its rows map to a contiguous range of the SpMV grid (32 rows per
+// block, 8 lanes per row) and its elements to a contiguous range of the vector
+// grid; a block finds its member by binary search over the range starts.  All
+// reductions run over the member's own blocks in a fixed order, so a member's
+// numbers are the same whichever other members share the launches: a request
+// gets the same spectrum alone and in any mix.
+//
+// Members live in device memory with their (alpha, beta) history; the host
+// adds and removes members between advances and reads the coefficients.
+
+namespace {
+
+constexpr int GG = 8;                    // lanes per row
+constexpr int G_ROWS = 256 / GG;         // rows per SpMV block
+constexpr int G_VEC = 1024;              // vector elements per vector block
+
+struct GState {
+    double2 c1;      // -beta of the last step (addend scale of the next product)
+    double norm;     // <v0|v0>
+    int run;         // recurrence still going
+    int steps;       // steps done; parity selects the current vector
+};
+
+struct GMember {
+    const int64_t *rowptr;
+    const int32_t *cols;
+    const void *vals;        // double2 (complex128 form) or float2 (split form off-diagonal)
+    const double2 *diag;     // split form diagonal
+    int64_t m;
+    double2 *v0, *v1;
+    double *alpha, *beta;    // [cap] each
+    GState *st;
+    int64_t blk0, nblk;      // SpMV blocks
+    int64_t vblk0, nvblk;    // vector blocks
+};
+
+__device__ __forceinline__ int g_find(const GMember *mem, int nmem, int64_t bid, bool vec) {
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t start = vec ? mem[mid].vblk0 : mem[mid].blk0;
+        if (start <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class VT, bool SPLIT>
+__global__ __launch_bounds__(256) void k_gspmv(const GMember *mem, int nmem, double2 *partials) {
+    const int64_t bid = blockIdx.x;
+    const GMember M = mem[g_find(mem, nmem, bid, false)];
+    const GState st = *M.st;
+    if (!st.run) {   // uniform over the block
+        if (threadIdx.x == 0) partials[bid] = make_double2(0, 0);
+        return;
+    }
+    const double2 *x = (st.steps & 1) ? M.v1 : M.v0;
+    double2 *y = (st.steps & 1) ? M.v0 : M.v1;
+    const VT *vals = (const VT *)M.vals;
+    const int lig = threadIdx.x & (GG - 1);
+    const int64_t r = (bid - M.blk0) * G_ROWS + threadIdx.x / GG;
+    double2 dot = make_double2(0, 0);
+    if (r < M.m) {
+        double sr = 0, si = 0;
+        for (int64_t j = M.rowptr[r] + lig; j < M.rowptr[r + 1]; j += GG) {
+            const VT a = nt_load2(vals + j);
+            const double2 xv = x[nt_load(M.cols + j)];
+            sr += (double)a.x * xv.x - (double)a.y * xv.y;
+            si += (double)a.x * xv.y + (double)a.y * xv.x;
+        }
+#pragma unroll
+        for (int off = GG / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, GG);
+            si += __shfl_xor(si, off, GG);
+        }
+        if (lig == 0) {
+            double2 ax = make_double2(sr, si);
+            if constexpr (SPLIT) ax = cadd(ax, cmul(M.diag[r], x[r]));
+            const double2 out = cadd(ax, cmul(st.c1, y[r]));
+            y[r] = out;
+            dot = cconjmul(x[r], out);
+        }
+    }
+    dot = block_sum_256(dot);
+    if (threadIdx.x == 0) partials[bid] = dot;
+}
+
+// alpha_i = Re <x|y> over the member's SpMV blocks (src/lanczos.f90:277-281)
+__global__ __launch_bounds__(256) void k_galpha(const GMember *mem, const double2 *partials) {
+    const GMember M = mem[blockIdx.x];
+    if (!M.st->run) return;
+    double2 s = make_double2(0, 0);
+    for (int64_t b = threadIdx.x; b < M.nblk; b += 256) s = cadd(s, partials[M.blk0 + b]);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) M.alpha[M.st->steps] = s.x;
+}
+
+// y -= alpha x; partial |y|^2 per vector block
+__global__ __launch_bounds__(256) void k_gaxpy(const GMember *mem, int nmem, double2 *vpartials) {
+    const int64_t bid = blockIdx.x;
+    const GMember M = mem[g_find(mem, nmem, bid, true)];
+    const GState st = *M.st;
+    if (!st.run) {
+        if (threadIdx.x == 0) vpartials[bid] = make_double2(0, 0);
+        return;
+    }
+    const double2 *x = (st.steps & 1) ? M.v1 : M.v0;
+    double2 *y = (st.steps & 1) ? M.v0 : M.v1;
+    const double al = M.alpha[st.steps];
+    const int64_t base = (bid - M.vblk0) * G_VEC;
+    double2 s = make_double2(0, 0);
+    for (int t = threadIdx.x; t < G_VEC; t += 256) {
+        const int64_t i = base + t;
+        if (i < M.m) {
+            const double2 v = make_double2(y[i].x - al * x[i].x, y[i].y - al * x[i].y);
+            y[i] = v;
+            s.x += v.x * v.x + v.y * v.y;
+        }
+    }
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) vpartials[bid] = s;
+}
+
+// beta_i = |y|; breakdown below 1e-10 (src/lanczos.f90:285-293) or a full
+// history ends the member; the step counter advances, flipping the vectors
+__global__ __launch_bounds__(256) void k_gbeta(const GMember *mem, const double2 *vpartials, int cap) {
+    const GMember M = mem[blockIdx.x];
+    if (!M.st->run) return;
+    double2 s = make_double2(0, 0);
+    for (int64_t b = threadIdx.x; b < M.nvblk; b += 256) s = cadd(s, vpartials[M.vblk0 + b]);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) {
+        GState st = *M.st;
+        const double b = sqrt(s.x);
+        M.beta[st.steps] = b;
+        st.c1 = make_double2(-b, -0.0);
+        st.steps += 1;
+        st.run = (fabs(b) >= 1e-10 && st.steps < cap) ? 1 : 0;
+        *M.st = st;
+    }
+}
+
+// the new current vector /= beta (members still running)
+__global__ __launch_bounds__(256) void k_gdiv(const GMember *mem, int nmem) {
+    const int64_t bid = blockIdx.x;
+    const GMember M = mem[g_find(mem, nmem, bid, true)];
+    const GState st = *M.st;
+    if (!st.run) return;
+    double2 *v = (st.steps & 1) ? M.v1 : M.v0;
+    const double b = M.beta[st.steps - 1];
+    const int64_t base = (bid - M.vblk0) * G_VEC;
+    for (int t = threadIdx.x; t < G_VEC; t += 256) {
+        const int64_t i = base + t;
+        if (i < M.m) v[i] = make_double2(v[i].x / b, v[i].y / b);
+    }
+}
+
+// member start: norm = <v0|v0> from dot partials, v0 /= sqrt(norm)
+__global__ __launch_bounds__(256) void k_ginit(const double2 *partials, int nb, GState *st) {
+    double2 s = make_double2(0, 0);
+    for (int b = threadIdx.x; b < nb; b += 256) s = cadd(s, partials[b]);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) {
+        GState g;
+        g.c1 = make_double2(-0.0, -0.0);
+        g.norm = s.x;
+        g.run = s.x > 0.0 ? 1 : 0;
+        g.steps = 0;
+        *st = g;
+    }
+}
+
+__global__ void k_gnormalize(int64_t n, double2 *v, const GState *st) {
+    const double nr = st->norm;
+    if (!(nr > 0.0)) return;
+    const double d = sqrt(nr);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = make_double2(v[i].x / d, v[i].y / d);
+}
+
+struct GroupMem {
+    int id;
+    const edx_csr *h;
+    double2 *v;        // 2 n
+    double *ab;        // alpha [cap], beta [cap]
+    GState *st;
+};
+
+}  // namespace
+
+struct edx_group {
+    int precision = 0;
+    int cap = 0;
+    int next_id = 1;
+    std::vector<GroupMem> mems;
+    bool dirty = true;
+    GMember *d_mem = nullptr;
+    size_t d_mem_cap = 0;
+    double2 *partials = nullptr;
+    size_t partials_cap = 0;
+    double2 *vpartials = nullptr;
+    size_t vpartials_cap = 0;
+    int64_t nblk = 0, nvblk = 0;
+    ~edx_group() {
+        for (auto &m : mems) {
+            edx::dfree(m.v);
+            edx::dfree(m.ab);
+            edx::dfree(m.st);
+        }
+        edx::dfree(d_mem);
+        edx::dfree(partials);
+        edx::dfree(vpartials);
+    }
+};
+
+namespace {
+
+int grow(void **p, size_t *cap, size_t need, const char *what) {
+    if (need <= *cap) return EDX_OK;
+    size_t nc = std::max(need, *cap * 2);
+    void *q = edx::dmalloc(nc, what);
+    if (!q) return EDX_ERR_NOMEM;
+    edx::dfree(*p);
+    *p = q;
+    *cap = nc;
+    return EDX_OK;
+}
+
+int group_layout(edx_group *g) {
+    if (!g->dirty) return EDX_OK;
+    std::vector<GMember> hm(g->mems.size());
+    int64_t blk = 0, vblk = 0;
+    for (size_t j = 0; j < g->mems.size(); ++j) {
+        const GroupMem &m = g->mems[j];
+        const edx_csr *h = m.h;
+        GMember &d = hm[j];
+        const bool split = g->precision == EDX_C64D;
+        d.rowptr = split ? h->d_off_rowptr : h->d_rowptr;
+        d.cols = split ? h->d_off_cols : h->d_cols;
+        d.vals = split ? (const void *)h->d_off_vals32 : (const void *)h->d_vals;
+        d.diag = split ? h->d_diag : nullptr;
+        d.m = h->m;
+        d.v0 = m.v;
+        d.v1 = m.v + h->n;
+        d.alpha = m.ab;
+        d.beta = m.ab + g->cap;
+        d.st = m.st;
+        d.nblk = (h->m + G_ROWS - 1) / G_ROWS;
+        d.nvblk = (h->m + G_VEC - 1) / G_VEC;
+        if (d.nblk < 1) d.nblk = 1;
+        if (d.nvblk < 1) d.nvblk = 1;
+        d.blk0 = blk;
+        d.vblk0 = vblk;
+        blk += d.nblk;
+        vblk += d.nvblk;
+    }
+    g->nblk = blk;
+    g->nvblk = vblk;
+    EDX_TRY(grow((void **)&g->d_mem, &g->d_mem_cap, std::max<size_t>(hm.size(), 1) * sizeof(GMember),
+                 "group members"));
+    EDX_TRY(grow((void **)&g->partials, &g->partials_cap, (size_t)std::max<int64_t>(blk, 1) * sizeof(double2),
+                 "group partials"));
+    EDX_TRY(grow((void **)&g->vpartials, &g->vpartials_cap,
+                 (size_t)std::max<int64_t>(vblk, 1) * sizeof(double2), "group partials"));
+    if (!hm.empty())
+        EDX_HIP(hipMemcpy(g->d_mem, hm.data(), hm.size() * sizeof(GMember), hipMemcpyHostToDevice));
+    g->dirty = false;
+    return EDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_group_create(int precision, int max_steps, edx_group **out) {
+    EDX_CHECK(out, EDX_ERR_ARG, "null output handle");
+    *out = nullptr;
+    EDX_CHECK(precision == EDX_C128 || precision == EDX_C64D, EDX_ERR_ARG,
+              "grouped recurrences run in complex128 or the split complex64 form (got %d)",
+              precision);
+    EDX_CHECK(max_steps >= 1, EDX_ERR_ARG, "max_steps %d < 1", max_steps);
+    edx_group *g = new edx_group;
+    g->precision = precision;
+    g->cap = max_steps;
+    *out = g;
+    return EDX_OK;
+}
+
+int edx_group_add(edx_group *g, const edx_csr *h, const void *d_v0, int *member) {
+    EDX_CHECK(g && h && d_v0 && member, EDX_ERR_ARG, "null argument");
+    EDX_CHECK(h->m == h->n && h->m > 0, EDX_ERR_ARG, "member matrix must be square and non-empty");
+    EDX_TRY(edx::ensure_precision(const_cast<edx_csr *>(h), g->precision));
+    GroupMem m{};
+    m.id = g->next_id;
+    m.h = h;
+    m.v = (double2 *)edx::dmalloc((size_t)2 * h->n * sizeof(double2), "group vectors");
+    m.ab = (double *)edx::dmalloc((size_t)2 * g->cap * sizeof(double), "group coefficients");
+    m.st = (GState *)edx::dmalloc(sizeof(GState), "group state");
+    if (!m.v || !m.ab || !m.st) {
+        edx::dfree(m.v);
+        edx::dfree(m.ab);
+        edx::dfree(m.st);
+        return EDX_ERR_NOMEM;
+    }
+    hipStream_t s = edx::stream();
+    const int64_t n = h->n;
+    const int nbv = vec_blocks(n);
+    double2 *part = (double2 *)edx::dmalloc((size_t)nbv * sizeof(double2), "group init partials");
+    if (!part) {
+        edx::dfree(m.v);
+        edx::dfree(m.ab);
+        edx::dfree(m.st);
+        return EDX_ERR_NOMEM;
+    }
+    int rc = EDX_OK;
+    if (hipMemcpyAsync(m.v, d_v0, n * sizeof(double2), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemsetAsync(m.v + n, 0, n * sizeof(double2), s) != hipSuccess ||
+        hipMemsetAsync(m.ab, 0, (size_t)2 * g->cap * sizeof(double), s) != hipSuccess) {
+        rc = edx::set_error(EDX_ERR_HIP, "group member upload failed");
+    } else {
+        hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, m.v, m.v, part, nullptr);
+        hipLaunchKernelGGL(k_ginit, dim3(1), dim3(256), 0, s, part, nbv, m.st);
+        hipLaunchKernelGGL(k_gnormalize, dim3(nbv), dim3(256), 0, s, n, m.v, m.st);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            rc = edx::set_error(EDX_ERR_HIP, "group member start failed");
+    }
+    edx::dfree(part);
+    if (rc) {
+        edx::dfree(m.v);
+        edx::dfree(m.ab);
+        edx::dfree(m.st);
+        return rc;
+    }
+    g->mems.push_back(m);
+    g->next_id++;
+    g->dirty = true;
+    *member = m.id;
+    return EDX_OK;
+}
+
+int edx_group_remove(edx_group *g, int member) {
+    EDX_CHECK(g, EDX_ERR_ARG, "null group");
+    for (size_t j = 0; j < g->mems.size(); ++j) {
+        if (g->mems[j].id != member) continue;
+        EDX_HIP(hipStreamSynchronize(edx::stream()));
+        edx::dfree(g->mems[j].v);
+        edx::dfree(g->mems[j].ab);
+        edx::dfree(g->mems[j].st);
+        g->mems.erase(g->mems.begin() + j);
+        g->dirty = true;
+        return EDX_OK;
+    }
+    return edx::set_error(EDX_ERR_ARG, "no member %d in the group", member);
+}
+
+int edx_group_advance(edx_group *g, int nsteps, int *live) {
+    EDX_CHECK(g && nsteps >= 0, EDX_ERR_ARG, "bad argument");
+    EDX_TRY(group_layout(g));
+    const int nmem = (int)g->mems.size();
+    hipStream_t s = edx::stream();
+    int nlive = nmem;
+    for (int i = 0; i < nsteps && nmem > 0 && nlive > 0; ++i) {
+        if (g->precision == EDX_C64D)
+            hipLaunchKernelGGL((k_gspmv<float2, true>), dim3((unsigned)g->nblk), dim3(256), 0, s,
+                               g->d_mem, nmem, g->partials);
+        else
+            hipLaunchKernelGGL((k_gspmv<double2, false>), dim3((unsigned)g->nblk), dim3(256), 0, s,
+                               g->d_mem, nmem, g->partials);
+        hipLaunchKernelGGL(k_galpha, dim3(nmem), dim3(256), 0, s, g->d_mem, g->partials);
+        hipLaunchKernelGGL(k_gaxpy, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem,
+                           g->vpartials);
+        hipLaunchKernelGGL(k_gbeta, dim3(nmem), dim3(256), 0, s, g->d_mem, g->vpartials, g->cap);
+        hipLaunchKernelGGL(k_gdiv, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem);
+        EDX_LAUNCH_CHECK();
+        if ((i + 1) % 32 == 0 || i + 1 == nsteps) {   // stop launching once every member has ended
+            nlive = 0;
+            for (const auto &m : g->mems) {
+                GState st;
+                EDX_HIP(hipMemcpy(&st, m.st, sizeof(GState), hipMemcpyDeviceToHost));
+                nlive += st.run;
+            }
+        }
+    }
+    if (live) *live = nlive;
+    return EDX_OK;
+}
+
+int edx_group_read(const edx_group *g, int member, int *steps, double *alpha, double *beta,
+                   double *norm, int *running) {
+    EDX_CHECK(g, EDX_ERR_ARG, "null group");
+    for (const auto &m : g->mems) {
+        if (m.id != member) continue;
+        GState st;
+        EDX_HIP(hipMemcpy(&st, m.st, sizeof(GState), hipMemcpyDeviceToHost));
+        if (steps) *steps = st.steps;
+        if (norm) *norm = st.norm;
+        if (running) *running = st.run;
+        if (alpha && st.steps)
+            EDX_HIP(hipMemcpy(alpha, m.ab, st.steps * sizeof(double), hipMemcpyDeviceToHost));
+        if (beta && st.steps)
+            EDX_HIP(hipMemcpy(beta, m.ab + g->cap, st.steps * sizeof(double), hipMemcpyDeviceToHost));
+        return EDX_OK;
+    }
+    return edx::set_error(EDX_ERR_ARG, "no member %d in the group", member);
+}
+
+void edx_group_free(edx_group *g) {
+    if (g) (void)hipStreamSynchronize(edx::stream());
+    delete g;
+}
+
+}  // extern "C"

@@ -204,6 +204,26 @@ int edx_krylov_batch_read(const edx_krylov_batch *state, int count, int *neff, d
                           double *beta, double *norm);
 void edx_krylov_batch_free(edx_krylov_batch *state);
 
+/* Grouped recurrences for a serving loop (no reference counterpart): many
+ * independent single-vector build_krylov_mp recurrences (src/lanczos.f90:
+ * 266-298), each over its own square matrix, advanced together so that one
+ * step of every live member is one SpMV launch plus four small launches.
+ * Precision EDX_C128 or EDX_C64D for every member; each member keeps up to
+ * max_steps (alpha, beta).  add copies and normalizes d_v0 (length rows(h))
+ * and returns a member id; h must stay alive while it is a member.  advance
+ * takes nsteps steps of every member still running (live = members still
+ * running).  read returns a member's steps so far, alpha_1..steps,
+ * beta_1..steps (pole-file convention), <v0|v0> and whether it is still
+ * running.  A member's numbers do not depend on the other members. */
+typedef struct edx_group edx_group;
+int edx_group_create(int precision, int max_steps, edx_group **out);
+int edx_group_add(edx_group *g, const edx_csr *h, const void *d_v0, int *member);
+int edx_group_remove(edx_group *g, int member);
+int edx_group_advance(edx_group *g, int nsteps, int *live);
+int edx_group_read(const edx_group *g, int member, int *steps, double *alpha, double *beta,
+                   double *norm, int *running);
+void edx_group_free(edx_group *g);
+
 /* Multi-shift Galerkin solve of (sigma_k I - H) x_k = b for nshift shifts from
  * ONE Lanczos recurrence on H, keeping only projections: for each of nproj
  * operators P_p (a CSR with H's row count as its column count, or NULL for the

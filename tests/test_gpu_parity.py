@@ -647,3 +647,64 @@ def test_converged_spectra_split_equals_block(core, monkeypatch):
     b = core.converged_spectra(A, Phi, 0.0, om, 0.5, tol=1e-9, chunk=30, min_steps=30)
     for k in range(3):
         assert np.array_equal(a[0][k], b[0][k]) and a[1][k] == b[1][k]
+
+
+# ----------------------------------------------------------- grouped recurrences
+
+@pytest.mark.parametrize("precision", ["c128", "c64d"])
+def test_group_members_match_oracle_and_are_mix_independent(core, precision):
+    m1, fn1, bn1, A1, R1 = _siam_hn(core, nbath=4, seed=2)
+    m2 = H.small_siam(nbath=3, seed=5)
+    f2 = H.fock_words(m2.norb, m2.nocc)
+    A2 = core.build_ham_i(core.Basis.from_words(f2, m2.norb), m2.hop_i, m2.coul_i)
+    R2 = o.build_ham_i(f2, m2.hop_i, m2.coul_i)
+    rng = np.random.default_rng(41)
+    va = rng.normal(size=A1.n) + 1j * rng.normal(size=A1.n)
+    vb = rng.normal(size=A2.n) + 1j * rng.normal(size=A2.n)
+    vc = rng.normal(size=A1.n) + 0j
+    # member a alone
+    g1 = core.KrylovGroup(precision, max_steps=200)
+    a = g1.add(A1, _to_dev(core, va))
+    g1.advance(60)
+    al1, be1, n1, run1 = g1.coefficients(a)
+    assert len(al1) == 60 and run1
+    if precision == "c128":
+        ra, rb, rn = o.build_krylov(R1, va, 60)
+        np.testing.assert_allclose(al1[:20], ra[:20], rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(be1[:20], rb[:20], rtol=1e-9, atol=1e-9)
+        assert abs(n1 - rn) <= 1e-12 * rn
+    # the same member in a changing mix: others join and leave mid-way
+    g2 = core.KrylovGroup(precision, max_steps=200)
+    b = g2.add(A2, _to_dev(core, vb))
+    g2.advance(7)
+    a2 = g2.add(A1, _to_dev(core, va))
+    g2.advance(20)
+    c = g2.add(A1, _to_dev(core, vc))
+    g2.advance(15)
+    g2.remove(b)
+    g2.advance(25)
+    al2, be2, n2, _ = g2.coefficients(a2)
+    assert np.array_equal(al2, al1) and np.array_equal(be2, be1) and n2 == n1
+    alb, beb, nb_, _ = g2.coefficients(c)
+    assert len(alb) == 40
+    if precision == "c128":
+        rb2 = o.build_krylov(R1, vc, 40)
+        np.testing.assert_allclose(alb[:20], rb2[0][:20], rtol=1e-9, atol=1e-9)
+    g1.close()
+    g2.close()
+
+
+def test_group_breakdown_and_capacity(core):
+    m, fn, bn, A, R = _siam_hn(core, nbath=2, seed=4)
+    ev = np.linalg.eigh(R.todense())[1][:, 3]
+    g = core.KrylovGroup("c128", max_steps=10)
+    e = g.add(A, _to_dev(core, ev))                     # eigenvector: breaks down after one step
+    r = g.add(A, _to_dev(core, np.ones(A.n) + 0j))      # runs into the 10-step capacity
+    live = g.advance(30)
+    ae, be, ne, rune = g.coefficients(e)
+    ar, br, nr, runr = g.coefficients(r)
+    assert live == 0 and not rune and not runr
+    assert len(ae) == 1 and be[0] < 1e-10 and len(ar) == 10
+    with pytest.raises(Exception, match="no member"):
+        g.remove(12345)
+    g.close()
