@@ -69,9 +69,13 @@ def parse():
     p.add_argument("--exact", action="store_true",
                    help="also compare with the fully reorthogonalized FP64 projection")
     p.add_argument("--cpu-rows", type=int, default=40000)
+    p.add_argument("--serve", type=int, default=12,
+                   help="requests in the serving-loop measurement (0: skip it)")
+    p.add_argument("--serve-budget-gb", type=float, default=96.0)
     p.add_argument("--plain-pairs", type=int, default=4,
                    help="RIXS (w_in, pol) pairs the plain FP64 reference procedure is timed on "
                         "(0: skip it; 2*n_in: all)")
+    p.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
     p.add_argument("--probe", action="store_true",
                    help="run only a short fixed set of SpMV/SpMM launches (for counter runs)")
     p.add_argument("--probe-form", default="c64d")
@@ -345,6 +349,28 @@ def cpu_baseline(work, info, rows):
             "seconds_per_step": total}
 
 
+def serve(args):
+    """The serving form: ``--serve`` requests with Poisson arrivals (20 per
+    second), norb drawn from {16, 18, 20, 22, 24}, N = norb / 2, per-request
+    model parameters, half XAS (isotropic, 3 components) and half RIXS (4
+    incident energies, one polarization pair), run by the continuous-batching
+    engine in c64d with spectra converged to 1e-6.  Reports latency
+    percentiles beside spectra/s."""
+    from edrixs_amd import serving
+    reqs = serving.make_requests(args.serve, rate=20.0, sizes=(16, 18, 20, 22, 24),
+                                 rixs_fraction=0.5, seed=args.seed)
+    eng = serving.Engine(budget_gb=args.serve_budget_gb, precision="c64d", check_every=100,
+                         chunk=25, cf_tol=1e-6, min_steps=200, max_steps=5000, n_in=4, npol=1,
+                         nw=1000, rixs_tol=1e-6, rixs_maxit=5000)
+    done = eng.run(reqs, realtime=True, log=log)
+    rep = serving.report(done)
+    rep.update({"arrival_rate_per_s": 20.0, "sizes_norb": [16, 18, 20, 22, 24],
+                "hbm_budget_gb": args.serve_budget_gb, "precision": "c64d",
+                "kinds": {k: sum(r.kind == k for r in done) for k in ("xas", "rixs")},
+                "norb_of_requests": [r.norb for r in sorted(done, key=lambda r: r.rid)]})
+    return rep
+
+
 def measured_traffic(kernel, form, norb):
     """HBM bytes per launch of a kernel as two rocprofv3 counter runs measured
     it (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
@@ -415,6 +441,9 @@ def main():
     torch.zeros(1, device="cuda")
     if args.probe:
         probe(args)
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": serve(args)}))
         return
     work = Workload(args)
     forms = args.forms.split(",")
@@ -499,6 +528,8 @@ def main():
             line["speedup_vs_plain_fp64"] = line["value"] / pl["spectra_per_s"]
     if args.exact and "c128" in results:
         line["accuracy_vs_exact_projection"] = exact_reference(work, results["c128"]["gs"], results)
+    if args.serve > 0:
+        line["serving"] = serve(args)
     if not args.no_cpu:
         cb = cpu_baseline(work, results[main_form]["info"], args.cpu_rows)
         line["cpu_baseline"] = cb

@@ -69,6 +69,12 @@ _SIGS = {
     "edx_krylov_batch_advance": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "edx_krylov_batch_read": (_I, [_P, _I, _P, _P, _P, _P]),
     "edx_krylov_batch_free": (None, [_P]),
+    "edx_mf_ham_i": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _D, _PP]),
+    "edx_mf_ham_n": (_I, [_P, _I, _I, _P, _I64, _P, _I64, _D, _D, _D, _PP]),
+    "edx_mf_size": (_I, [_P, ctypes.POINTER(_I64)]),
+    "edx_mf_apply": (_I, [_P, _P, _P]),
+    "edx_build_krylov_mf": (_I, [_P, _P, _I, ctypes.POINTER(_I), _P, _P, ctypes.POINTER(_D)]),
+    "edx_mf_free": (None, [_P]),
     "edx_group_create": (_I, [_I, _I, _PP]),
     "edx_group_add": (_I, [_P, _P, _P, ctypes.POINTER(_I)]),
     "edx_group_remove": (_I, [_P, _I]),

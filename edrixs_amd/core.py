@@ -383,6 +383,64 @@ class KrylovBatch:
         self.close()
 
 
+class MatrixFreeOp:
+    """A Hamiltonian applied on the fly (edx_mf_*): the CSR build's terms, signs
+    and columns, without storing the matrix.  Keeps its basis alive."""
+
+    def __init__(self, handle, basis):
+        self._h = handle
+        self._basis = basis
+        n = ctypes.c_int64()
+        _lib.call("edx_mf_size", handle, ctypes.byref(n))
+        self.m = self.n = n.value
+
+    def apply(self, x, y):
+        """y += H x for complex128 device tensors."""
+        _check_vec(x, self.n)
+        _check_vec(y, self.m)
+        _lib.call("edx_mf_apply", self._h, _dptr(x), _dptr(y))
+        return y
+
+    def matvec(self, x):
+        y = torch().zeros(self.m, dtype=torch().complex128, device=x.device)
+        return self.apply(x, y)
+
+    def build_krylov(self, v0, maxn):
+        """build_krylov_mp over the operator: alpha, beta (pole-file convention), norm."""
+        _check_vec(v0, self.n)
+        alpha = np.zeros(maxn)
+        beta = np.zeros(maxn)
+        neff = ctypes.c_int()
+        norm = ctypes.c_double()
+        _lib.call("edx_build_krylov_mf", self._h, _dptr(v0), int(maxn), ctypes.byref(neff),
+                  _ptr(alpha), _ptr(beta), ctypes.byref(norm))
+        k = neff.value
+        return alpha[:k].copy(), beta[:k].copy(), norm.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None and _lib._lib is not None:
+            _lib._lib.edx_mf_free(h)
+        self._h = None
+
+
+def mf_ham_i(basis, hop, coul, omega=0j, ham_sign=1.0):
+    h2, h4 = as_terms2(hop), as_terms4(coul)
+    h = ctypes.c_void_p()
+    _lib.call("edx_mf_ham_i", basis._h, _ptr(h2), len(h2), _ptr(h4), len(h4),
+              float(np.real(omega)), float(np.imag(omega)), float(ham_sign), ctypes.byref(h))
+    return MatrixFreeOp(h, basis)
+
+
+def mf_ham_n(basis_n, num_val_orbs, num_core_orbs, hop, coul, omega=0j, ham_sign=1.0):
+    h2, h4 = as_terms2(hop), as_terms4(coul)
+    h = ctypes.c_void_p()
+    _lib.call("edx_mf_ham_n", basis_n._h, int(num_val_orbs), int(num_core_orbs), _ptr(h2),
+              len(h2), _ptr(h4), len(h4), float(np.real(omega)), float(np.imag(omega)),
+              float(ham_sign), ctypes.byref(h))
+    return MatrixFreeOp(h, basis_n)
+
+
 class KrylovGroup:
     """Independent single-vector Krylov recurrences over different matrices,
     advanced together (edx_group_*): one SpMV launch per step serves every
@@ -580,7 +638,7 @@ def profile(enable=True):
     _lib.call("edx_profile", int(bool(enable)))
 
 
-KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3}
+KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4}
 
 
 def profile_read(precision="c128", kernel=None):

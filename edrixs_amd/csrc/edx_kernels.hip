@@ -15,6 +15,7 @@
 #include "autotune.hip"
 #include "blas.hip"
 #include "krylov.hip"
+#include "mf.hip"
 #include "batch.hip"
 #include "multishift.hip"
 #include "group.hip"

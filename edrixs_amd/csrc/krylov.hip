@@ -52,14 +52,16 @@ int apply(const edx_csr *h, int precision, const double2 *x, float2 *x32, const 
     return edx::spmv_launch(h, precision, x, e, 0, nb);
 }
 
-int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int *neff,
-               double *alpha, double *beta, double *norm) {
-    const int64_t n = h->n;
-    DevBuf vb, x32, part, st, ab, runf;
+// build_krylov_mp (src/lanczos.f90:210-303) over any product
+// prod(x, epilogue, &nblocks) that writes y = c0 A x + c1 z with dot partials;
+// partials_len bounds the partials one product writes.
+template <class Prod>
+int krylov_loop(int64_t n, Prod prod, int64_t partials_len, const double2 *v0, int maxn,
+                int *neff, double *alpha, double *beta, double *norm) {
+    DevBuf vb, part, st, ab, runf;
     EDX_TRY(vb.alloc(2 * n * sizeof(double2), "krylov vectors"));
-    if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "krylov x32"));
     int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc((size_t)std::max<int64_t>(edx::spmv_partials_len(h), nbv) * sizeof(double2),
+    EDX_TRY(part.alloc((size_t)std::max<int64_t>(partials_len, nbv) * sizeof(double2),
                        "partials"));
     EDX_TRY(st.alloc(sizeof(KrylovDev), "krylov state"));
     EDX_TRY(ab.alloc((2 * (size_t)maxn + 2) * sizeof(double), "alpha beta"));
@@ -92,7 +94,7 @@ int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int
         e.partials = part.as<double2>();
         e.run = run + (i - 1);
         int nb = 0;
-        EDX_TRY(apply(h, precision, v[curr], x32.as<float2>(), e, &nb, run + (i - 1)));
+        EDX_TRY(prod((const double2 *)v[curr], e, &nb, run + (i - 1)));
         hipLaunchKernelGGL(k_krylov_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
                            d_alpha, i, run);
         hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
@@ -118,6 +120,17 @@ int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int
     *norm = hs.norm;
     for (int i = 0; i < maxn; ++i) beta[i] = hb[i + 1];   // beta_1 .. beta_maxn
     return EDX_OK;
+}
+
+int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int *neff,
+               double *alpha, double *beta, double *norm) {
+    DevBuf x32;
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(h->n * sizeof(float2), "krylov x32"));
+    float2 *x32p = x32.as<float2>();
+    auto prod = [&](const double2 *x, const SpmvEpi &e, int *nb, const int *run) {
+        return apply(h, precision, x, x32p, e, nb, run);
+    };
+    return krylov_loop(h->n, prod, edx::spmv_partials_len(h), v0, maxn, neff, alpha, beta, norm);
 }
 
 int minres_run(const edx_csr *a, int precision, const double2 *b, double2 *x, double tol, int maxit,

@@ -269,6 +269,9 @@ double spmv_bytes_k(const edx_csr *a, int precision, int K, bool z, bool dvec) {
     return (double)a->nnz * (sv + 4.0) + (double)(a->m + 1) * 8.0 + K * ((double)a->n * sv + yv);
 }
 
+// Algorithmic bytes of a matrix-free apply: x once, y written, the addend read.
+double mf_bytes(int64_t m, int64_t n) { return (double)n * 16.0 + (double)m * 32.0; }
+
 double spmv_bytes(const edx_csr *a, int precision, int nrhs, bool z, bool dvec) {
     return spmv_bytes_k(a, precision, nrhs, z, dvec);
 }

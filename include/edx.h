@@ -134,11 +134,36 @@ enum edx_kernel {
     EDX_KERNEL_SPMV = 0,      /* k_spmv: one vector */
     EDX_KERNEL_SPMM_IL = 1,   /* k_spmm_il: K < 8 interleaved right-hand sides */
     EDX_KERNEL_SPMM_RL = 2,   /* k_spmm_rl: K >= 8, RHS-parallel lanes */
-    EDX_KERNEL_SPMM_CM = 3    /* k_spmm: column-major blocks (edx_spmm) */
+    EDX_KERNEL_SPMM_CM = 3,   /* k_spmm: column-major blocks (edx_spmm) */
+    EDX_KERNEL_MF = 4         /* k_mf: matrix-free apply (recorded as precision EDX_C128) */
 };
 int edx_profile(int enable);
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
                      double *total_bytes);
+
+/* ------------------------------------------------- matrix-free operators */
+
+/* H applied on the fly, without storing it (no reference counterpart; the
+ * semantics are those of build_ham_i / build_ham_n, src/ham.f90:2-711, followed
+ * by matvec_csr, src/spmv.f90:73-103): every term is tested, signed and ranked
+ * exactly as the CSR build does, so columns and signs are the build's; only
+ * the order in which duplicate terms are summed differs.  The basis must
+ * outlive the operator.  Complex128 vectors, FP64 accumulation. */
+typedef struct edx_mfop edx_mfop;
+int edx_mf_ham_i(const edx_basis *fock, const edx_tensor2 *hop, int64_t nhopp,
+                 const edx_tensor4 *coul, int64_t ncoul, double omega_re, double omega_im,
+                 double ham_sign, edx_mfop **out);
+int edx_mf_ham_n(const edx_basis *fock_n, int num_val_orbs, int num_core_orbs,
+                 const edx_tensor2 *hop, int64_t nhopp, const edx_tensor4 *coul, int64_t ncoul,
+                 double omega_re, double omega_im, double ham_sign, edx_mfop **out);
+int edx_mf_size(const edx_mfop *op, int64_t *n);
+/* d_y += H d_x */
+int edx_mf_apply(const edx_mfop *op, const void *d_x, void *d_y);
+/* build_krylov_mp (src/lanczos.f90:210-303) over the matrix-free operator; same
+ * outputs as edx_build_krylov. */
+int edx_build_krylov_mf(const edx_mfop *op, const void *d_v0, int maxn, int *neff, double *alpha,
+                        double *beta, double *norm);
+void edx_mf_free(edx_mfop *op);
 
 /* --------------------------------------------------------------- solvers */
 

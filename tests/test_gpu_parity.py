@@ -708,3 +708,71 @@ def test_group_breakdown_and_capacity(core):
     with pytest.raises(Exception, match="no member"):
         g.remove(12345)
     g.close()
+
+
+# ------------------------------------------------------------ matrix-free apply
+
+def test_matrix_free_apply_matches_csr(core):
+    m = H.small_siam(nbath=4, seed=7)
+    nv, nc, N = m.norb, m.ncore, m.nocc
+    fi, fn = H.fock_words(nv, N), H.fock_words(nv, N + 1)
+    bi, bn = core.Basis.from_words(fi, nv), core.Basis.from_words(fn, nv)
+    rng = np.random.default_rng(13)
+    cases = [(core.mf_ham_n(bn, nv, nc, m.hop_n, m.coul_n),
+              o.build_ham_n(fn, nv, nc, m.hop_n, m.coul_n)),
+             (core.mf_ham_n(bn, nv, nc, m.hop_n, m.coul_n, omega=-3.25 + 0.5j, ham_sign=-1.0),
+              o.build_ham_n(fn, nv, nc, m.hop_n, m.coul_n, omega=-3.25 + 0.5j, ham_sign=-1.0)),
+             (core.mf_ham_i(bi, m.hop_i, m.coul_i), o.build_ham_i(fi, m.hop_i, m.coul_i))]
+    sub = fi[::2].copy()             # an incomplete basis: binary-search column lookup
+    cases.append((core.mf_ham_i(core.Basis.from_words(sub, nv), m.hop_i, m.coul_i),
+                  o.build_ham_i(sub, m.hop_i, m.coul_i)))
+    for mf, R in cases:
+        assert mf.n == R.m
+        x = rng.normal(size=R.m) + 1j * rng.normal(size=R.m)
+        y0 = rng.normal(size=R.m) + 1j * rng.normal(size=R.m)
+        y = mf.apply(_to_dev(core, x), _to_dev(core, y0)).cpu().numpy()
+        ref = o.matvec(R, x, y0)
+        scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
+        assert np.abs(y - ref).max() < 1e-14 * scale
+        # a column of H: columns and signs exactly as the CSR build places them
+        j = int(rng.integers(R.m))
+        col = mf.matvec(_to_dev(core, np.eye(1, R.m, j)[0] + 0j)).cpu().numpy()
+        dense_col = np.zeros(R.m, complex)
+        for r in range(R.m):
+            lo, hi = R.rowptr[r], R.rowptr[r + 1]
+            hit = np.nonzero(R.cols[lo:hi] == j)[0]
+            if len(hit):
+                dense_col[r] = R.vals[lo + hit[0]]
+        np.testing.assert_allclose(col, dense_col, rtol=1e-15, atol=1e-15 * np.abs(R.vals).max())
+
+
+def test_matrix_free_krylov_matches_csr(core):
+    m = H.small_siam(nbath=4, seed=2)
+    fn = H.fock_words(m.norb, m.nocc + 1)
+    bn = core.Basis.from_words(fn, m.norb)
+    mf = core.mf_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    R = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    phi = np.random.default_rng(8).normal(size=mf.n) + 0.5j
+    a, b, nrm = mf.build_krylov(_to_dev(core, phi), 60)
+    ra, rb, rn = o.build_krylov(R, phi, 60)
+    assert len(a) == 60 and abs(nrm - rn) <= 1e-12 * rn
+    np.testing.assert_allclose(a[:25], ra[:25], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(b[:25], rb[:25], rtol=1e-9, atol=1e-9)
+
+
+def test_matrix_free_rows_with_many_terms(core):
+    # dense random terms: rows have more surviving terms than one compaction pass holds
+    n, N = 14, 7
+    rng = np.random.default_rng(23)
+    from edrixs_amd import siam
+    ht = siam.emat_terms(H.random_hermitian_emat(n, rng, density=1.0))
+    ut = siam.umat_terms(H.random_hermitian_umat(n, rng, nnz=3000))
+    fock = H.fock_words(n, N)
+    R = o.build_ham_i(fock, ht, ut)
+    assert np.diff(R.rowptr).max() > 128
+    mf = core.mf_ham_i(core.Basis.from_words(fock, n), ht, ut)
+    x = rng.normal(size=R.m) + 1j * rng.normal(size=R.m)
+    y = mf.matvec(_to_dev(core, x)).cpu().numpy()
+    ref = o.matvec(R, x)
+    scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
+    assert np.abs(y - ref).max() < 1e-14 * scale
