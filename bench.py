@@ -76,6 +76,8 @@ def parse():
                    help="RIXS (w_in, pol) pairs the plain FP64 reference procedure is timed on "
                         "(0: skip it; 2*n_in: all)")
     p.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
+    p.add_argument("--config4", action="store_true",
+                   help="run only the norb=26 ground state + XAS + matrix-free measurement")
     p.add_argument("--probe", action="store_true",
                    help="run only a short fixed set of SpMV/SpMM launches (for counter runs)")
     p.add_argument("--probe-form", default="c64d")
@@ -371,6 +373,79 @@ def serve(args):
     return rep
 
 
+def config4(args):
+    """The largest single-GPU case: norb = 26 (10 d + 16 bath), N = 13.  The
+    ground state of H_i (10,400,600 rows) in c64d with FP64 refinement, its
+    energy error against an FP64 Lanczos run, isotropic XAS over H_n
+    (57,946,200 rows) to converged continued fractions, and the matrix-free
+    apply of the same H_n checked against the stored CSR and timed."""
+    import torch
+    c = __import__("edrixs_amd.core", fromlist=["core"])
+    from edrixs_amd import siam
+    out = {"workload": "norb=26 N=13 SIAM (10 d + 16 bath, 2p core)"}
+    m = siam.SiamModel(nbath=16, seed=args.seed)
+    t0 = time.perf_counter()
+    bi = c.Basis.combinations(m.norb, m.nocc)
+    bn = c.Basis.combinations(m.norb, m.nocc + 1)
+    Hi = c.build_ham_i(bi, m.hop_i, m.coul_i)
+    Hi.prepare("c64d")
+    torch.cuda.synchronize()
+    out["H_i"] = {"rows": Hi.m, "nnz": Hi.nnz, "build_s": time.perf_counter() - t0}
+    t0 = time.perf_counter()
+    ev64, _, it64 = c.ground_state(Hi, nev=1, precision="c128", maxiter=500, tol=1e-10)
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ev, vecs, it = c.ground_state(Hi, nev=1, precision="c64d", maxiter=500, tol=1e-10)
+    tmix = time.perf_counter() - t0
+    out["ground_state"] = {"E_fp64": float(ev64[0]), "E_c64d_refined": float(ev[0]),
+                           "abs_err_eV": abs(float(ev[0]) - float(ev64[0])),
+                           "fp64_s": t64, "c64d_s": tmix}
+    log("config4 ground state: %.10f (err %.2e)" % (ev[0], out["ground_state"]["abs_err_eV"]))
+    gs = vecs[0].contiguous()
+    egs = float(ev[0])
+    t0 = time.perf_counter()
+    Hn = c.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    Hn.prepare("c64d")
+    torch.cuda.synchronize()
+    out["H_n"] = {"rows": Hn.m, "nnz": Hn.nnz, "build_s": time.perf_counter() - t0}
+    D = [c.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[k]) for k in range(3)]
+    Phi = torch.stack([D[k].matvec(gs) for k in range(3)]).contiguous()
+    omega = np.linspace(-10.0, 30.0, args.nw)
+    c.profile(True)
+    t0 = time.perf_counter()
+    comp, steps, change = c.converged_spectra(Hn, Phi, egs, omega, 0.5, precision="c64d",
+                                              tol=args.cf_tol, chunk=args.cf_chunk,
+                                              min_steps=args.nkryl, max_steps=args.nkryl_max)
+    torch.cuda.synchronize()
+    txas = time.perf_counter() - t0
+    n, ms, by = c.profile_read("c64d", "k_spmv")
+    c.profile(False)
+    out["xas"] = {"seconds": txas, "krylov_steps": steps, "cf_change": change,
+                  "k_spmv_launches": n, "k_spmv_avg_us": ms / max(n, 1) * 1e3,
+                  "k_spmv_alg_gbs": by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0}
+    log("config4 XAS: %.1f s, steps %s" % (txas, steps))
+    # matrix-free apply of the same H_n
+    mf = c.mf_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    x = gs.new_zeros(Hn.n)
+    x.copy_(Phi[2])
+    y_csr = Hn.matvec(x)
+    y_mf = mf.matvec(x)
+    diff = float((y_mf - y_csr).abs().max() / y_csr.abs().max())
+    c.profile(True)
+    for _ in range(3):
+        mf.matvec(x)
+    nm, msm, bym = c.profile_read("c128", "k_mf")
+    c.profile(False)
+    csr_bytes = Hn.nnz * 20.0 + (Hn.m + 1) * 8.0 + Hn.n * 16.0 + Hn.m * 32.0
+    out["matrix_free"] = {"rows": mf.n, "avg_ms": msm / max(nm, 1),
+                          "alg_bytes": bym / max(nm, 1), "alg_gbs": bym / (msm * 1e-3) / 1e9,
+                          "stored_csr_equivalent_bytes": csr_bytes,
+                          "csr_equivalent_gbs": csr_bytes / (msm / max(nm, 1) * 1e-3) / 1e9,
+                          "rel_diff_vs_csr": diff}
+    log("config4 matrix-free apply: %.1f ms, rel diff vs CSR %.1e" % (msm / max(nm, 1), diff))
+    return out
+
+
 def measured_traffic(kernel, form, norb):
     """HBM bytes per launch of a kernel as two rocprofv3 counter runs measured
     it (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
@@ -444,6 +519,9 @@ def main():
         return
     if args.serve_only:
         print(json.dumps({"serving": serve(args)}))
+        return
+    if args.config4:
+        print(json.dumps({"config4": config4(args)}))
         return
     work = Workload(args)
     forms = args.forms.split(",")
