@@ -76,6 +76,8 @@ def parse():
                    help="RIXS (w_in, pol) pairs the plain FP64 reference procedure is timed on "
                         "(0: skip it; 2*n_in: all)")
     p.add_argument("--serve-only", action="store_true", help="run only the serving measurement")
+    p.add_argument("--config2", action="store_true",
+                   help="run only the 128-RHS batched RIXS comparison on the norb=20 model")
     p.add_argument("--config4", action="store_true",
                    help="run only the norb=26 ground state + XAS + matrix-free measurement")
     p.add_argument("--probe", action="store_true",
@@ -446,6 +448,109 @@ def config4(args):
     return out
 
 
+def config2(args):
+    """Batched RIXS on the norb=20 model: 64 incident energies over 10 eV x 2
+    polarizations = 128 shifted systems (w_in + E_gs + i Gamma_c - H_n) x =
+    D_pol|gs> to relative residual 1e-8, then final-state continued fractions
+    over H_i on 2000 loss points.  Three routes to the same 128 spectra:
+    (1) c64d multi-shift recurrences (one per polarization, projected by D_out),
+    (2) c64d batched MINRES over the interleaved SpMM (128 right-hand sides),
+    (3) the reference's per-RHS FP64 MINRES loop (edx_minres, one system at a
+    time).  Routes 1 and 2 are compared with route 3 spectrum by spectrum."""
+    import torch
+    c = __import__("edrixs_amd.core", fromlist=["core"])
+    from edrixs_amd import siam
+    m = siam.SiamModel(nbath=10, seed=args.seed)
+    bi = c.Basis.combinations(m.norb, m.nocc)
+    bn = c.Basis.combinations(m.norb, m.nocc + 1)
+    Hi = c.build_ham_i(bi, m.hop_i, m.coul_i)
+    Hn = c.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    ev, vecs, _ = c.ground_state(Hi, nev=1, precision="c128", maxiter=400, tol=1e-10)
+    egs = float(ev[0])
+    gs = vecs[0].contiguous()
+    e0n, _, _ = c.lanczos_eigs(Hn, 1, maxiter=300, tol=1e-8)
+    edge = float(e0n[0]) - egs
+    n_in = 64
+    w_in = np.linspace(edge - 2.0, edge + 8.0, n_in)
+    sigma = w_in + egs + 0.5j
+    eloss = np.linspace(0.0, 10.0, 2000)
+    pols = (((1.0, 0.0, 0.0), (0.0, 1.0, 0.0)), ((0.0, 1.0, 0.0), (0.0, 0.0, 1.0)))
+    Din = [c.build_transop_i(bn, bi, m.norb, m.ncore, m.pol_trans(ei)[0]) for ei, _ in pols]
+    Dout = [c.build_transop_f(bi, bn, m.norb, m.ncore, m.pol_trans(ef)[1]) for _, ef in pols]
+    b = [Din[p].matvec(gs) for p in range(2)]
+    Hi.prepare("c64d")
+    Hn.prepare("c64d")
+    kw = dict(tol=args.cf_tol, chunk=args.cf_chunk, min_steps=args.nkryl, max_steps=args.nkryl_max)
+
+    def finals(phis, precision):
+        P = torch.stack(phis).contiguous()
+        spec, steps, _ = c.converged_spectra(Hi, P, egs, eloss, 0.1, precision=precision, **kw)
+        return spec, steps
+
+    out = {"workload": "norb=20 N=10 SIAM, 64 w_in x 2 pols = 128 RHS on H_n (%d rows)" % Hn.n}
+    torch.cuda.synchronize()
+    # (1) multi-shift, c64d
+    t0 = time.perf_counter()
+    phis, its = [], []
+    for p in range(2):
+        (Y,), it, res = c.shifted_fom(Hn, b[p], sigma, projections=(Dout[p],), tol=1e-8,
+                                      maxit=8000, precision="c64d")
+        phis += [Y[k] for k in range(n_in)]
+        its.append(int(it.max()))
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    s1, st1 = finals(phis, "c64d")
+    torch.cuda.synchronize()
+    out["multishift_c64d"] = {"solve_s": t_solve, "total_s": time.perf_counter() - t0,
+                              "recurrence_steps": its}
+    log("config2 multi-shift: solve %.2f s, total %.2f s" % (t_solve, out["multishift_c64d"]["total_s"]))
+    # (2) batched MINRES over the SpMM, c64d
+    t0 = time.perf_counter()
+    B = torch.stack([b[p] for p in range(2) for _ in range(n_in)]).contiguous()
+    sig2 = np.concatenate([sigma, sigma])
+    bn_min = float(torch.linalg.vector_norm(B, dim=1).min())
+    X, it2, res2 = c.minres_shifted_batch(Hn, sig2, B, tol=1e-8 * bn_min, maxit=8000,
+                                          precision="c64d")
+    torch.cuda.synchronize()
+    t_solve2 = time.perf_counter() - t0
+    phis2 = [Dout[q // n_in].matvec(X[q].contiguous()) for q in range(2 * n_in)]
+    s2, st2 = finals(phis2, "c64d")
+    torch.cuda.synchronize()
+    out["batched_minres_c64d"] = {"solve_s": t_solve2, "total_s": time.perf_counter() - t0,
+                                  "iterations_max": int(it2.max())}
+    log("config2 batched MINRES: solve %.2f s, total %.2f s" % (t_solve2, out["batched_minres_c64d"]["total_s"]))
+    del X, B
+    # (3) per-RHS FP64 MINRES loop (the reference's procedure, matrices resident)
+    t0 = time.perf_counter()
+    phis3, its3 = [], []
+    for p in range(2):
+        nb = float(torch.linalg.vector_norm(b[p]))
+        for k in range(n_in):
+            A = c.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, omega=sigma[k], ham_sign=-1.0)
+            x, it3, _ = c.minres(A, b[p], tol=1e-8 * nb, maxit=8000)
+            phis3.append(Dout[p].matvec(x))
+            its3.append(int(it3))
+            del A
+    torch.cuda.synchronize()
+    t_solve3 = time.perf_counter() - t0
+    s3 = []
+    for q in range(2 * n_in):
+        sp, _ = finals([phis3[q]], "c128")
+        s3.append(sp[0])
+    torch.cuda.synchronize()
+    out["per_rhs_minres_fp64"] = {"solve_s": t_solve3, "total_s": time.perf_counter() - t0,
+                                  "iterations_max": max(its3)}
+    log("config2 per-RHS FP64 MINRES: solve %.2f s, total %.2f s" % (t_solve3, out["per_rhs_minres_fp64"]["total_s"]))
+    out["multishift_vs_fp64_rel_l2_max"] = max(rel(a, r) for a, r in zip(s1, s3))
+    out["batched_minres_vs_fp64_rel_l2_max"] = max(rel(a, r) for a, r in zip(s2, s3))
+    out["speedup_multishift_vs_per_rhs_fp64"] = (out["per_rhs_minres_fp64"]["total_s"] /
+                                                 out["multishift_c64d"]["total_s"])
+    out["speedup_batched_minres_vs_per_rhs_fp64"] = (out["per_rhs_minres_fp64"]["total_s"] /
+                                                     out["batched_minres_c64d"]["total_s"])
+    out["spectra"] = 2 * n_in
+    return out
+
+
 def measured_traffic(kernel, form, norb):
     """HBM bytes per launch of a kernel as two rocprofv3 counter runs measured
     it (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
@@ -522,6 +627,9 @@ def main():
         return
     if args.config4:
         print(json.dumps({"config4": config4(args)}))
+        return
+    if args.config2:
+        print(json.dumps({"config2": config2(args)}))
         return
     work = Workload(args)
     forms = args.forms.split(",")

@@ -16,7 +16,10 @@
 // gets the same spectrum alone and in any mix.
 //
 // Members live in device memory with their (alpha, beta) history; the host
-// adds and removes members between advances and reads the coefficients.
+// adds and removes members between advances and reads the coefficients.  The
+// step's launches read the step counter and vector parity from device state,
+// so GRAPH_STEPS steps are captured once per membership as a HIP graph and
+// replayed (EDX_NO_GRAPH=1 launches them one by one instead).
 
 namespace {
 
@@ -205,6 +208,10 @@ struct edx_group {
     int next_id = 1;
     std::vector<GroupMem> mems;
     bool dirty = true;
+    // GRAPH_STEPS steps captured once per membership and replayed on the
+    // null stream (the step's launches take no step-dependent arguments)
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
     GMember *d_mem = nullptr;
     size_t d_mem_cap = 0;
     double2 *partials = nullptr;
@@ -213,6 +220,8 @@ struct edx_group {
     size_t vpartials_cap = 0;
     int64_t nblk = 0, nvblk = 0;
     ~edx_group() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         for (auto &m : mems) {
             edx::dfree(m.v);
             edx::dfree(m.ab);
@@ -237,8 +246,60 @@ int grow(void **p, size_t *cap, size_t need, const char *what) {
     return EDX_OK;
 }
 
+constexpr int GRAPH_STEPS = 16;
+
+// the launches of one step of every member, on stream s
+int group_step(edx_group *g, hipStream_t s) {
+    const int nmem = (int)g->mems.size();
+    if (g->precision == EDX_C64D)
+        hipLaunchKernelGGL((k_gspmv<float2, true>), dim3((unsigned)g->nblk), dim3(256), 0, s,
+                           g->d_mem, nmem, g->partials);
+    else
+        hipLaunchKernelGGL((k_gspmv<double2, false>), dim3((unsigned)g->nblk), dim3(256), 0, s,
+                           g->d_mem, nmem, g->partials);
+    hipLaunchKernelGGL(k_galpha, dim3(nmem), dim3(256), 0, s, g->d_mem, g->partials);
+    hipLaunchKernelGGL(k_gaxpy, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem,
+                       g->vpartials);
+    hipLaunchKernelGGL(k_gbeta, dim3(nmem), dim3(256), 0, s, g->d_mem, g->vpartials, g->cap);
+    hipLaunchKernelGGL(k_gdiv, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem);
+    EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
+bool graphs_enabled() {
+    const char *off = getenv("EDX_NO_GRAPH");
+    return !(off && off[0] == '1');
+}
+
+// capture GRAPH_STEPS steps for the current membership
+int group_capture(edx_group *g) {
+    if (g->exec) return EDX_OK;
+    if (!g->cap_stream) EDX_HIP(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+    EDX_HIP(hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = EDX_OK;
+    for (int i = 0; i < GRAPH_STEPS && rc == EDX_OK; ++i) rc = group_step(g, g->cap_stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->cap_stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return edx::hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g->exec = nullptr;
+        return edx::hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    return EDX_OK;
+}
+
 int group_layout(edx_group *g) {
     if (!g->dirty) return EDX_OK;
+    if (g->exec) {
+        EDX_HIP(hipGraphExecDestroy(g->exec));
+        g->exec = nullptr;
+    }
     std::vector<GMember> hm(g->mems.size());
     int64_t blk = 0, vblk = 0;
     for (size_t j = 0; j < g->mems.size(); ++j) {
@@ -370,20 +431,20 @@ int edx_group_advance(edx_group *g, int nsteps, int *live) {
     const int nmem = (int)g->mems.size();
     hipStream_t s = edx::stream();
     int nlive = nmem;
-    for (int i = 0; i < nsteps && nmem > 0 && nlive > 0; ++i) {
-        if (g->precision == EDX_C64D)
-            hipLaunchKernelGGL((k_gspmv<float2, true>), dim3((unsigned)g->nblk), dim3(256), 0, s,
-                               g->d_mem, nmem, g->partials);
-        else
-            hipLaunchKernelGGL((k_gspmv<double2, false>), dim3((unsigned)g->nblk), dim3(256), 0, s,
-                               g->d_mem, nmem, g->partials);
-        hipLaunchKernelGGL(k_galpha, dim3(nmem), dim3(256), 0, s, g->d_mem, g->partials);
-        hipLaunchKernelGGL(k_gaxpy, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem,
-                           g->vpartials);
-        hipLaunchKernelGGL(k_gbeta, dim3(nmem), dim3(256), 0, s, g->d_mem, g->vpartials, g->cap);
-        hipLaunchKernelGGL(k_gdiv, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem);
-        EDX_LAUNCH_CHECK();
-        if ((i + 1) % 32 == 0 || i + 1 == nsteps) {   // stop launching once every member has ended
+    const bool graph = graphs_enabled() && nmem > 0 && nsteps >= GRAPH_STEPS;
+    if (graph) EDX_TRY(group_capture(g));
+    for (int i = 0; i < nsteps && nmem > 0 && nlive > 0;) {
+        int done;
+        if (graph && nsteps - i >= GRAPH_STEPS) {
+            EDX_HIP(hipGraphLaunch(g->exec, s));
+            done = GRAPH_STEPS;
+        } else {
+            EDX_TRY(group_step(g, s));
+            done = 1;
+        }
+        const int before = i;
+        i += done;
+        if (i / 32 != before / 32 || i == nsteps) {   // stop launching once every member has ended
             nlive = 0;
             for (const auto &m : g->mems) {
                 GState st;

@@ -776,3 +776,34 @@ def test_matrix_free_rows_with_many_terms(core):
     ref = o.matvec(R, x)
     scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
     assert np.abs(y - ref).max() < 1e-14 * scale
+
+
+def test_group_graph_replays_equal_eager_launches(core, monkeypatch):
+    # steps replayed from a captured HIP graph give the numbers of steps launched
+    # one by one, across several replays, re-captures and eager work between them
+    m1, fn1, bn1, A1, R1 = _siam_hn(core, nbath=4, seed=2)
+    m2 = H.small_siam(nbath=3, seed=5)
+    f2 = H.fock_words(m2.norb, m2.nocc)
+    A2 = core.build_ham_i(core.Basis.from_words(f2, m2.norb), m2.hop_i, m2.coul_i)
+    rng = np.random.default_rng(3)
+    va = _to_dev(core, rng.normal(size=A1.n) + 1j * rng.normal(size=A1.n))
+    vb = _to_dev(core, rng.normal(size=A2.n) + 0j)
+
+    def run():
+        g = core.KrylovGroup("c64d", max_steps=400)
+        a = g.add(A1, va)
+        g.advance(37)                    # two replays + 5 single steps
+        b = g.add(A2, vb)                # new membership: capture again
+        _ = A1.matvec(va)                # eager work between replays
+        g.advance(50)
+        g.advance(16)
+        out = (g.coefficients(a), g.coefficients(b))
+        g.close()
+        return out
+
+    monkeypatch.setenv("EDX_NO_GRAPH", "1")
+    eager = run()
+    monkeypatch.delenv("EDX_NO_GRAPH")
+    graph = run()
+    for (ae, be, ne, _), (ag, bg, ng, _) in zip(eager, graph):
+        assert len(ae) > 0 and np.array_equal(ae, ag) and np.array_equal(be, bg) and ne == ng
