@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libedx.so")
 EDX_C128 = 0
 EDX_C64 = 1
 EDX_C64D = 2
+EDX_C128I = 3
 
 _STATUS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "I/O error",
            4: "out of device memory", 5: "numerical failure", 6: "capacity exceeded"}

@@ -25,7 +25,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EDX_C128, EDX_C64, EDX_C64D
+from ._lib import EDX_C128, EDX_C64, EDX_C64D, EDX_C128I
 
 T2 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("re", "<f8"), ("im", "<f8")])
 T4 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("ind3", "<i4"), ("ind4", "<i4"),
@@ -34,7 +34,8 @@ T4 = np.dtype([("ind1", "<i4"), ("ind2", "<i4"), ("ind3", "<i4"), ("ind4", "<i4"
 PRECISIONS = {"c128": EDX_C128, "fp64": EDX_C128, "complex128": EDX_C128,
               "c64": EDX_C64, "complex64": EDX_C64,
               "c64d": EDX_C64D, "mixed": EDX_C64D,
-              EDX_C128: EDX_C128, EDX_C64: EDX_C64, EDX_C64D: EDX_C64D}
+              "c128i": EDX_C128I, "indexed": EDX_C128I,
+              EDX_C128: EDX_C128, EDX_C64: EDX_C64, EDX_C64D: EDX_C64D, EDX_C128I: EDX_C128I}
 
 
 def _prec(p):
@@ -282,11 +283,12 @@ def lanczos_refine(H, start, nev, maxiter=200, tol=1e-10, nvec=None, precision="
 
 def ground_state(H, nev=1, precision="c128", maxiter=500, tol=1e-10, seed=1, deflate=False,
                  refine_maxiter=200):
-    """Lowest eigenpairs.  With precision 'c64' the Lanczos products use complex64
-    values and the result is finished by an FP64 Lanczos started from the
-    complex64 eigenvectors; returns (evals, evecs, iterations)."""
-    if _prec(precision) == EDX_C128:
-        return lanczos_eigs(H, nev, maxiter, tol, seed, deflate, nev, "c128")
+    """Lowest eigenpairs.  With precision 'c64' or 'c64d' the Lanczos products
+    round values to complex64 and the result is finished by an FP64 Lanczos
+    started from those eigenvectors; 'c128' and 'c128i' are exact and need no
+    refinement.  Returns (evals, evecs, iterations)."""
+    if _prec(precision) in (EDX_C128, EDX_C128I):   # exact products: no refinement
+        return lanczos_eigs(H, nev, maxiter, tol, seed, deflate, nev, precision)
     ev, vecs, it = lanczos_eigs(H, nev, maxiter, max(tol, 1e-6), seed, deflate, nev, precision)
     ev2, vecs2, it2 = lanczos_refine(H, vecs, nev, refine_maxiter, tol, nev, "c128")
     return ev2, vecs2, it + it2

@@ -21,6 +21,7 @@ constexpr int TUNE_REPS = 5;
 int *lanes_slot(edx_csr *a, int precision) {
     if (precision == EDX_C64) return &a->spmv_lanes32;
     if (precision == EDX_C64D) return &a->spmv_lanes64d;
+    if (precision == EDX_C128I) return &a->spmv_lanes128i;
     return &a->spmv_lanes;
 }
 
@@ -101,7 +102,7 @@ extern "C" {
 
 int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes) {
     EDX_CHECK(a, EDX_ERR_ARG, "null matrix");
-    EDX_CHECK(precision >= EDX_C128 && precision <= EDX_C64D, EDX_ERR_ARG, "bad precision %d",
+    EDX_CHECK(precision >= EDX_C128 && precision <= EDX_C128I, EDX_ERR_ARG, "bad precision %d",
               precision);
     EDX_CHECK(set == 0 || set == 4 || set == 8 || set == 16 || set == 32 || set == 64, EDX_ERR_ARG,
               "lanes per row must be 4, 8, 16, 32 or 64 (got %d)", set);

@@ -44,17 +44,6 @@ struct BEpi {
     double2 *partials;     // [nblocks * K]
 };
 
-template <class VT>
-__device__ __forceinline__ VT ld_val(const VT *p);
-template <>
-__device__ __forceinline__ double2 ld_val<double2>(const double2 *p) {
-    return nt_load2(p);
-}
-template <>
-__device__ __forceinline__ float2 ld_val<float2>(const float2 *p) {
-    return nt_load2(p);
-}
-
 __device__ __forceinline__ double2 epilogue(const BEpi &e, int k, int64_t o, double2 hx,
                                             const double2 *X64) {
     double2 out = e.c0 ? cmul(e.c0[k], hx) : hx;
@@ -64,10 +53,9 @@ __device__ __forceinline__ double2 epilogue(const BEpi &e, int k, int64_t o, dou
 }
 
 // Row-split schedule: G lanes per row, KT right-hand sides per pass (blockIdx.y).
-template <int G, int KT, class VT, class XT, class ACC, bool SPLIT>
+template <int G, int KT, class VP, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t *__restrict__ rowptr,
-                                                 const int32_t *__restrict__ cols,
-                                                 const VT *__restrict__ vals,
+                                                 const int32_t *__restrict__ cols, VP vals,
                                                  const double2 *__restrict__ diag,
                                                  const XT *__restrict__ X,
                                                  const double2 *__restrict__ X64, BEpi e) {
@@ -84,7 +72,7 @@ __global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t
 #pragma unroll
         for (int q = 0; q < KT; ++q) sr[q] = si[q] = (ACC)0;
         for (int64_t j = b + lig; j < end; j += G) {
-            const VT a = ld_val<VT>(vals + j);
+            const auto a = nt_load2(vals + j);
             const int64_t c = nt_load(cols + j);
             const XT *xr = X + c * K + k0;
 #pragma unroll
@@ -131,10 +119,9 @@ __global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t
 
 // RHS-parallel schedule: 16 groups of 16 lanes per block, one row per group,
 // lane l owns right-hand sides k0 + l + 16 q (q < NPL), k0 = blockIdx.y * 128.
-template <int NPL, class VT, class XT, class ACC, bool SPLIT>
+template <int NPL, class VP, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t *__restrict__ rowptr,
-                                                 const int32_t *__restrict__ cols,
-                                                 const VT *__restrict__ vals,
+                                                 const int32_t *__restrict__ cols, VP vals,
                                                  const double2 *__restrict__ diag,
                                                  const XT *__restrict__ X,
                                                  const double2 *__restrict__ X64, BEpi e) {
@@ -153,11 +140,11 @@ __global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t
         for (int q = 0; q < NPL; ++q) sr[q] = si[q] = (ACC)0;
         int64_t j = b;
         for (; j + 3 < end; j += 4) {
-            VT a[4];
+            decltype(nt_load2(vals)) a[4];
             int64_t c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                a[u] = ld_val<VT>(vals + j + u);
+                a[u] = nt_load2(vals + j + u);
                 c[u] = nt_load(cols + j + u);
             }
 #pragma unroll
@@ -174,7 +161,7 @@ __global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t
             }
         }
         for (; j < end; ++j) {
-            const VT a = ld_val<VT>(vals + j);
+            const auto a = nt_load2(vals + j);
             const int64_t c = nt_load(cols + j);
 #pragma unroll
             for (int q = 0; q < NPL; ++q) {
@@ -560,6 +547,90 @@ __global__ void k_split_fill(int64_t m, const int64_t *rowptr, const int32_t *co
     }
 }
 
+// ---- indexed form: a hash table of the distinct off-diagonal values
+constexpr int VTAB_SLOTS = 1 << 16;
+constexpr int VTAB_PROBES = 1024;   // a longer probe means the table is too full
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {   // splitmix64 finalizer
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t value_fp(double2 v) {
+    const uint64_t f = mix64((uint64_t)__double_as_longlong(v.x) ^
+                             mix64((uint64_t)__double_as_longlong(v.y) + 0x9e3779b97f4a7c15ull));
+    return f ? f : 1ull;
+}
+
+// claim (or find) the slot of every off-diagonal value by its 64-bit
+// fingerprint (compare-and-swap on an empty slot, linear probing) and store
+// the value there; flag bit 1 = table full
+__global__ void k_vt_insert(int64_t m, const int64_t *rowptr, const int32_t *cols,
+                            const double2 *vals, unsigned long long *fp, double2 *tab, int *flag) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        if (*(volatile int *)flag) return;   // already failed: stop early
+        unsigned long long last = 0;         // the value this thread placed last
+        for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            if (cols[j] == r) continue;
+            const double2 v = vals[j];
+            const unsigned long long f = value_fp(v);
+            if (f == last) continue;
+            unsigned h = (unsigned)f & (VTAB_SLOTS - 1);
+            int k = 0;
+            for (; k < VTAB_PROBES; ++k) {
+                // a plain read first: most values are already in the table, and a
+                // compare-and-swap on the few hot slots would serialize
+                const unsigned long long cur = ((volatile unsigned long long *)fp)[h];
+                if (cur == f) break;
+                if (cur == 0ull) {
+                    const unsigned long long old = atomicCAS(&fp[h], 0ull, f);
+                    if (old == 0ull) {
+                        tab[h] = v;   // the claiming thread stores the value
+                        break;
+                    }
+                    if (old == f) break;
+                }
+                h = (h + 1) & (VTAB_SLOTS - 1);
+            }
+            if (k == VTAB_PROBES) {
+                atomicOr(flag, 1);
+                return;
+            }
+            last = f;
+        }
+    }
+}
+
+// the slot number of every off-diagonal value, in the split form's order;
+// flag bit 2 = two different values share a fingerprint
+__global__ void k_vt_index(int64_t m, const int64_t *rowptr, const int32_t *cols,
+                           const double2 *vals, const int64_t *orow,
+                           const unsigned long long *fp, const double2 *tab, uint16_t *idx,
+                           int *flag) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = orow[r];
+        for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            if (cols[j] == r) continue;
+            const double2 v = vals[j];
+            const unsigned long long f = value_fp(v);
+            unsigned h = (unsigned)f & (VTAB_SLOTS - 1);
+            int k = 0;
+            while (fp[h] != f && k < VTAB_PROBES) {
+                h = (h + 1) & (VTAB_SLOTS - 1);
+                ++k;
+            }
+            const double2 t = tab[h];
+            if (k == VTAB_PROBES || __double_as_longlong(t.x) != __double_as_longlong(v.x) ||
+                __double_as_longlong(t.y) != __double_as_longlong(v.y))
+                atomicOr(flag, 2);
+            idx[o++] = (uint16_t)h;
+        }
+    }
+}
+
 }  // namespace
 
 namespace edx {
@@ -595,17 +666,21 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
 #define EDX_RL(NP)                                                                                 \
     case NP:                                                                                       \
         if (precision == EDX_C128)                                                                 \
-            hipLaunchKernelGGL((k_spmm_rl<NP, double2, double2, double, false>), grid, dim3(256), 0, \
-                               stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals, nullptr, X64, \
-                               X64, e);                                                             \
+            hipLaunchKernelGGL((k_spmm_rl<NP, const double2 *, double2, double, false>), grid,      \
+                               dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,            \
+                               (const double2 *)a->d_vals, nullptr, X64, X64, e);                 \
         else if (precision == EDX_C64)                                                             \
-            hipLaunchKernelGGL((k_spmm_rl<NP, float2, float2, float, false>), grid, dim3(256), 0,   \
-                               stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals32, nullptr,    \
-                               X32, X64, e);                                                        \
+            hipLaunchKernelGGL((k_spmm_rl<NP, const float2 *, float2, float, false>), grid,         \
+                               dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,            \
+                               (const float2 *)a->d_vals32, nullptr, X32, X64, e);                \
+        else if (precision == EDX_C128I)                                                           \
+            hipLaunchKernelGGL((k_spmm_rl<NP, IdxPtr, double2, double, true>), grid, dim3(256), 0,  \
+                               stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,                  \
+                               IdxPtr{a->d_off_idx, a->d_vtab}, a->d_diag, X64, X64, e);           \
         else                                                                                       \
-            hipLaunchKernelGGL((k_spmm_rl<NP, float2, double2, double, true>), grid, dim3(256), 0,  \
-                               stream(), a->m, K, a->d_off_rowptr, a->d_off_cols, a->d_off_vals32, \
-                               a->d_diag, X64, X64, e);                                            \
+            hipLaunchKernelGGL((k_spmm_rl<NP, const float2 *, double2, double, true>), grid,        \
+                               dim3(256), 0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,    \
+                               (const float2 *)a->d_off_vals32, a->d_diag, X64, X64, e);          \
         break;
         switch (npl) {
             EDX_RL(1)
@@ -621,17 +696,21 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
 #define EDX_BSPMM(GG)                                                                              \
     case GG:                                                                                       \
         if (precision == EDX_C128)                                                                 \
-            hipLaunchKernelGGL((k_spmm_il<GG, BKT, double2, double2, double, false>), grid, dim3(256), \
-                               0, stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals, nullptr,   \
-                               X64, X64, e);                                                        \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, const double2 *, double2, double, false>), grid, \
+                               dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,            \
+                               (const double2 *)a->d_vals, nullptr, X64, X64, e);                 \
         else if (precision == EDX_C64)                                                             \
-            hipLaunchKernelGGL((k_spmm_il<GG, BKT, float2, float2, float, false>), grid, dim3(256),  \
-                               0, stream(), a->m, K, a->d_rowptr, a->d_cols, a->d_vals32, nullptr, \
-                               X32, X64, e);                                                        \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, const float2 *, float2, float, false>), grid,    \
+                               dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,            \
+                               (const float2 *)a->d_vals32, nullptr, X32, X64, e);                \
+        else if (precision == EDX_C128I)                                                           \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, IdxPtr, double2, double, true>), grid,           \
+                               dim3(256), 0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,    \
+                               IdxPtr{a->d_off_idx, a->d_vtab}, a->d_diag, X64, X64, e);           \
         else                                                                                       \
-            hipLaunchKernelGGL((k_spmm_il<GG, BKT, float2, double2, double, true>), grid, dim3(256), \
-                               0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols, a->d_off_vals32, \
-                               a->d_diag, X64, X64, e);                                            \
+            hipLaunchKernelGGL((k_spmm_il<GG, BKT, const float2 *, double2, double, true>), grid,   \
+                               dim3(256), 0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,    \
+                               (const float2 *)a->d_off_vals32, a->d_diag, X64, X64, e);          \
         break;
         switch (G) {
             EDX_BSPMM(4)
@@ -686,9 +765,81 @@ int csr_add_split(edx_csr *a) {
     return EDX_OK;
 }
 
+// The indexed form: the split form's pattern and diagonal, and per
+// off-diagonal nonzero the table slot of its exact value.
+int csr_add_indexed(edx_csr *a) {
+    if (a->d_off_idx) return EDX_OK;
+    EDX_TRY(csr_add_split(a));
+    const int64_t m = a->m;
+    auto *fp = (unsigned long long *)dmalloc(VTAB_SLOTS * sizeof(unsigned long long), "value fingerprints");
+    auto *tab = (double2 *)dmalloc(VTAB_SLOTS * sizeof(double2), "value table");
+    auto *idx = (uint16_t *)dmalloc((a->off_nnz ? a->off_nnz : 1) * sizeof(uint16_t), "value numbers");
+    int *flag = (int *)dmalloc(sizeof(int), "index flag");
+    int rc = EDX_OK;
+    std::vector<unsigned long long> hfp;
+    int hflag = 0;
+    if (!fp || !tab || !idx || !flag) {
+        rc = EDX_ERR_NOMEM;
+        goto fail;
+    }
+    if (hipMemsetAsync(fp, 0, VTAB_SLOTS * sizeof(unsigned long long), stream()) != hipSuccess ||
+        hipMemsetAsync(tab, 0, VTAB_SLOTS * sizeof(double2), stream()) != hipSuccess ||
+        hipMemsetAsync(flag, 0, sizeof(int), stream()) != hipSuccess) {
+        rc = set_error(EDX_ERR_HIP, "indexed form: memset failed");
+        goto fail;
+    }
+    {
+        const int g = elem_blocks(m);
+        hipLaunchKernelGGL(k_vt_insert, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
+                           a->d_vals, fp, tab, flag);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = set_error(EDX_ERR_HIP, "indexed form: kernels failed");
+            goto fail;
+        }
+        if (hflag & 1) {
+            rc = set_error(EDX_ERR_CAPACITY,
+                           "too many distinct off-diagonal values for the %d-slot table: no indexed form",
+                           VTAB_SLOTS);
+            goto fail;
+        }
+        hipLaunchKernelGGL(k_vt_index, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
+                           a->d_vals, a->d_off_rowptr, fp, tab, idx, flag);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = set_error(EDX_ERR_HIP, "indexed form: kernels failed");
+            goto fail;
+        }
+    }
+    if (hflag & 2) {
+        rc = set_error(EDX_ERR_CAPACITY, "two distinct values share a 64-bit fingerprint: no indexed form");
+        goto fail;
+    }
+    hfp.resize(VTAB_SLOTS);
+    if (hipMemcpy(hfp.data(), fp, VTAB_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+        rc = set_error(EDX_ERR_HIP, "indexed form: download failed");
+        goto fail;
+    }
+    a->nvtab = 0;
+    for (unsigned long long f : hfp) a->nvtab += f != 0ull;
+    dfree(fp);
+    dfree(flag);
+    a->d_vtab = tab;
+    a->d_off_idx = idx;
+    return EDX_OK;
+fail:
+    dfree(fp);
+    dfree(tab);
+    dfree(idx);
+    dfree(flag);
+    return rc;
+}
+
 int ensure_precision(edx_csr *a, int precision) {
     if (precision == EDX_C64) EDX_TRY(edx_csr_add_c64(a));
     else if (precision == EDX_C64D) EDX_TRY(csr_add_split(a));
+    else if (precision == EDX_C128I) EDX_TRY(csr_add_indexed(a));
     else if (precision != EDX_C128) return set_error(EDX_ERR_ARG, "unknown precision %d", precision);
     if (a->m == a->n) EDX_TRY(tune_spmv(a, precision));
     return EDX_OK;

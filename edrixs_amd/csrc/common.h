@@ -67,6 +67,7 @@ struct edx_csr {
     int spmv_lanes;      // lanes per row chosen for SpMV (0 = not tuned)
     int spmv_lanes32;
     int spmv_lanes64d;
+    int spmv_lanes128i;
     int spmm_lanes;
     // split form (EDX_C64D): exact diagonal + complex64 off-diagonal CSR
     double2 *d_diag;
@@ -74,6 +75,11 @@ struct edx_csr {
     int32_t *d_off_cols;
     float2 *d_off_vals32;
     int64_t off_nnz;
+    // indexed form (EDX_C128I): the split form's off-diagonal pattern with, per
+    // nonzero, the number of its exact value in a table of distinct values
+    uint16_t *d_off_idx;
+    double2 *d_vtab;     // VTAB_SLOTS entries (hash-addressed; unused slots zero)
+    int64_t nvtab;       // distinct off-diagonal values
 };
 
 // Lookup descriptor for a column space basis.

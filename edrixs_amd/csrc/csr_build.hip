@@ -630,6 +630,8 @@ void edx_csr_free(edx_csr *a) {
     edx::dfree(a->d_off_rowptr);
     edx::dfree(a->d_off_cols);
     edx::dfree(a->d_off_vals32);
+    edx::dfree(a->d_off_idx);
+    edx::dfree(a->d_vtab);
     delete a;
 }
 

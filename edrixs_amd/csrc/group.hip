@@ -38,7 +38,9 @@ struct GMember {
     const int64_t *rowptr;
     const int32_t *cols;
     const void *vals;        // double2 (complex128 form) or float2 (split form off-diagonal)
-    const double2 *diag;     // split form diagonal
+    const uint16_t *idx;     // indexed form: value numbers ...
+    const double2 *tab;      // ... into this table
+    const double2 *diag;     // split and indexed forms: diagonal
     int64_t m;
     double2 *v0, *v1;
     double *alpha, *beta;    // [cap] each
@@ -58,7 +60,22 @@ __device__ __forceinline__ int g_find(const GMember *mem, int nmem, int64_t bid,
     return lo;
 }
 
-template <class VT, bool SPLIT>
+template <class VP>
+__device__ __forceinline__ VP g_vals(const GMember &M);
+template <>
+__device__ __forceinline__ const double2 *g_vals<const double2 *>(const GMember &M) {
+    return (const double2 *)M.vals;
+}
+template <>
+__device__ __forceinline__ const float2 *g_vals<const float2 *>(const GMember &M) {
+    return (const float2 *)M.vals;
+}
+template <>
+__device__ __forceinline__ IdxPtr g_vals<IdxPtr>(const GMember &M) {
+    return IdxPtr{M.idx, M.tab};
+}
+
+template <class VP, bool SPLIT>
 __global__ __launch_bounds__(256) void k_gspmv(const GMember *mem, int nmem, double2 *partials) {
     const int64_t bid = blockIdx.x;
     const GMember M = mem[g_find(mem, nmem, bid, false)];
@@ -69,14 +86,14 @@ __global__ __launch_bounds__(256) void k_gspmv(const GMember *mem, int nmem, dou
     }
     const double2 *x = (st.steps & 1) ? M.v1 : M.v0;
     double2 *y = (st.steps & 1) ? M.v0 : M.v1;
-    const VT *vals = (const VT *)M.vals;
+    const VP vals = g_vals<VP>(M);
     const int lig = threadIdx.x & (GG - 1);
     const int64_t r = (bid - M.blk0) * G_ROWS + threadIdx.x / GG;
     double2 dot = make_double2(0, 0);
     if (r < M.m) {
         double sr = 0, si = 0;
         for (int64_t j = M.rowptr[r] + lig; j < M.rowptr[r + 1]; j += GG) {
-            const VT a = nt_load2(vals + j);
+            const auto a = nt_load2(vals + j);
             const double2 xv = x[nt_load(M.cols + j)];
             sr += (double)a.x * xv.x - (double)a.y * xv.y;
             si += (double)a.x * xv.y + (double)a.y * xv.x;
@@ -252,11 +269,14 @@ constexpr int GRAPH_STEPS = 16;
 int group_step(edx_group *g, hipStream_t s) {
     const int nmem = (int)g->mems.size();
     if (g->precision == EDX_C64D)
-        hipLaunchKernelGGL((k_gspmv<float2, true>), dim3((unsigned)g->nblk), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_gspmv<const float2 *, true>), dim3((unsigned)g->nblk), dim3(256), 0,
+                           s, g->d_mem, nmem, g->partials);
+    else if (g->precision == EDX_C128I)
+        hipLaunchKernelGGL((k_gspmv<IdxPtr, true>), dim3((unsigned)g->nblk), dim3(256), 0, s,
                            g->d_mem, nmem, g->partials);
     else
-        hipLaunchKernelGGL((k_gspmv<double2, false>), dim3((unsigned)g->nblk), dim3(256), 0, s,
-                           g->d_mem, nmem, g->partials);
+        hipLaunchKernelGGL((k_gspmv<const double2 *, false>), dim3((unsigned)g->nblk), dim3(256),
+                           0, s, g->d_mem, nmem, g->partials);
     hipLaunchKernelGGL(k_galpha, dim3(nmem), dim3(256), 0, s, g->d_mem, g->partials);
     hipLaunchKernelGGL(k_gaxpy, dim3((unsigned)g->nvblk), dim3(256), 0, s, g->d_mem, nmem,
                        g->vpartials);
@@ -306,10 +326,12 @@ int group_layout(edx_group *g) {
         const GroupMem &m = g->mems[j];
         const edx_csr *h = m.h;
         GMember &d = hm[j];
-        const bool split = g->precision == EDX_C64D;
+        const bool split = g->precision == EDX_C64D || g->precision == EDX_C128I;
         d.rowptr = split ? h->d_off_rowptr : h->d_rowptr;
         d.cols = split ? h->d_off_cols : h->d_cols;
-        d.vals = split ? (const void *)h->d_off_vals32 : (const void *)h->d_vals;
+        d.vals = g->precision == EDX_C64D ? (const void *)h->d_off_vals32 : (const void *)h->d_vals;
+        d.idx = h->d_off_idx;
+        d.tab = h->d_vtab;
         d.diag = split ? h->d_diag : nullptr;
         d.m = h->m;
         d.v0 = m.v;
@@ -347,8 +369,9 @@ extern "C" {
 int edx_group_create(int precision, int max_steps, edx_group **out) {
     EDX_CHECK(out, EDX_ERR_ARG, "null output handle");
     *out = nullptr;
-    EDX_CHECK(precision == EDX_C128 || precision == EDX_C64D, EDX_ERR_ARG,
-              "grouped recurrences run in complex128 or the split complex64 form (got %d)",
+    EDX_CHECK(precision == EDX_C128 || precision == EDX_C64D || precision == EDX_C128I,
+              EDX_ERR_ARG,
+              "grouped recurrences run in complex128, the split or the indexed form (got %d)",
               precision);
     EDX_CHECK(max_steps >= 1, EDX_ERR_ARG, "max_steps %d < 1", max_steps);
     edx_group *g = new edx_group;

@@ -28,6 +28,19 @@ __device__ __forceinline__ float2 nt_load2(const float2 *p) {
     return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
 }
 
+// Value source of the indexed form: nonzero j holds the 16-bit number of its
+// value in a table of the matrix's distinct off-diagonal values (exact
+// complex128).  Behaves like a value pointer in the kernels: vals + j, nt_load2.
+struct IdxPtr {
+    const uint16_t *idx;
+    const double2 *tab;
+    __device__ __forceinline__ IdxPtr operator+(int64_t j) const { return IdxPtr{idx + j, tab}; }
+};
+
+__device__ __forceinline__ double2 nt_load2(IdxPtr p) {
+    return p.tab[__builtin_nontemporal_load(p.idx)];
+}
+
 template <class A>
 struct AccT;
 template <>
@@ -74,10 +87,9 @@ __device__ __forceinline__ double2 block_sum_256(double2 v) {
 // SPLIT: the split form (complex64 off-diagonal values in the CSR arrays, the
 // exact FP64 diagonal in diag, complex128 x, FP64 accumulation); the leader
 // adds diag[r] * x[r] to the off-diagonal sum.
-template <int G, class VT, class XT, class ACC, bool SPLIT>
+template <int G, class VP, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restrict__ rowptr,
-                                              const int32_t *__restrict__ cols,
-                                              const VT *__restrict__ vals,
+                                              const int32_t *__restrict__ cols, VP vals,
                                               const XT *__restrict__ x,
                                               const double2 *__restrict__ diag, SpmvEpi e) {
     if (e.run && *e.run == 0) return;
@@ -93,8 +105,8 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
         acc_t sr = 0, si = 0;
         int64_t j = b + lig;
         for (; j + G < end; j += 2 * G) {   // two independent loads in flight per lane
-            const VT a0 = nt_load2(vals + j);
-            const VT a1 = nt_load2(vals + j + G);
+            const auto a0 = nt_load2(vals + j);
+            const auto a1 = nt_load2(vals + j + G);
             const int32_t k0 = nt_load(cols + j);
             const int32_t k1 = nt_load(cols + j + G);
             const XT x0 = x[k0];
@@ -105,7 +117,7 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
             si += (acc_t)a1.x * (acc_t)x1.y + (acc_t)a1.y * (acc_t)x1.x;
         }
         if (j < end) {
-            const VT a0 = nt_load2(vals + j);
+            const auto a0 = nt_load2(vals + j);
             const XT x0 = x[nt_load(cols + j)];
             sr += (acc_t)a0.x * (acc_t)x0.x - (acc_t)a0.y * (acc_t)x0.y;
             si += (acc_t)a0.x * (acc_t)x0.y + (acc_t)a0.y * (acc_t)x0.x;
@@ -265,6 +277,9 @@ double spmv_bytes_k(const edx_csr *a, int precision, int K, bool z, bool dvec) {
     if (precision == EDX_C64D)
         return (double)a->off_nnz * 12.0 + (double)(a->m + 1) * 8.0 + (double)a->m * 16.0 +
                K * ((double)a->n * 16.0 + yv);
+    if (precision == EDX_C128I)   // int32 column + uint16 value number per nonzero
+        return (double)a->off_nnz * 6.0 + (double)(a->m + 1) * 8.0 + (double)a->m * 16.0 +
+               (double)a->nvtab * 16.0 + K * ((double)a->n * 16.0 + yv);
     const double sv = precision == EDX_C64 ? 8.0 : 16.0;
     return (double)a->nnz * (sv + 4.0) + (double)(a->m + 1) * 8.0 + K * ((double)a->n * sv + yv);
 }
@@ -297,10 +312,13 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
                 int *nblocks_out) {
     if (precision == EDX_C64D && !a->d_diag)
         return set_error(EDX_ERR_ARG, "split (c64d) form not prepared");
+    if (precision == EDX_C128I && !a->d_off_idx)
+        return set_error(EDX_ERR_ARG, "indexed (c128i) form not prepared");
     int G = lanes;
     if (!G) {
         if (precision == EDX_C64) G = a->spmv_lanes32 ? a->spmv_lanes32 : lanes_for(a);
         else if (precision == EDX_C64D) G = a->spmv_lanes64d ? a->spmv_lanes64d : lanes_for_split(a);
+        else if (precision == EDX_C128I) G = a->spmv_lanes128i ? a->spmv_lanes128i : lanes_for_split(a);
         else G = a->spmv_lanes ? a->spmv_lanes : lanes_for(a);
     }
     int nb = spmv_blocks(a, G);
@@ -311,17 +329,21 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
 #define EDX_SPMV_CASE(GG)                                                                     \
     case GG:                                                                                  \
         if (precision == EDX_C128)                                                            \
-            hipLaunchKernelGGL((k_spmv<GG, double2, double2, double, false>), dim3(nb), dim3(256), \
-                               0, stream(), a->m, a->d_rowptr, a->d_cols, a->d_vals,           \
-                               (const double2 *)x, nullptr, e);                                \
+            hipLaunchKernelGGL((k_spmv<GG, const double2 *, double2, double, false>), dim3(nb),   \
+                               dim3(256), 0, stream(), a->m, a->d_rowptr, a->d_cols,           \
+                               (const double2 *)a->d_vals, (const double2 *)x, nullptr, e);    \
         else if (precision == EDX_C64D)                                                       \
-            hipLaunchKernelGGL((k_spmv<GG, float2, double2, double, true>), dim3(nb), dim3(256),  \
+            hipLaunchKernelGGL((k_spmv<GG, const float2 *, double2, double, true>), dim3(nb),     \
+                               dim3(256), 0, stream(), a->m, a->d_off_rowptr, a->d_off_cols,   \
+                               (const float2 *)a->d_off_vals32, (const double2 *)x, a->d_diag, e); \
+        else if (precision == EDX_C128I)                                                      \
+            hipLaunchKernelGGL((k_spmv<GG, IdxPtr, double2, double, true>), dim3(nb), dim3(256),  \
                                0, stream(), a->m, a->d_off_rowptr, a->d_off_cols,              \
-                               a->d_off_vals32, (const double2 *)x, a->d_diag, e);             \
+                               IdxPtr{a->d_off_idx, a->d_vtab}, (const double2 *)x, a->d_diag, e); \
         else                                                                                  \
-            hipLaunchKernelGGL((k_spmv<GG, float2, float2, float, false>), dim3(nb), dim3(256),   \
-                               0, stream(), a->m, a->d_rowptr, a->d_cols, a->d_vals32,         \
-                               (const float2 *)x, nullptr, e);                                 \
+            hipLaunchKernelGGL((k_spmv<GG, const float2 *, float2, float, false>), dim3(nb),      \
+                               dim3(256), 0, stream(), a->m, a->d_rowptr, a->d_cols,           \
+                               (const float2 *)a->d_vals32, (const float2 *)x, nullptr, e);    \
         break;
     switch (G) {
         EDX_SPMV_CASE(4)
@@ -429,7 +451,7 @@ int edx_profile_read(int precision, int kernel, int64_t *launches, double *total
 int edx_spmv(const edx_csr *a, int precision, const void *d_x, void *d_y) {
     EDX_CHECK(a && d_x && d_y, EDX_ERR_ARG, "null argument");
     EDX_CHECK(precision == EDX_C128 || (precision == EDX_C64 && a->d_vals32) ||
-                  (precision == EDX_C64D && a->d_diag),
+                  (precision == EDX_C64D && a->d_diag) || (precision == EDX_C128I && a->d_off_idx),
               EDX_ERR_ARG, "precision %d not prepared on this matrix", precision);
     SpmvEpi e{};
     e.y = (double2 *)d_y;

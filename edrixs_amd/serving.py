@@ -84,7 +84,7 @@ def _dims(norb):
 class Engine:
     """Continuous-batching XAS / RIXS server on one GPU."""
 
-    def __init__(self, budget_gb=64.0, precision="c64d", check_every=100, chunk=25,
+    def __init__(self, budget_gb=64.0, precision="c128i", check_every=100, chunk=25,
                  cf_tol=1e-6, min_steps=200, max_steps=4000, n_in=4, npol=1, nw=1000,
                  rixs_tol=1e-6, rixs_maxit=4000):
         self.budget = budget_gb * 1e9
@@ -108,7 +108,7 @@ class Engine:
         """HBM a request holds while it runs: its CSR matrices (nonzeros per row
         from the model's term counts, bounded) and its Krylov vectors."""
         n_i, n_n = _dims(req.norb)
-        per_nnz = 12.0 if self.precision == "c64d" else 20.0
+        per_nnz = {"c64d": 32.0, "c128i": 34.0}.get(self.precision, 20.0)   # CSR + its copies
         mats = (n_i * 18 + n_n * 26) * per_nnz + (n_i + n_n) * 32.0
         vecs = 16.0 * (2 * 3 * n_n if req.kind == "xas"
                        else 2 * self.n_in * self.npol * n_i + 6 * n_n + 2 * self.n_in * n_i)

@@ -37,8 +37,15 @@ enum edx_status {
  *   EDX_C128  exact complex128 values, complex128 vectors, FP64 accumulation;
  *   EDX_C64   complex64 values and vectors, FP32 accumulation;
  *   EDX_C64D  complex64 off-diagonal values, exact complex128 diagonal kept
- *             apart, complex128 vectors, FP64 accumulation. */
-enum edx_precision { EDX_C128 = 0, EDX_C64 = 1, EDX_C64D = 2 };
+ *             apart, complex128 vectors, FP64 accumulation;
+ *   EDX_C128I exact: the diagonal kept apart in complex128, each off-diagonal
+ *             nonzero stored as the 16-bit number of its value in a table of
+ *             the matrix's distinct off-diagonal values (a Hamiltonian built
+ *             from a few hundred terms has a few dozen), complex128 vectors,
+ *             FP64 accumulation.  Preparing it fails (EDX_ERR_CAPACITY) when
+ *             the distinct off-diagonal values do not fit the 65536-slot
+ *             table (in practice: more than a few tens of thousands). */
+enum edx_precision { EDX_C128 = 0, EDX_C64 = 1, EDX_C64D = 2, EDX_C128I = 3 };
 
 /* Term records, layout-compatible with T_tensor2 / T_tensor4
  * (src/m_types.f90:7-20): 1-based spin-orbital indices, complex value. */
@@ -233,7 +240,7 @@ void edx_krylov_batch_free(edx_krylov_batch *state);
  * independent single-vector build_krylov_mp recurrences (src/lanczos.f90:
  * 266-298), each over its own square matrix, advanced together so that one
  * step of every live member is one SpMV launch plus four small launches.
- * Precision EDX_C128 or EDX_C64D for every member; each member keeps up to
+ * Precision EDX_C128, EDX_C64D or EDX_C128I for every member; each member keeps up to
  * max_steps (alpha, beta).  add copies and normalizes d_v0 (length rows(h))
  * and returns a member id; h must stay alive while it is a member.  advance
  * takes nsteps steps of every member still running (live = members still

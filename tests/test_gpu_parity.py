@@ -807,3 +807,64 @@ def test_group_graph_replays_equal_eager_launches(core, monkeypatch):
     graph = run()
     for (ae, be, ne, _), (ag, bg, ng, _) in zip(eager, graph):
         assert len(ae) > 0 and np.array_equal(ae, ag) and np.array_equal(be, bg) and ne == ng
+
+
+# ---------------------------------------------------------- indexed exact form
+
+def test_indexed_form_products_are_exact(core):
+    m, fn, bn, A, R = _siam_hn(core)
+    A.prepare("c128i")
+    rng = np.random.default_rng(19)
+    x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    y0 = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    ref = o.matvec(R, x, y0)
+    scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
+    y = A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128i").cpu().numpy()
+    assert np.abs(y - ref).max() < 1e-14 * scale            # FP64 rounding only
+    # Krylov coefficients: those of the exact operator
+    a1, b1, n1 = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+    ra, rb, rn = o.build_krylov(R, x, 40)
+    np.testing.assert_allclose(a1[:25], ra[:25], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(b1[:25], rb[:25], rtol=1e-9, atol=1e-9)
+    # batched (K < 8 and K >= 8 schedules) and grouped recurrences accept it
+    V = rng.normal(size=(10, A.n)) + 0j
+    for K in (3, 10):
+        al, be, nr = core.build_krylov_batch(A, _to_dev(core, V[:K]), 30, precision="c128i")
+        rk = o.build_krylov(R, V[K - 1], 30)
+        np.testing.assert_allclose(al[K - 1][:20], rk[0][:20], rtol=1e-9, atol=1e-9)
+    g = core.KrylovGroup("c128i", max_steps=64)
+    mid = g.add(A, _to_dev(core, V[0]))
+    g.advance(30)
+    ag = g.coefficients(mid)[0]
+    np.testing.assert_allclose(ag[:20], o.build_krylov(R, V[0], 30)[0][:20], rtol=1e-9, atol=1e-9)
+    g.close()
+    # ground state in the indexed form equals FP64
+    ev, _, _ = core.ground_state(A, nev=1, precision="c128i", tol=1e-12)
+    assert abs(ev[0] - np.linalg.eigvalsh(R.todense())[0]) < 1e-10
+
+
+def test_indexed_form_many_values_and_capacity(core):
+    from edrixs_amd import siam
+    # dense random terms, 14 orbitals: 32,484 distinct off-diagonal values still fit
+    n, N = 14, 7
+    rng = np.random.default_rng(23)
+    ht = siam.emat_terms(H.random_hermitian_emat(n, rng, density=1.0))
+    ut = siam.umat_terms(H.random_hermitian_umat(n, rng, nnz=3000))
+    fock = H.fock_words(n, N)
+    A = core.build_ham_i(core.Basis.from_words(fock, n), ht, ut)
+    A.prepare("c128i")
+    R = o.build_ham_i(fock, ht, ut)
+    x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    y = A.matvec(_to_dev(core, x), precision="c128i").cpu().numpy()
+    scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
+    assert np.abs(y - o.matvec(R, x)).max() < 1e-14 * scale
+    # 16 orbitals: 149,766 distinct values do not fit the table
+    rng = np.random.default_rng(23)
+    ht = siam.emat_terms(H.random_hermitian_emat(16, rng, density=1.0))
+    ut = siam.umat_terms(H.random_hermitian_umat(16, rng, nnz=6000))
+    B = core.build_ham_i(core.Basis.combinations(16, 8), ht, ut)
+    with pytest.raises(Exception, match="distinct off-diagonal values"):
+        B.prepare("c128i")
+    # the matrix stays usable in the other forms
+    xb = _to_dev(core, np.ones(B.n) + 0j)
+    assert np.isfinite(B.matvec(xb, precision="c64d").cpu().numpy()).all()
