@@ -7,13 +7,17 @@ budget next to the requests already running, and leaves as soon as its
 spectra have converged, freeing its memory for the next one.  Every Krylov
 recurrence of every running request (the three XAS dipole components, the
 RIXS final states of each (incident energy, polarization)) is a member of
-one :class:`core.KrylovGroup`: a step of all of them is one SpMV launch over
-all their rows, whatever the mix of sizes, so requests that arrive while
-others run share the device instead of queueing behind them.
+the :class:`core.KrylovGroup` of its request's size class (norb): a step of
+all of them is one SpMV launch over all their rows, so requests that arrive
+while others run share the device instead of queueing behind them.  Each
+round gives every size class about ``slice_s`` of device time (its step
+count is set from its measured step time), so small requests are not held to
+the pace of the largest ones.
 
 Per-request work that is not a Krylov step -- CSR builds, the ground state
 (Lanczos), the edge estimate and the RIXS multi-shift solve -- runs between
-group steps, one request at a time.
+rounds of group steps, one request per round; the smallest queued request is
+set up first unless one has waited longer than ``max_wait_s``.
 
 Results do not depend on the mix: a member's coefficients are the same in any
 group, and a spectrum is taken at the first multiple of ``check_every`` of
@@ -86,7 +90,7 @@ class Engine:
 
     def __init__(self, budget_gb=64.0, precision="c128i", check_every=100, chunk=25,
                  cf_tol=1e-6, min_steps=200, max_steps=4000, n_in=4, npol=1, nw=1000,
-                 rixs_tol=1e-6, rixs_maxit=4000):
+                 rixs_tol=1e-6, rixs_maxit=4000, slice_s=0.1, max_wait_s=10.0):
         self.budget = budget_gb * 1e9
         self.precision = precision
         self.check_every = check_every
@@ -100,8 +104,20 @@ class Engine:
         self.eloss = np.linspace(0.0, 10.0, nw)
         self.rixs_tol = rixs_tol
         self.rixs_maxit = rixs_maxit
-        self.group = core.KrylovGroup(precision, max_steps=max_steps)
+        self.slice_s = slice_s
+        self.max_wait_s = max_wait_s
+        self.groups = {}        # norb -> KrylovGroup of that size class
+        self.step_s = {}        # norb -> measured seconds per group step
         self.used = 0.0
+
+    def group_for(self, norb):
+        if norb not in self.groups:
+            self.groups[norb] = core.KrylovGroup(self.precision, max_steps=self.max_steps)
+        return self.groups[norb]
+
+    @property
+    def members(self):
+        return [m for g in self.groups.values() for m in g.members]
 
     # ------------------------------------------------------------ per request
     def estimate_bytes(self, req):
@@ -132,11 +148,13 @@ class Engine:
         gs = vecs[0].contiguous()
         req.egs = egs
         Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
-        st = {"req": req, "mats": [Hi, Hn], "members": [], "grid": None, "gamma": None}
+        group = self.group_for(req.norb)
+        st = {"req": req, "mats": [Hi, Hn], "members": [], "grid": None, "gamma": None,
+              "group": group}
         if req.kind == "xas":
             for k in range(3):
                 D = core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[k])
-                st["members"].append(self.group.add(Hn, D.matvec(gs)))
+                st["members"].append(group.add(Hn, D.matvec(gs)))
             st["grid"], st["gamma"] = self.omega, GAMMA_C
             st["combine"] = "average"
         else:
@@ -157,7 +175,7 @@ class Engine:
                                               tol=self.rixs_tol, maxit=self.rixs_maxit,
                                               precision=p)
                 for k in range(len(sigma)):
-                    st["members"].append(self.group.add(Hi, Y[k].contiguous()))
+                    st["members"].append(group.add(Hi, Y[k].contiguous()))
             st["grid"], st["gamma"] = self.eloss, GAMMA_F
             st["combine"] = "each"
             st["mats"] = [Hi]          # H_n no longer needed
@@ -173,7 +191,7 @@ class Engine:
         for mid in st["members"]:
             if mid in st["result"]:
                 continue
-            a, b, nrm, running = self.group.coefficients(mid)
+            a, b, nrm, running = st["group"].coefficients(mid)
             steps = len(a)
             while st["checked"][mid] + ce <= steps or (not running and st["checked"][mid] < steps):
                 c = min(st["checked"][mid] + ce, steps)
@@ -200,7 +218,7 @@ class Engine:
             req.spectra = [s for s, _ in res]
         req.steps = [c for _, c in res]
         for mid in st["members"]:
-            self.group.remove(mid)
+            st["group"].remove(mid)
         st["mats"].clear()
 
     # -------------------------------------------------------------- the loop
@@ -219,12 +237,22 @@ class Engine:
             t = now()
             while pending and (not realtime or pending[0].arrival <= t):
                 queue.append(pending.pop(0))
-            # admission in arrival order against the HBM budget
-            while queue:
-                need = self.estimate_bytes(queue[0])
-                if active and self.used + need > self.budget:
-                    break
-                req = queue.pop(0)
+            # admission against the HBM budget, one setup per round so running
+            # requests keep stepping between setups; the smallest queued request
+            # goes first unless one has waited longer than max_wait_s
+            if queue:
+                t = now()
+                old = [r for r in queue if t - r.arrival > self.max_wait_s]
+                req = min(old, key=lambda r: r.arrival) if old else \
+                    min(queue, key=lambda r: (r.norb, r.arrival))
+                need = self.estimate_bytes(req)
+                if not active or self.used + need <= self.budget:
+                    queue.remove(req)
+                else:
+                    req = None
+            else:
+                req = None
+            if req is not None:
                 req.admitted = now()
                 st = self.setup(req)
                 st["bytes"] = need
@@ -234,7 +262,16 @@ class Engine:
                     log("admit r%d %s norb=%d at %.2f s (%d running)" %
                         (req.rid, req.kind, req.norb, req.admitted, len(active)))
             if active:
-                self.group.advance(self.chunk)
+                # every size class gets about slice_s of device time per round:
+                # small requests take many cheap steps while large ones take few
+                for norb in sorted({st["req"].norb for st in active}):
+                    g = self.groups[norb]
+                    per = self.step_s.get(norb)
+                    n = self.chunk if per is None else int(min(1000, max(1, self.slice_s / per)))
+                    t1 = time.perf_counter()
+                    g.advance(n)
+                    dt = (time.perf_counter() - t1) / n
+                    self.step_s[norb] = dt if per is None else 0.5 * (per + dt)
                 still = []
                 for st in active:
                     if self.poll(st):
@@ -250,7 +287,7 @@ class Engine:
                     else:
                         still.append(st)
                 active = still
-            elif pending:
+            elif pending and not queue:
                 time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
         return done
 

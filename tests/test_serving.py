@@ -55,4 +55,4 @@ def test_serving_report_and_budget_admission():
     assert rep["requests"] == len(reqs) and rep["spectra"] == sum(r.nspectra for r in done)
     assert rep["latency_p50_s"] <= rep["latency_p99_s"] <= rep["latency_max_s"]
     assert all(r.done >= r.admitted >= r.arrival for r in done)
-    assert abs(eng.used) < 1.0 and not eng.group.members
+    assert abs(eng.used) < 1.0 and not eng.members
