@@ -1,22 +1,36 @@
-// Launch-shape autotuner for the single-vector SpMV (part of edx_kernels.hip).
+// Launch-shape autotuner (part of edx_kernels.hip).
 //
-// The best number of lanes per row depends on the row-length distribution and
-// on how well the gathers of x hit the caches, which the mean row length does
-// not predict (H_n at norb=24: 24 nonzeros per row, yet 32 lanes per row beat
-// 8 because a row's loads are then all in flight at once).  So each matrix is
-// timed once per precision on itself: every supported G runs a warm-up and
-// three timed launches on scratch vectors, and the fastest is kept in the
-// matrix (spmv_lanes / spmv_lanes32 / spmv_lanes64d), which every later
-// single-vector product of that matrix uses.  The timed launches carry the
-// epilogue of a Krylov step (addend and dot partials): without it the ranking
-// of the choices differs from the one inside the solvers.  Matrices below TUNE_MIN_ROWS
-// keep the mean-row-length rule: their products are too short to time and too
-// cheap to matter.  EDX_NO_TUNE=1 in the environment turns tuning off.
+// Single-vector SpMV: the best number of lanes per row depends on the
+// row-length distribution and on how well the gathers of x hit the caches,
+// which the mean row length does not predict.  Multi-RHS products: the
+// schedule (row-split with 4-RHS tiles and 8 or 16 lanes per row, or
+// RHS-parallel with tiles of 16 to 128 right-hand sides) trades reading the
+// matrix once per tile against keeping each tile's slice of the vector block
+// in the last-level cache, which depends on the vector length and K.  So each
+// choice is timed on the matrix itself -- a warm-up and TUNE_REPS timed
+// launches on scratch vectors, with the epilogue of a Krylov step (addend and
+// dot partials), since without it the ranking differs from the one inside the
+// solvers -- and the fastest is kept.  Choices are cached by matrix shape
+// (rows, columns, nonzeros), precision and K, so a matrix rebuilt with the
+// same structure reuses them.  Matrices below TUNE_MIN_ROWS keep the rules;
+// EDX_NO_TUNE=1 in the environment turns tuning off.
 
 namespace {
 
-constexpr int64_t TUNE_MIN_ROWS = 1 << 17;
 constexpr int TUNE_REPS = 5;
+
+using TuneKey = std::tuple<int64_t, int64_t, int64_t, int, int>;   // m, n, nnz, precision, K
+std::map<TuneKey, int> g_tuned;   // single vector: lanes; multi-RHS: rl * 256 + p
+std::mutex g_tune_mu;
+
+TuneKey tune_key(const edx_csr *a, int precision, int K) {
+    return TuneKey{a->m, a->n, a->nnz, precision, K};
+}
+
+bool tuning_off() {
+    const char *off = getenv("EDX_NO_TUNE");
+    return off && off[0] == '1';
+}
 
 int *lanes_slot(edx_csr *a, int precision) {
     if (precision == EDX_C64) return &a->spmv_lanes32;
@@ -32,8 +46,15 @@ namespace edx {
 int tune_spmv(edx_csr *a, int precision) {
     int *slot = lanes_slot(a, precision);
     if (*slot) return EDX_OK;
-    const char *off = getenv("EDX_NO_TUNE");
-    if (a->m < TUNE_MIN_ROWS || (off && off[0] == '1')) return EDX_OK;
+    if (a->m < TUNE_MIN_ROWS || tuning_off()) return EDX_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(tune_key(a, precision, 1));
+        if (it != g_tuned.end()) {
+            *slot = it->second;
+            return EDX_OK;
+        }
+    }
     const size_t sx = precision == EDX_C64 ? sizeof(float2) : sizeof(double2);
     void *x = dmalloc((size_t)a->n * sx, "tune x");
     void *y = dmalloc((size_t)a->m * sizeof(double2), "tune y");
@@ -85,6 +106,10 @@ int tune_spmv(edx_csr *a, int precision) {
         }
     }
     *slot = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(a, precision, 1)] = best;
+    }
 done:
     g_prof = prof_saved;
     for (hipEvent_t e : ev)
@@ -92,6 +117,83 @@ done:
     dfree(x);
     dfree(y);
     dfree(d);
+    dfree(part);
+    return rc;
+}
+
+int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
+    if (a->m < TUNE_MIN_ROWS || tuning_off()) return EDX_OK;
+    const TuneKey key = tune_key(a, precision, K);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            *sched = BSched{it->second / 256, it->second % 256};
+            return EDX_OK;
+        }
+    }
+    std::vector<BSched> cand = {BSched{0, 8}, BSched{0, 16}};
+    if (K >= 8)
+        for (int npl : {1, 2, 4, 8})
+            if (16 * npl <= ((K + 15) / 16) * 16) cand.push_back(BSched{1, npl});
+    const size_t nk = (size_t)a->n * K;
+    double2 *X = (double2 *)dmalloc(nk * sizeof(double2), "tune X");
+    float2 *X32 = precision == EDX_C64 ? (float2 *)dmalloc(nk * sizeof(float2), "tune X32") : nullptr;
+    double2 *Y = (double2 *)dmalloc((size_t)a->m * K * sizeof(double2), "tune Y");
+    double2 *part = (double2 *)dmalloc((size_t)bspmm_blocks(a->m) * K * sizeof(double2), "tune partials");
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = EDX_OK;
+    BSched best = *sched;
+    float best_ms = 0.f;
+    bool have = false;
+    const bool prof_saved = g_prof;
+    g_prof = false;
+    if (!X || !Y || !part || (precision == EDX_C64 && !X32)) {
+        rc = EDX_ERR_NOMEM;
+        goto done;
+    }
+    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess ||
+        hipMemsetAsync(X, 0, nk * sizeof(double2), stream()) != hipSuccess ||
+        hipMemsetAsync(Y, 0, (size_t)a->m * K * sizeof(double2), stream()) != hipSuccess ||
+        (X32 && hipMemsetAsync(X32, 0, nk * sizeof(float2), stream()) != hipSuccess)) {
+        rc = set_error(EDX_ERR_HIP, "tune: setup failed");
+        goto done;
+    }
+    for (const BSched &c : cand) {
+        BEpi e{};
+        e.y = Y;
+        e.z = Y;
+        e.dvec = X;
+        e.partials = part;
+        if ((rc = bspmm_launch(a, precision, K, X, X32, e, c)) != EDX_OK) goto done;
+        (void)hipEventRecord(ev[0], stream());
+        for (int r = 0; r < TUNE_REPS; ++r)
+            if ((rc = bspmm_launch(a, precision, K, X, X32, e, c)) != EDX_OK) goto done;
+        (void)hipEventRecord(ev[1], stream());
+        if (hipEventSynchronize(ev[1]) != hipSuccess) {
+            rc = set_error(EDX_ERR_HIP, "tune: synchronize failed");
+            goto done;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (!have || ms < best_ms) {
+            best = c;
+            best_ms = ms;
+            have = true;
+        }
+    }
+    *sched = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = best.rl * 256 + best.p;
+    }
+done:
+    g_prof = prof_saved;
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    dfree(X);
+    dfree(X32);
+    dfree(Y);
     dfree(part);
     return rc;
 }

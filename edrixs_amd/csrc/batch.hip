@@ -118,7 +118,8 @@ __global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t
 }
 
 // RHS-parallel schedule: 16 groups of 16 lanes per block, one row per group,
-// lane l owns right-hand sides k0 + l + 16 q (q < NPL), k0 = blockIdx.y * 128.
+// lane l owns right-hand sides k0 + l + 16 q (q < NPL); a pass (blockIdx.y)
+// covers a tile of 16 NPL right-hand sides, k0 = blockIdx.y * 16 NPL.
 template <int NPL, class VP, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ cols, VP vals,
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t
     __shared__ double2 red[16 * 16 * NPL];
     const int l = threadIdx.x & 15;
     const int grp = threadIdx.x >> 4;
-    const int k0 = blockIdx.y * RL_MAXK;
+    const int k0 = blockIdx.y * 16 * NPL;   // this pass's tile of 16 NPL right-hand sides
     const int64_t ngroups = (int64_t)gridDim.x * 16;
     double2 dot[NPL];
 #pragma unroll
@@ -635,34 +636,24 @@ __global__ void k_vt_index(int64_t m, const int64_t *rowptr, const int32_t *cols
 
 namespace edx {
 
-// Y = epilogue(H X) for K interleaved RHS.  For C64, X32 holds the complex64
-// copy of X (interleaved); X64 is always the complex128 X (split diagonal and
-// shift terms read it).
-int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
-          const BEpi &e, int *nblocks_out) {
-    if (K == 1 && a->m >= TUNE_MIN_ROWS && !e.c2) {
-        // one right-hand side of a large matrix: the tuned single-vector kernel
-        // (its vector then fits the last-level cache, which an interleaved
-        // block of several does not)
-        SpmvEpi se{};
-        se.y = e.y;
-        se.z = e.z;
-        se.c0 = e.c0;
-        se.c1 = e.c1;
-        se.dvec = e.dvec;
-        se.partials = e.partials;
-        return spmv_launch(a, precision, precision == EDX_C64 ? (const void *)X32 : (const void *)X64,
-                           se, 0, nblocks_out);
-    }
+// schedule of a multi-RHS product: row-split (rl = 0, p lanes per row) or
+// RHS-parallel (rl = 1, tiles of 16 p right-hand sides)
+struct BSched {
+    int rl;
+    int p;
+};
+int tune_spmv(edx_csr *a, int precision);
+int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched);
+
+// One launch of the multi-RHS product with an explicit schedule: row-split
+// (sched.rl == 0: sched.p lanes per row, tiles of BKT right-hand sides) or
+// RHS-parallel (sched.rl == 1: tiles of 16 * sched.p right-hand sides).
+int bspmm_launch(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
+                 const BEpi &e, BSched sched) {
     const int nb = bspmm_blocks(a->m);
-    if (nblocks_out) *nblocks_out = nb;
-    if (a->m == 0 || K == 0) return EDX_OK;
-    hipEvent_t pe = prof_start(precision, K >= 8 ? EDX_KERNEL_SPMM_RL : EDX_KERNEL_SPMM_IL,
-                               spmv_bytes_k(a, precision, K, e.z != nullptr, e.dvec != nullptr));
-    if (K >= 8) {
-        const int kp = K < RL_MAXK ? K : RL_MAXK;
-        const int npl = kp <= 16 ? 1 : (kp <= 32 ? 2 : (kp <= 64 ? 4 : 8));
-        dim3 grid((unsigned)nb, (unsigned)((K + RL_MAXK - 1) / RL_MAXK));
+    if (sched.rl) {
+        const int npl = sched.p;
+        dim3 grid((unsigned)nb, (unsigned)((K + 16 * npl - 1) / (16 * npl)));
 #define EDX_RL(NP)                                                                                 \
     case NP:                                                                                       \
         if (precision == EDX_C128)                                                                 \
@@ -687,11 +678,11 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
             EDX_RL(2)
             EDX_RL(4)
             EDX_RL(8)
-            default: return set_error(EDX_ERR_ARG, "bad rhs count %d", K);
+            default: return set_error(EDX_ERR_ARG, "bad RHS tile %d", 16 * npl);
         }
 #undef EDX_RL
     } else {
-        const int G = pick_lanes(a);
+        const int G = sched.p;
         dim3 grid((unsigned)nb, (unsigned)((K + BKT - 1) / BKT));
 #define EDX_BSPMM(GG)                                                                              \
     case GG:                                                                                       \
@@ -722,6 +713,46 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
 #undef EDX_BSPMM
     }
     EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
+// The schedule the mean row length and K suggest (used below TUNE_MIN_ROWS
+// rows, where products are too short to time).
+BSched bspmm_rule(const edx_csr *a, int K) {
+    if (K >= 8) {
+        const int kp = K < RL_MAXK ? K : RL_MAXK;
+        return BSched{1, kp <= 16 ? 1 : (kp <= 32 ? 2 : (kp <= 64 ? 4 : 8))};
+    }
+    return BSched{0, pick_lanes(a)};
+}
+
+// Y = epilogue(H X) for K interleaved RHS.  For C64, X32 holds the complex64
+// copy of X (interleaved); X64 is always the complex128 X (split diagonal and
+// shift terms read it).
+int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
+          const BEpi &e, int *nblocks_out) {
+    if (K == 1 && a->m >= TUNE_MIN_ROWS && !e.c2) {
+        // one right-hand side of a large matrix: the tuned single-vector kernel
+        // (its vector then fits the last-level cache, which an interleaved
+        // block of several does not)
+        SpmvEpi se{};
+        se.y = e.y;
+        se.z = e.z;
+        se.c0 = e.c0;
+        se.c1 = e.c1;
+        se.dvec = e.dvec;
+        se.partials = e.partials;
+        return spmv_launch(a, precision, precision == EDX_C64 ? (const void *)X32 : (const void *)X64,
+                           se, 0, nblocks_out);
+    }
+    const int nb = bspmm_blocks(a->m);
+    if (nblocks_out) *nblocks_out = nb;
+    if (a->m == 0 || K == 0) return EDX_OK;
+    BSched sched = bspmm_rule(a, K);
+    if (a->m >= TUNE_MIN_ROWS) EDX_TRY(tune_bspmm(a, precision, K, &sched));
+    hipEvent_t pe = prof_start(precision, sched.rl ? EDX_KERNEL_SPMM_RL : EDX_KERNEL_SPMM_IL,
+                               spmv_bytes_k(a, precision, K, e.z != nullptr, e.dvec != nullptr));
+    EDX_TRY(bspmm_launch(a, precision, K, X64, X32, e, sched));
     prof_stop(pe);
     return EDX_OK;
 }

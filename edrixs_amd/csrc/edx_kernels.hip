@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -12,11 +15,11 @@
 #include "basis.hip"
 #include "csr_build.hip"
 #include "spmv.hip"
-#include "autotune.hip"
 #include "blas.hip"
 #include "krylov.hip"
 #include "mf.hip"
 #include "batch.hip"
+#include "autotune.hip"
 #include "multishift.hip"
 #include "group.hip"
 #include "observables.hip"

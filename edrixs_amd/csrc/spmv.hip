@@ -208,6 +208,10 @@ __global__ void k_to_c64_vec(int64_t n, const double2 *in, float2 *out, const in
 // the one-block scalar kernels that consume them stay short.
 constexpr int FOLD_N = 256;
 
+// Matrices with fewer rows keep the launch-shape rules: their products are
+// too short to time and too cheap to matter.
+constexpr int64_t TUNE_MIN_ROWS = 1 << 17;
+
 __global__ __launch_bounds__(256) void k_fold_partials(double2 *partials, int nb, const int *run) {
     if (run && *run == 0) return;
     const int chunk = (nb + FOLD_N - 1) / FOLD_N;
