@@ -1002,6 +1002,9 @@ int kb_advance(edx_krylov_batch *q, int nsteps) {
     const BKrylov &st = q->st;
     double2 *v[2] = {q->vb.as<double2>(), q->vb.as<double2>() + (size_t)n * K};
     const int nbe = elem_grid_x(n);
+    const int nbv = vec_blocks(n);
+    // the same condition routes the product to the single-vector SpMV (bspmm)
+    const bool single = K == 1 && q->h->m >= TUNE_MIN_ROWS;
     const int end = q->done + nsteps;
     int checked = q->done;
     for (int i = q->done + 1; i <= end; ++i) {
@@ -1017,11 +1020,22 @@ int kb_advance(edx_krylov_batch *q, int nsteps) {
         EDX_TRY(edx::bspmm(q->h, q->precision, K, v[curr], q->x32.as<float2>(), e, &nb));
         hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nb, K, st.sums);
         hipLaunchKernelGGL(k_bk_alpha, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
-        ELAUNCH(k_baxpy_norm, n, K, n, K, v[prev], v[curr], st.alpha + (size_t)(i - 1) * K,
-                q->part.as<double2>());
-        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nbe, K, st.sums);
-        hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
-        ELAUNCH(k_bdiv_beta, n, K, n, K, v[prev], st.beta + (size_t)i * K, st.run + (size_t)i * K);
+        if (single) {   // one vector of a large matrix: the plain vector kernels
+            hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
+                               st.alpha + (i - 1), q->part.as<double2>(), nullptr);
+            hipLaunchKernelGGL(k_breduce, dim3(1), dim3(256), 0, s, q->part.as<double2>(), nbv, 1,
+                               st.sums);
+            hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
+            hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[prev], st.beta + i,
+                               st.run + i);
+        } else {
+            ELAUNCH(k_baxpy_norm, n, K, n, K, v[prev], v[curr], st.alpha + (size_t)(i - 1) * K,
+                    q->part.as<double2>());
+            hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nbe, K,
+                               st.sums);
+            hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
+            ELAUNCH(k_bdiv_beta, n, K, n, K, v[prev], st.beta + (size_t)i * K, st.run + (size_t)i * K);
+        }
         EDX_LAUNCH_CHECK();
         std::swap(q->curr, q->prev);
         q->done = i;
