@@ -1,0 +1,68 @@
+"""Host-side logic that needs no device: serving request generation and
+reports, the bench's reading of measured traffic, precision names."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_make_requests_poisson_and_parameter_ranges():
+    from edrixs_amd import serving
+    reqs = serving.make_requests(400, rate=20.0, sizes=(16, 18, 20, 22, 24), seed=5)
+    t = np.array([r.arrival for r in reqs])
+    assert t[0] == 0.0 and np.all(np.diff(t) >= 0)
+    gaps = np.diff(t)
+    assert abs(gaps.mean() - 1 / 20.0) < 0.25 / 20.0          # mean inter-arrival 1/rate
+    assert {r.norb for r in reqs} == {16, 18, 20, 22, 24}
+    assert all(8 <= r.F2 <= 12 and 0.5 <= r.ten_dq <= 2 and 1 <= r.V <= 2 for r in reqs)
+    assert {r.kind for r in reqs} == {"xas", "rixs"}
+    again = serving.make_requests(400, rate=20.0, seed=5)
+    assert [(r.kind, r.norb, r.seed) for r in again] == [(r.kind, r.norb, r.seed) for r in reqs]
+
+
+def test_serving_report_percentiles():
+    from edrixs_amd import serving
+    done = []
+    for i, lat in enumerate([1.0, 2.0, 3.0, 10.0]):
+        r = serving.Request(rid=i, kind="xas", norb=16, seed=i, arrival=float(i))
+        r.done = r.arrival + lat
+        r.spectra = [np.zeros(3)]
+        done.append(r)
+    rep = serving.report(done)
+    assert rep["requests"] == 4 and rep["spectra"] == 4
+    assert rep["latency_p50_s"] == pytest.approx(2.5)
+    assert rep["latency_max_s"] == 10.0 and rep["latency_p99_s"] <= 10.0
+    assert rep["makespan_s"] == pytest.approx(13.0)
+
+
+def test_serving_memory_estimate_grows_with_size():
+    from edrixs_amd import serving
+    eng = serving.Engine.__new__(serving.Engine)     # no device: only the estimator
+    eng.precision, eng.n_in, eng.npol = "c128i", 4, 1
+    est = [eng.estimate_bytes(serving.Request(0, k, n, 0, 0.0))
+           for k in ("xas", "rixs") for n in (16, 20, 24)]
+    assert est[0] < est[1] < est[2] and est[3] < est[4] < est[5]
+    n_n = 6 * math.comb(24, 13)
+    assert est[2] > n_n * 26 * 26                    # at least the H_n CSR bytes
+
+
+def test_bench_reads_measured_traffic():
+    sys.path.insert(0, ROOT)
+    import bench
+    for kernel, form in (("k_spmv", "c128i"), ("k_spmv", "c64d"), ("k_spmm_rl", "c128i")):
+        v = bench.measured_traffic(kernel, form, 24)
+        assert v is not None and 1e9 < v < 1e11
+    assert bench.measured_traffic("k_spmv", "c128i", 20) is None
+    assert bench.measured_traffic("k_nope", "c128i", 24) is None
+
+
+def test_precision_names():
+    from edrixs_amd import core
+    assert core._prec("c128") == core._prec("fp64") == 0
+    assert core._prec("c64") == 1 and core._prec("mixed") == 2 and core._prec("indexed") == 3
+    with pytest.raises(ValueError):
+        core._prec("fp16")
