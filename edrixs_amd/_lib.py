@@ -91,9 +91,12 @@ _SIGS = {
     "edx_profile": (_I, [_I]),
     "edx_profile_read": (_I, [_I, _I, ctypes.POINTER(_I64), ctypes.POINTER(_D),
                               ctypes.POINTER(_D)]),
+    "edx_profile_read_flops": (_I, [_I, _I, ctypes.POINTER(_D)]),
     "edx_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,
                               ctypes.POINTER(_I)]),
     "edx_lanczos_refine": (_I, [_P, _I, _I, _I, _D, _I, _P, _I, _P, _P, ctypes.POINTER(_I)]),
+    "edx_block_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _P, _P,
+                                    ctypes.POINTER(_I), _P]),
     "edx_build_krylov": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
                               ctypes.POINTER(_D)]),
     "edx_minres": (_I, [_P, _I, _P, _P, _D, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),

@@ -14,6 +14,7 @@ The functions mirror the Fortran routines they replace:
 ``build_transop_f``         ``build_transop_f``  src/ham.f90:915-1115
 ``CSRMatrix.spmv``          ``matvec_csr``       src/spmv.f90:73-103 (y += A x)
 ``lanczos_eigs``            ``diag_ham_lanczos`` src/lanczos.f90:6-167
+``block_lanczos_eigs``      ``diag_ham_lanczos`` src/lanczos.f90:6-167 (8-vector blocks)
 ``build_krylov``            ``build_krylov_mp``  src/lanczos.f90:210-303
 ``minres``                  ``pminres_csr``      src/linsys.f90:3-128
 ``denmat``                  density matrix       src/ed_driver.f90:219-248
@@ -279,6 +280,21 @@ def lanczos_refine(H, start, nev, maxiter=200, tol=1e-10, nvec=None, precision="
     _lib.call("edx_lanczos_refine", H._h, _prec(precision), nev, maxiter, float(tol),
               st.shape[0], _dptr(st), nvec, _ptr(ev), _dptr(vecs), ctypes.byref(it))
     return ev, vecs[:nvec], it.value
+
+
+def block_lanczos_eigs(H, nev, maxblocks=60, tol=1e-10, seed=1, nvec=None, precision="c128"):
+    """Lowest nev eigenpairs by block Lanczos (blocks of 8, reorthogonalization
+    on the FP64 matrix cores).  Returns (evals, evecs (nvec x n device tensor),
+    block steps, residual estimates)."""
+    T = torch()
+    nvec = nev if nvec is None else nvec
+    ev = np.zeros(nev)
+    res = np.zeros(nev)
+    vecs = T.zeros((max(nvec, 1), H.n), dtype=T.complex128, device=device())
+    nb = ctypes.c_int()
+    _lib.call("edx_block_lanczos_eigs", H._h, _prec(precision), nev, maxblocks, float(tol),
+              int(seed), nvec, _ptr(ev), _dptr(vecs), ctypes.byref(nb), _ptr(res))
+    return ev, vecs[:nvec], nb.value, res
 
 
 def ground_state(H, nev=1, precision="c128", maxiter=500, tol=1e-10, seed=1, deflate=False,
@@ -641,6 +657,7 @@ def profile(enable=True):
 
 
 KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4}
+BLOCK_KERNELS = {"k_bproj": 5, "k_bupd": 6}   # matrix-core kernels (recorded as c128)
 
 
 def profile_read(precision="c128", kernel=None):
@@ -649,7 +666,14 @@ def profile_read(precision="c128", kernel=None):
     n = ctypes.c_int64()
     ms = ctypes.c_double()
     by = ctypes.c_double()
-    k = -1 if kernel is None else KERNELS[kernel]
+    k = -1 if kernel is None else {**KERNELS, **BLOCK_KERNELS}[kernel]
     _lib.call("edx_profile_read", _prec(precision), k, ctypes.byref(n), ctypes.byref(ms),
               ctypes.byref(by))
     return n.value, ms.value, by.value
+
+
+def profile_read_flops(kernel, precision="c128"):
+    """Summed algorithmic flops of the recorded launches of a matrix-core kernel."""
+    f = ctypes.c_double()
+    _lib.call("edx_profile_read_flops", _prec(precision), BLOCK_KERNELS[kernel], ctypes.byref(f))
+    return f.value

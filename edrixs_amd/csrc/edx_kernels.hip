@@ -3,6 +3,7 @@
 // device code.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -22,4 +23,5 @@
 #include "autotune.hip"
 #include "multishift.hip"
 #include "group.hip"
+#include "block.hip"
 #include "observables.hip"

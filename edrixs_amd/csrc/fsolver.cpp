@@ -26,7 +26,7 @@
 #include "common.h"
 
 namespace edx {
-int herm_eig(int n, std::complex<double> *A, double *w, std::complex<double> *Z);
+int herm_eig_lowest(int n, std::complex<double> *A, double *w, int nvec, std::complex<double> *Z);
 }
 
 namespace {
@@ -352,18 +352,20 @@ int ed_driver() {   // src/ed_driver.f90:1-281
         std::vector<int32_t> cols(nnz);
         std::vector<std::complex<double>> vals(nnz);
         EDX_TRY(edx_csr_download(H.a, rp.data(), cols.data(), vals.data()));
-        std::vector<std::complex<double>> A((size_t)ndim * ndim, 0.0), Z((size_t)ndim * ndim);
+        // only the nvector lowest eigenvectors are kept (src/ed_driver.f90:147-148)
+        const int nz = std::max(nvector, 1);
+        std::vector<std::complex<double>> A((size_t)ndim * ndim, 0.0), Z((size_t)ndim * nz);
         for (int64_t r = 0; r < ndim; ++r)
             for (int64_t k = rp[r]; k < rp[r + 1]; ++k) A[(size_t)r * ndim + cols[k]] = vals[k];
         std::vector<double> w(ndim);
-        EDX_CHECK(edx::herm_eig((int)ndim, A.data(), w.data(), Z.data()) == 0, EDX_ERR_NUMERIC,
-                  "dense eigensolver did not converge");
+        EDX_CHECK(edx::herm_eig_lowest((int)ndim, A.data(), w.data(), nvector, Z.data()) == 0,
+                  EDX_ERR_NUMERIC, "dense eigensolver did not converge");
         for (int k = 0; k < neval; ++k) evals[k] = w[k];
         std::vector<double2> hv((size_t)ndim * nvector);
         for (int j = 0; j < nvector; ++j)
             for (int64_t r = 0; r < ndim; ++r)
-                hv[(size_t)j * ndim + r] = make_double2(Z[(size_t)r * ndim + j].real(),
-                                                        Z[(size_t)r * ndim + j].imag());
+                hv[(size_t)j * ndim + r] = make_double2(Z[(size_t)r * nvector + j].real(),
+                                                        Z[(size_t)r * nvector + j].imag());
         EDX_HIP(hipMemcpy(vecs.p, hv.data(), hv.size() * sizeof(double2), hipMemcpyHostToDevice));
     } else {   // Lanczos (ed_solver=1; ed_solver=2 is served by the same solver)
         int nev = std::max(neval, nvector);

@@ -253,16 +253,17 @@ struct ProfRec {
     int kind;      // precision
     int kernel;    // EDX_KERNEL_*
     double bytes;
+    double flops;
 };
 static bool g_prof = false;
 static std::vector<hipEvent_t> g_ev;      // pairs: 2k start, 2k+1 stop
 static std::vector<ProfRec> g_rec;
 constexpr size_t PROF_CAP = 16384;
 
-static hipEvent_t prof_start(int kind, int kernel, double bytes) {
+static hipEvent_t prof_start(int kind, int kernel, double bytes, double flops = 0.0) {
     if (!g_prof || g_rec.size() >= PROF_CAP) return nullptr;
     if (g_ev.size() < 2 * PROF_CAP) return nullptr;
-    g_rec.push_back(ProfRec{kind, kernel, bytes});
+    g_rec.push_back(ProfRec{kind, kernel, bytes, flops});
     hipEvent_t e = g_ev[2 * (g_rec.size() - 1)];
     (void)hipEventRecord(e, stream());
     return e;
@@ -449,6 +450,16 @@ int edx_profile_read(int precision, int kernel, int64_t *launches, double *total
     if (launches) *launches = n;
     if (total_ms) *total_ms = ms;
     if (total_bytes) *total_bytes = by;
+    return EDX_OK;
+}
+
+int edx_profile_read_flops(int precision, int kernel, double *total_flops) {
+    EDX_CHECK(total_flops, EDX_ERR_ARG, "null argument");
+    double f = 0.0;
+    for (size_t i = 0; i < edx::g_rec.size(); ++i)
+        if (edx::g_rec[i].kind == precision && (kernel < 0 || edx::g_rec[i].kernel == kernel))
+            f += edx::g_rec[i].flops;
+    *total_flops = f;
     return EDX_OK;
 }
 

@@ -142,11 +142,16 @@ enum edx_kernel {
     EDX_KERNEL_SPMM_IL = 1,   /* k_spmm_il: K < 8 interleaved right-hand sides */
     EDX_KERNEL_SPMM_RL = 2,   /* k_spmm_rl: K >= 8, RHS-parallel lanes */
     EDX_KERNEL_SPMM_CM = 3,   /* k_spmm: column-major blocks (edx_spmm) */
-    EDX_KERNEL_MF = 4         /* k_mf: matrix-free apply (recorded as precision EDX_C128) */
+    EDX_KERNEL_MF = 4,        /* k_mf: matrix-free apply (recorded as precision EDX_C128) */
+    EDX_KERNEL_BPROJ = 5,     /* k_bproj: block projection V^H W on the matrix cores (EDX_C128) */
+    EDX_KERNEL_BUPD = 6       /* k_bupd: block update W - V C on the matrix cores (EDX_C128) */
 };
 int edx_profile(int enable);
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
                      double *total_bytes);
+/* Summed algorithmic floating-point operations of the same launches (nonzero
+ * for the matrix-core kernels only: 512 real flops per row and basis block). */
+int edx_profile_read_flops(int precision, int kernel, double *total_flops);
 
 /* ------------------------------------------------- matrix-free operators */
 
@@ -189,6 +194,19 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
 int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, double tol,
                        int nstart, const void *d_start, int nvec, double *evals, void *d_evecs,
                        int *niter_out);
+/* Lowest nev eigenpairs by block Lanczos (replaces diag_ham_lanczos,
+ * src/lanczos.f90:6-167, for several states in one Krylov space): blocks of
+ * 8 vectors, one 8-RHS SpMM per step, full reorthogonalization of each block
+ * against the whole basis on the FP64 matrix cores (block classical
+ * Gram-Schmidt twice, shifted Cholesky QR), Ritz pairs of the block-tridiagonal
+ * projection.  Stops when the nev lowest Ritz residuals |B s_last| are below
+ * tol, after maxblocks steps, or when the basis spans the space.  Needs
+ * n >= 16.  evals[nev]; d_evecs n x nvec complex128 column-major (NULL when
+ * nvec == 0); nblocks_out: block steps taken; resid_out[nev] (may be NULL):
+ * final residual estimates.  Keeps maxblocks + 1 blocks of 128 n bytes. */
+int edx_block_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxblocks, double tol,
+                           uint64_t seed, int nvec, double *evals, void *d_evecs,
+                           int *nblocks_out, double *resid_out);
 /* Krylov tridiagonalization from an unnormalised start vector (replaces
  * build_krylov_mp, src/lanczos.f90:210-303): alpha[maxn], beta[maxn]
  * (beta_1..beta_neff as written to the pole files), norm = <v|v>. */

@@ -869,3 +869,77 @@ def test_indexed_form_many_values_and_capacity(core):
     # the matrix stays usable in the other forms
     xb = _to_dev(core, np.ones(B.n) + 0j)
     assert np.isfinite(B.matvec(xb, precision="c64d").cpu().numpy()).all()
+
+
+@pytest.mark.parametrize("precision", ["c128", "c128i"])
+def test_block_lanczos_matches_dense(core, precision):
+    m = H.small_siam(nbath=2, seed=5)          # dim 924
+    fi = H.fock_words(m.norb, m.nocc)
+    A = core.build_ham_i(core.Basis.from_words(fi, m.norb), m.hop_i, m.coul_i)
+    Hd = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    e = np.linalg.eigvalsh(Hd)
+    ev, vecs, nb, res = core.block_lanczos_eigs(A, 3, maxblocks=115, tol=1e-10, seed=1,
+                                                precision=precision)
+    np.testing.assert_allclose(ev, e[:3], atol=1e-10)
+    assert (res < 1e-10).all(), (nb, res)
+    V = vecs.cpu().numpy()
+    np.testing.assert_allclose(V.conj() @ V.T, np.eye(3), atol=1e-12)
+    for k in range(3):
+        assert np.linalg.norm(Hd @ V[k] - ev[k] * V[k]) < 1e-9
+
+
+def test_block_lanczos_degenerate_multiplets_and_invariant_subspaces(core):
+    from edrixs_amd import siam
+    # d^3 with Slater F's and a cubic field (dim 120): the lowest levels are
+    # 4-, 4- and 6-fold; one block of 8 resolves each copy
+    ht = siam.emat_terms(siam.cf_cubic_d(1.5))
+    ut = siam.umat_terms(siam.umat_slater_d(4.0, 2.0, 0.5))
+    fock = H.fock_words(10, 3)
+    A = core.build_ham_i(core.Basis.from_words(fock, 10), ht, ut)
+    Hd = o.build_ham_i(fock, ht, ut).todense()
+    e = np.linalg.eigvalsh(Hd)
+    ev, vecs, nb, res = core.block_lanczos_eigs(A, 14, maxblocks=15, tol=1e-10, seed=3)
+    np.testing.assert_allclose(ev, e[:14], atol=1e-10)
+    V = vecs.cpu().numpy()
+    np.testing.assert_allclose(V.conj() @ V.T, np.eye(14), atol=1e-11)
+    assert np.abs(Hd @ V.T - V.T * ev).max() < 1e-9
+    # d^2 in a cubic field without Coulomb terms (dim 45): three levels, 15-,
+    # 24- and 6-fold.  The block Krylov space is invariant after a few steps;
+    # the deficient block is refilled with random directions, which bring the
+    # copies of the 15-fold ground level beyond the first eight
+    ut0 = siam.umat_terms(np.zeros((10, 10, 10, 10), complex))
+    fock2 = H.fock_words(10, 2)
+    A2 = core.build_ham_i(core.Basis.from_words(fock2, 10), ht, ut0)
+    e2 = np.linalg.eigvalsh(o.build_ham_i(fock2, ht, ut0).todense())
+    ev2, vecs2, nb2, res2 = core.block_lanczos_eigs(A2, 12, maxblocks=5, tol=1e-10, seed=4)
+    np.testing.assert_allclose(ev2, e2[:12], atol=1e-10)
+    W = vecs2.cpu().numpy()
+    np.testing.assert_allclose(W.conj() @ W.T, np.eye(12), atol=1e-11)
+
+
+def test_block_lanczos_matches_single_vector_lanczos(core):
+    m = H.small_siam(nbath=6, seed=7)          # 16 orbitals, N=8: dim 12,870
+    fi = H.fock_words(m.norb, m.nocc)
+    A = core.build_ham_i(core.Basis.from_words(fi, m.norb), m.hop_i, m.coul_i)
+    ev1, v1, _ = core.lanczos_eigs(A, 3, maxiter=800, tol=1e-11, seed=2, deflate=True)
+    ev, vecs, nb, res = core.block_lanczos_eigs(A, 3, maxblocks=150, tol=1e-10, seed=2)
+    assert (res < 1e-10).all(), (nb, res)
+    np.testing.assert_allclose(ev, ev1, atol=1e-10)
+    # eigenvectors: |<v_block|v_single>| = 1 for the nondegenerate ground state
+    ov = abs(complex((vecs[0].conj() * v1[0]).sum().item()))
+    assert abs(ov - 1.0) < 1e-9
+    # true residual |H x - e x| of every returned vector
+    for k in range(3):
+        y = A.matvec(vecs[k].contiguous())
+        assert float((y - ev[k] * vecs[k]).norm()) < 1e-9
+    # mixed-precision products: eigenvalues of the rounded matrix
+    evd, _, _, _ = core.block_lanczos_eigs(A, 3, maxblocks=150, tol=1e-9, seed=2,
+                                           precision="c64d", nvec=0)
+    assert np.abs(evd - ev1).max() < 1e-5
+    # one d electron: 10 states, below one block of 16 rows
+    from edrixs_amd import siam
+    ht = siam.emat_terms(siam.cf_cubic_d(1.5))
+    ut = siam.umat_terms(siam.umat_slater_d(4.0, 2.0, 0.5))
+    tiny = core.build_ham_i(core.Basis.from_words(H.fock_words(10, 1), 10), ht, ut)
+    with pytest.raises(Exception, match="at least 16 rows"):
+        core.block_lanczos_eigs(tiny, 1)
