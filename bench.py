@@ -121,10 +121,16 @@ class Workload:
         T = {}
         t0 = time.perf_counter()
         Hi = c.build_ham_i(self.bi, m.hop_i, m.coul_i)
-        Hn = c.build_ham_n(self.bn, m.norb, m.ncore, m.hop_n, m.coul_n)
-        D = [c.build_transop_i(self.bn, self.bi, m.norb, m.ncore, m.trans_i[k]) for k in range(3)]
-        Din = [c.build_transop_i(self.bn, self.bi, m.norb, m.ncore, ti) for ti, _ in self.pol_terms]
-        Dout = [c.build_transop_f(self.bi, self.bn, m.norb, m.ncore, tf) for _, tf in self.pol_terms]
+        # core-hole space in the interleaved layout (valence * 6 + core slot): the
+        # same operators permuted, spectra equal up to rounding
+        lay = "interleaved"
+        Hn = c.build_ham_n(self.bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout=lay)
+        D = [c.build_transop_i(self.bn, self.bi, m.norb, m.ncore, m.trans_i[k], layout=lay)
+             for k in range(3)]
+        Din = [c.build_transop_i(self.bn, self.bi, m.norb, m.ncore, ti, layout=lay)
+               for ti, _ in self.pol_terms]
+        Dout = [c.build_transop_f(self.bi, self.bn, m.norb, m.ncore, tf, layout=lay)
+                for _, tf in self.pol_terms]
         Hi.prepare(precision)
         Hn.prepare(precision)
         torch.cuda.synchronize()

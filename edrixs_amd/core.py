@@ -236,24 +236,47 @@ def build_ham_i(basis, hop, coul, omega=0j, ham_sign=1.0, precision="c128"):
                        _prec(precision))
 
 
+# index layout of the core-hole space (edx_set_corehole_layout): "reference" is
+# core_slot * dim + valence (src/ham.f90:419-446), "interleaved" is
+# valence * num_core_orbs + core_slot (the same matrix, permuted)
+COREHOLE_LAYOUTS = {"reference": 0, "interleaved": 1}
+
+
+def _build_in_layout(layout, fn, *args):
+    prev = ctypes.c_int()
+    _lib.call("edx_set_corehole_layout", COREHOLE_LAYOUTS[layout], ctypes.byref(prev))
+    try:
+        return _handle_out(fn, *args)
+    finally:
+        _lib.call("edx_set_corehole_layout", prev.value, None)
+
+
+def corehole_permutation(dim_val, num_core_orbs):
+    """p with x_interleaved = x_reference[p] for core-hole-space vectors."""
+    i = np.arange(dim_val * num_core_orbs, dtype=np.int64)
+    return (i % num_core_orbs) * dim_val + i // num_core_orbs
+
+
 def build_ham_n(basis_n, num_val_orbs, num_core_orbs, hop, coul, omega=0j, ham_sign=1.0,
-                precision="c128"):
+                precision="c128", layout="reference"):
     h, c = as_terms2(hop), as_terms4(coul)
-    return _handle_out("edx_build_ham_n", basis_n._h, num_val_orbs, num_core_orbs, _ptr(h),
-                       len(h), _ptr(c), len(c), float(np.real(omega)), float(np.imag(omega)),
-                       float(ham_sign), _prec(precision))
+    return _build_in_layout(layout, "edx_build_ham_n", basis_n._h, num_val_orbs, num_core_orbs,
+                            _ptr(h), len(h), _ptr(c), len(c), float(np.real(omega)),
+                            float(np.imag(omega)), float(ham_sign), _prec(precision))
 
 
-def build_transop_i(basis_n, basis_i, num_val_orbs, num_core_orbs, trans, precision="c128"):
+def build_transop_i(basis_n, basis_i, num_val_orbs, num_core_orbs, trans, precision="c128",
+                    layout="reference"):
     t = as_terms2(trans)
-    return _handle_out("edx_build_transop_i", basis_n._h, basis_i._h, num_val_orbs,
-                       num_core_orbs, _ptr(t), len(t), _prec(precision))
+    return _build_in_layout(layout, "edx_build_transop_i", basis_n._h, basis_i._h, num_val_orbs,
+                            num_core_orbs, _ptr(t), len(t), _prec(precision))
 
 
-def build_transop_f(basis_f, basis_n, num_val_orbs, num_core_orbs, trans, precision="c128"):
+def build_transop_f(basis_f, basis_n, num_val_orbs, num_core_orbs, trans, precision="c128",
+                    layout="reference"):
     t = as_terms2(trans)
-    return _handle_out("edx_build_transop_f", basis_f._h, basis_n._h, num_val_orbs,
-                       num_core_orbs, _ptr(t), len(t), _prec(precision))
+    return _build_in_layout(layout, "edx_build_transop_f", basis_f._h, basis_n._h, num_val_orbs,
+                            num_core_orbs, _ptr(t), len(t), _prec(precision))
 
 
 def lanczos_eigs(H, nev, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None,

@@ -12,6 +12,9 @@
 namespace edx {
 
 int set_error(int code, const char *fmt, ...);
+// layout of core-hole-space indices for the next builds on this thread
+// (edx_set_corehole_layout)
+int corehole_layout();
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 // Device allocation with accounting; returns nullptr (error set) on failure.
@@ -80,6 +83,9 @@ struct edx_csr {
     uint16_t *d_off_idx;
     double2 *d_vtab;     // VTAB_SLOTS entries (hash-addressed; unused slots zero)
     int64_t nvtab;       // distinct off-diagonal values
+    // index layout of the core-hole space on the rows and/or columns that are
+    // in it (edx_corehole_layout); 0 for matrices without one
+    int corehole_layout;
 };
 
 // Lookup descriptor for a column space basis.

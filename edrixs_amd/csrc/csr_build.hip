@@ -40,6 +40,7 @@ struct BuildParams {
     int32_t *cols;
     double2 *vals;
     int *overflow;
+    int inter;             // core-hole index valence * nc + slot (else slot * n + valence)
 };
 
 __device__ __forceinline__ bool op_ann(uint64_t &m, int p, int &par) {
@@ -61,8 +62,8 @@ __device__ __forceinline__ uint64_t row_word(const BuildParams &p, int64_t r) {
     if (RK == ROW_PLAIN) return (uint64_t)p.left[r];
     const uint64_t full = (1ull << p.nc) - 1ull;
     if (RK == ROW_COREHOLE) {
-        int64_t k = r / p.nleft;
-        int64_t v = r - k * p.nleft;
+        const int64_t k = p.inter ? r % p.nc : r / p.nleft;
+        const int64_t v = p.inter ? r / p.nc : r - k * p.nleft;
         uint64_t core = full ^ (1ull << (p.nc - 1 - k));   // src/ham.f90:419-421
         return (core << p.nv) | (uint64_t)p.left[v];
     }
@@ -83,7 +84,7 @@ __device__ __forceinline__ int64_t col_of(const BuildParams &p, uint64_t w) {
     int64_t k = p.nc - 1 - b;
     int64_t l = basis_index(p.right, val);
     if (l < 0) return -1;
-    return k * p.right.n + l;
+    return p.inter ? l * p.nc + k : k * p.right.n + l;
 }
 
 __device__ __forceinline__ int lanes_below(uint64_t mask) {
@@ -446,6 +447,7 @@ int build_csr(BuildParams p, int64_t ncols, int precision, edx_csr **out) {
         return rc;
     }
     edx_csr *a = new edx_csr{};
+    a->corehole_layout = p.inter;
     a->m = p.M;
     a->n = ncols;
     a->nnz = nnz;
@@ -522,6 +524,7 @@ int edx_build_ham_n(const edx_basis *fock_n, int num_val_orbs, int num_core_orbs
     p.right = view_of(fock_n);
     p.nv = num_val_orbs;
     p.nc = num_core_orbs;
+    p.inter = edx::corehole_layout() == EDX_COREHOLE_INTERLEAVED;
     p.omega_on = ham_sign < 0;
     p.omega = make_double2(omega_re, omega_im);
     p.negate = ham_sign < 0;
@@ -542,6 +545,7 @@ int edx_build_transop_i(const edx_basis *fock_n, const edx_basis *fock_i, int nu
     p.right = view_of(fock_i);
     p.nv = num_val_orbs;
     p.nc = num_core_orbs;
+    p.inter = edx::corehole_layout() == EDX_COREHOLE_INTERLEAVED;
     return build_csr<ROW_COREHOLE, COL_VALONLY>(p, fock_i->n, precision, out);
 }
 
@@ -559,6 +563,7 @@ int edx_build_transop_f(const edx_basis *fock_f, const edx_basis *fock_n, int nu
     p.right = view_of(fock_n);
     p.nv = num_val_orbs;
     p.nc = num_core_orbs;
+    p.inter = edx::corehole_layout() == EDX_COREHOLE_INTERLEAVED;
     return build_csr<ROW_FULLCORE, COL_COREHOLE>(p, fock_n->n * num_core_orbs, precision, out);
 }
 

@@ -13,6 +13,9 @@ namespace edx {
 
 static thread_local std::string g_last_error;
 static thread_local int g_device = 0;
+static thread_local int g_corehole_layout = EDX_COREHOLE_REFERENCE;
+
+int corehole_layout() { return g_corehole_layout; }
 
 int set_error(int code, const char *fmt, ...) {
     char buf[1024];
@@ -86,6 +89,14 @@ int num_cus() {
 extern "C" {
 
 const char *edx_last_error(void) { return edx::g_last_error.c_str(); }
+
+int edx_set_corehole_layout(int layout, int *previous) {
+    EDX_CHECK(layout == EDX_COREHOLE_REFERENCE || layout == EDX_COREHOLE_INTERLEAVED, EDX_ERR_ARG,
+              "unknown core-hole layout %d", layout);
+    if (previous) *previous = edx::g_corehole_layout;
+    edx::g_corehole_layout = layout;
+    return EDX_OK;
+}
 
 int edx_init(int device) {
     int n = 0;

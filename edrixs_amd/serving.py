@@ -35,6 +35,7 @@ from . import core, siam
 
 GAMMA_C = 0.5
 GAMMA_F = 0.1
+LAYOUT = "interleaved"   # core-hole index valence * 6 + slot (permuted operators, same spectra)
 
 
 @dataclasses.dataclass
@@ -147,13 +148,13 @@ class Engine:
         egs = float(ev[0])
         gs = vecs[0].contiguous()
         req.egs = egs
-        Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+        Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout=LAYOUT)
         group = self.group_for(req.norb)
         st = {"req": req, "mats": [Hi, Hn], "members": [], "grid": None, "gamma": None,
               "group": group}
         if req.kind == "xas":
             for k in range(3):
-                D = core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[k])
+                D = core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[k], layout=LAYOUT)
                 st["members"].append(group.add(Hn, D.matvec(gs)))
             st["grid"], st["gamma"] = self.omega, GAMMA_C
             st["combine"] = "average"
@@ -169,8 +170,8 @@ class Engine:
                 eout = np.eye(3)[(pol + 1) % 3]
                 tin = m.pol_trans(ein)[0]
                 tout = m.pol_trans(eout)[1]
-                Din = core.build_transop_i(bn, bi, m.norb, m.ncore, tin)
-                Dout = core.build_transop_f(bi, bn, m.norb, m.ncore, tout)
+                Din = core.build_transop_i(bn, bi, m.norb, m.ncore, tin, layout=LAYOUT)
+                Dout = core.build_transop_f(bi, bn, m.norb, m.ncore, tout, layout=LAYOUT)
                 (Y,), _, _ = core.shifted_fom(Hn, Din.matvec(gs), sigma, projections=(Dout,),
                                               tol=self.rixs_tol, maxit=self.rixs_maxit,
                                               precision=p)

@@ -81,6 +81,24 @@ void edx_basis_free(edx_basis *b);
 
 /* ------------------------------------------------------------ CSR build */
 
+/* Index layout of the core-hole space (core_slot, valence) of H_n and of the
+ * core-hole side of the transition operators:
+ *   EDX_COREHOLE_REFERENCE   core_slot * dim(fock_n) + valence, the reference's
+ *                            (src/ham.f90:419-446);
+ *   EDX_COREHOLE_INTERLEAVED valence * num_core_orbs + core_slot: the matrix is
+ *                            P H P^T with the same value at every permuted
+ *                            entry, and the terms that move the core hole (core
+ *                            spin-orbit, core-valence exchange) land next to the
+ *                            diagonal instead of dim(fock_n) rows away.
+ * Only the internal order of core-hole-space vectors changes: spectra and pole
+ * files from a consistent set of matrices differ by rounding alone.
+ * edx_set_corehole_layout sets the layout of the following edx_build_ham_n,
+ * edx_build_transop_i and edx_build_transop_f calls on the calling thread
+ * (default EDX_COREHOLE_REFERENCE) and returns the previous one.  No reference
+ * counterpart. */
+enum edx_corehole_layout { EDX_COREHOLE_REFERENCE = 0, EDX_COREHOLE_INTERLEAVED = 1 };
+int edx_set_corehole_layout(int layout, int *previous);
+
 /* H = sum hop + sum coul on one space; rows/columns = basis
  * (src/ham.f90:2-349 build_ham_i).  ham_sign must be +1 or -1; with -1 the
  * result is omega*I - H, omega added last on the diagonal (src/ham.f90:156-176). */

@@ -943,3 +943,53 @@ def test_block_lanczos_matches_single_vector_lanczos(core):
     tiny = core.build_ham_i(core.Basis.from_words(H.fock_words(10, 1), 10), ht, ut)
     with pytest.raises(Exception, match="at least 16 rows"):
         core.block_lanczos_eigs(tiny, 1)
+
+
+def test_interleaved_corehole_layout_is_the_permuted_matrix(core):
+    m = H.small_siam(nbath=2, seed=11)
+    fi, fn = H.fock_words(m.norb, m.nocc), H.fock_words(m.norb, m.nocc + 1)
+    bi, bn = core.Basis.from_words(fi, m.norb), core.Basis.from_words(fn, m.norb)
+    p = core.corehole_permutation(len(fn), m.ncore)      # x_inter = x_ref[p]
+    pinv = np.empty_like(p)
+    pinv[p] = np.arange(len(p))
+
+    def triplets(A, rmap=None, cmap=None):
+        rp, c, v = A.download()
+        r = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        c = c.astype(np.int64)
+        if rmap is not None:
+            r = rmap[r]
+        if cmap is not None:
+            c = cmap[c]
+        order = np.lexsort((c, r))
+        return r[order], c[order], v[order]
+
+    pairs = [
+        (core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n),
+         core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout="interleaved"), True, True),
+        (core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[0]),
+         core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[0], layout="interleaved"), True, False),
+        (core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1]),
+         core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1], layout="interleaved"), False, True),
+    ]
+    for ref, inter, rows_ch, cols_ch in pairs:
+        a = triplets(ref, pinv if rows_ch else None, pinv if cols_ch else None)
+        b = triplets(inter)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)          # same entries, bit for bit
+    # the layout is per call: a plain build afterwards is the reference layout again
+    again = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    for x, y in zip(triplets(again), triplets(pairs[0][0])):
+        np.testing.assert_array_equal(x, y)
+    # XAS Krylov on the permuted pair: same (alpha, beta) up to rounding
+    Hn_r, Hn_i = pairs[0][0], pairs[0][1]
+    D_r, D_i = pairs[1][0], pairs[1][1]
+    gs = _to_dev(core, np.random.default_rng(4).normal(size=len(fi)) + 0j)
+    a1, b1, n1 = core.build_krylov(Hn_r, D_r.matvec(gs), 60)
+    a2, b2, n2 = core.build_krylov(Hn_i, D_i.matvec(gs), 60)
+    # (alpha ~ 100 here: rounding of the reordered sums, relative 1e-12)
+    np.testing.assert_allclose(a2, a1, rtol=1e-12)
+    np.testing.assert_allclose(b2, b1, rtol=1e-12, atol=1e-12 * np.abs(a1).max())
+    assert abs(n1 - n2) < 1e-12 * n1
+    with pytest.raises(Exception, match="unknown core-hole layout"):
+        core._lib.call("edx_set_corehole_layout", 7, None)
