@@ -651,7 +651,7 @@ def probe(args):
     bi = c.Basis.combinations(m.norb, m.nocc)
     bn = c.Basis.combinations(m.norb, m.nocc + 1)
     Hi = c.build_ham_i(bi, m.hop_i, m.coul_i)
-    Hn = c.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    Hn = c.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout="interleaved")   # as the step
     f = args.probe_form
     Hi.prepare(f)
     Hn.prepare(f)

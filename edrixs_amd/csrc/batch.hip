@@ -120,21 +120,31 @@ __global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t
 // RHS-parallel schedule: 16 groups of 16 lanes per block, one row per group,
 // lane l owns right-hand sides k0 + l + 16 q (q < NPL); a pass (blockIdx.y)
 // covers a tile of 16 NPL right-hand sides, k0 = blockIdx.y * 16 NPL.
+// With xmap (gridDim.x a multiple of 8), the workgroups the hardware places on
+// one XCD (blockIdx.x % 8) sweep one contiguous eighth of the rows, so rows
+// that gather the same X lines share that XCD's L2; otherwise consecutive
+// 16-row groups go to consecutive workgroups, i.e. to all eight XCDs.
 template <int NPL, class VP, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmm_rl(int64_t m, int K, const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ cols, VP vals,
                                                  const double2 *__restrict__ diag,
                                                  const XT *__restrict__ X,
-                                                 const double2 *__restrict__ X64, BEpi e) {
+                                                 const double2 *__restrict__ X64, BEpi e, int xmap) {
     __shared__ double2 red[16 * 16 * NPL];
     const int l = threadIdx.x & 15;
     const int grp = threadIdx.x >> 4;
     const int k0 = blockIdx.y * 16 * NPL;   // this pass's tile of 16 NPL right-hand sides
-    const int64_t ngroups = (int64_t)gridDim.x * 16;
+    int64_t rfirst = (int64_t)blockIdx.x * 16 + grp, rend = m, ngroups = (int64_t)gridDim.x * 16;
+    if (xmap) {
+        const int64_t per = (m + 7) / 8, xcd = blockIdx.x % 8;
+        rfirst = xcd * per + (int64_t)(blockIdx.x / 8) * 16 + grp;
+        rend = min(m, (xcd + 1) * per);
+        ngroups = (int64_t)(gridDim.x / 8) * 16;
+    }
     double2 dot[NPL];
 #pragma unroll
     for (int q = 0; q < NPL; ++q) dot[q] = make_double2(0, 0);
-    for (int64_t r = (int64_t)blockIdx.x * 16 + grp; r < m; r += ngroups) {
+    for (int64_t r = rfirst; r < rend; r += ngroups) {
         const int64_t b = rowptr[r], end = rowptr[r + 1];
         ACC sr[NPL], si[NPL];
 #pragma unroll
@@ -407,6 +417,12 @@ __global__ void k_bdiv_beta(int64_t n, int K, double2 *v, const double *beta_i, 
 
 // ------------------------------------------------------- batched MINRES
 
+// HIP-graph replay of fixed launch sequences (EDX_NO_GRAPH=1 turns it off)
+bool graphs_enabled() {
+    const char *off = getenv("EDX_NO_GRAPH");
+    return !(off && off[0] == '1');
+}
+
 struct BMinres {
     double *beta0;   // [K]
     double2 *sigma0, *sigma1, *gamma0, *gamma1, *eta, *res;
@@ -415,7 +431,13 @@ struct BMinres {
     int *iters;          // [K]
     double tol;
     int maxit;
+    const int *istep;    // nullptr: the iteration is the launch argument i; else *istep
 };
+
+// the iteration a MINRES kernel works on
+__device__ __forceinline__ int bm_step(const BMinres &s, int i) { return s.istep ? *s.istep : i; }
+
+__global__ void k_bm_next(int *istep) { *istep += 1; }
 
 __global__ void k_bm_init(BMinres s, int K, const double2 *sums) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -430,10 +452,10 @@ __global__ void k_bm_init(BMinres s, int K, const double2 *sums) {
     s.run[k] = (b0 > s.tol && 0 < s.maxit) ? 1 : 0;
 }
 
-__global__ void k_bm_divbeta(int64_t n, int K, double2 *v, const double *beta0, const int *run_i) {
+__global__ void k_bm_divbeta(int64_t n, int K, double2 *v, BMinres s, int i) {
     ELEM_SETUP
-    if (!kv || !run_i[k]) return;
-    const double d = beta0[k];
+    if (!kv || !s.run[(int64_t)bm_step(s, i) * K + k]) return;
+    const double d = s.beta0[k];
     for (int64_t r = rstart; r < n; r += rstride) {
         const double2 t = v[r * K + k];
         v[r * K + k] = make_double2(t.x / d, t.y / d);
@@ -442,6 +464,7 @@ __global__ void k_bm_divbeta(int64_t n, int K, double2 *v, const double *beta0, 
 
 __global__ void k_bm_alpha(BMinres s, int K, int i, const double2 *sums) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    i = bm_step(s, i);
     if (k >= K || !s.run[i * K + k]) return;
     s.alpha[k] = sums[k];
 }
@@ -451,6 +474,7 @@ __global__ void k_bm_vupdate(int64_t n, int K, const double2 *tmp, const double2
                              BMinres s, int i, double2 *partials) {
     ELEM_SETUP
     double2 acc = make_double2(0, 0);
+    i = bm_step(s, i);
     if (kv && s.run[(int64_t)i * K + k]) {
         const double2 al = s.alpha[k];
         const double b0 = s.beta0[k];
@@ -466,6 +490,7 @@ __global__ void k_bm_vupdate(int64_t n, int K, const double2 *tmp, const double2
 
 __global__ void k_bm_scalars(BMinres s, int K, int i, const double2 *sums) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    i = bm_step(s, i);
     if (k >= K) return;
     if (!s.run[i * K + k]) {
         s.run[(i + 1) * K + k] = 0;
@@ -501,7 +526,7 @@ __global__ void k_bm_scalars(BMinres s, int K, int i, const double2 *sums) {
 __global__ void k_bm_wx(int64_t n, int K, const double2 *vc, double2 *wp, const double2 *wc,
                         double2 *x, BMinres s, int i) {
     ELEM_SETUP
-    if (!kv || !s.run[(int64_t)i * K + k]) return;
+    if (!kv || !s.run[(int64_t)bm_step(s, i) * K + k]) return;
     const double2 r3 = s.rho3[k], r2 = s.rho2[k], ir1 = s.inv_rho1[k], xc = s.xcoef[k];
     for (int64_t r = rstart; r < n; r += rstride) {
         const int64_t o = r * K + k;
@@ -654,24 +679,25 @@ int bspmm_launch(const edx_csr *a, int precision, int K, const double2 *X64, con
     if (sched.rl) {
         const int npl = sched.p;
         dim3 grid((unsigned)nb, (unsigned)((K + 16 * npl - 1) / (16 * npl)));
+        const int xmap = nb % 8 == 0 && xcd_map_enabled();
 #define EDX_RL(NP)                                                                                 \
     case NP:                                                                                       \
         if (precision == EDX_C128)                                                                 \
             hipLaunchKernelGGL((k_spmm_rl<NP, const double2 *, double2, double, false>), grid,      \
                                dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,            \
-                               (const double2 *)a->d_vals, nullptr, X64, X64, e);                 \
+                               (const double2 *)a->d_vals, nullptr, X64, X64, e, xmap);           \
         else if (precision == EDX_C64)                                                             \
             hipLaunchKernelGGL((k_spmm_rl<NP, const float2 *, float2, float, false>), grid,         \
                                dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,            \
-                               (const float2 *)a->d_vals32, nullptr, X32, X64, e);                \
+                               (const float2 *)a->d_vals32, nullptr, X32, X64, e, xmap);          \
         else if (precision == EDX_C128I)                                                           \
             hipLaunchKernelGGL((k_spmm_rl<NP, IdxPtr, double2, double, true>), grid, dim3(256), 0,  \
                                stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,                  \
-                               IdxPtr{a->d_off_idx, a->d_vtab}, a->d_diag, X64, X64, e);           \
+                               IdxPtr{a->d_off_idx, a->d_vtab}, a->d_diag, X64, X64, e, xmap);     \
         else                                                                                       \
             hipLaunchKernelGGL((k_spmm_rl<NP, const float2 *, double2, double, true>), grid,        \
                                dim3(256), 0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,    \
-                               (const float2 *)a->d_off_vals32, a->d_diag, X64, X64, e);          \
+                               (const float2 *)a->d_off_vals32, a->d_diag, X64, X64, e, xmap);    \
         break;
         switch (npl) {
             EDX_RL(1)
@@ -1131,37 +1157,98 @@ int bminres_run(const edx_csr *h, int precision, int K, const double2 *sigma, co
     hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, sums);
     hipLaunchKernelGGL(k_bm_init, dim3(kb), dim3(64), 0, s, st, K, sums);
     EDX_LAUNCH_CHECK();
-    int curr = 0, prev = 1, checked = 0;
-    for (int i = 0; i < maxit; ++i) {
-        const int *run_i = st.run + (size_t)i * K;
-        ELAUNCH(k_bm_divbeta, n, K, n, K, v[curr], st.beta0, run_i);
-        if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[curr], x32.as<float2>());
+    st.istep = nullptr;
+    // one iteration on edx::stream(); its step is i, or *sti.istep (then
+    // advanced) when the launches are captured
+    auto iterate = [&](int i, int cur, int prv, const BMinres &sti) -> int {
+        hipStream_t ls = edx::stream();
+        ELAUNCH(k_bm_divbeta, n, K, n, K, v[cur], sti, i);
+        if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[cur], x32.as<float2>());
         BEpi e{};   // tmp = sigma_k v - H v ; partial <v|tmp>
         e.y = tmp;
         e.c0 = neg1;
         e.c2 = sig;
-        e.dvec = v[curr];
+        e.dvec = v[cur];
         e.partials = part.as<double2>();
         int nb = 0;
-        EDX_TRY(edx::bspmm(h, precision, K, v[curr], x32.as<float2>(), e, &nb));
-        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nb, K, sums);
-        hipLaunchKernelGGL(k_bm_alpha, dim3(kb), dim3(64), 0, s, st, K, i, sums);
-        ELAUNCH(k_bm_vupdate, n, K, n, K, tmp, v[curr], v[prev], st, i, part.as<double2>());
-        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, part.as<double2>(), nbe, K, sums);
-        hipLaunchKernelGGL(k_bm_scalars, dim3(kb), dim3(64), 0, s, st, K, i, sums);
-        ELAUNCH(k_bm_wx, n, K, n, K, v[curr], w[prev], w[curr], x, st, i);
+        EDX_TRY(edx::bspmm(h, precision, K, v[cur], x32.as<float2>(), e, &nb));
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, ls, part.as<double2>(), nb, K, sums);
+        hipLaunchKernelGGL(k_bm_alpha, dim3(kb), dim3(64), 0, ls, sti, K, i, sums);
+        ELAUNCH(k_bm_vupdate, n, K, n, K, tmp, v[cur], v[prv], sti, i, part.as<double2>());
+        hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, ls, part.as<double2>(), nbe, K, sums);
+        hipLaunchKernelGGL(k_bm_scalars, dim3(kb), dim3(64), 0, ls, sti, K, i, sums);
+        ELAUNCH(k_bm_wx, n, K, n, K, v[cur], w[prv], w[cur], x, sti, i);
+        if (sti.istep) hipLaunchKernelGGL(k_bm_next, dim3(1), dim3(1), 0, ls, (int *)sti.istep);
         EDX_LAUNCH_CHECK();
-        std::swap(curr, prev);
-        if (i + 1 - checked >= 16) {
+        return EDX_OK;
+    };
+    // iteration 0 runs eagerly (it also times the SpMM schedules on first use);
+    // then an odd/even pair of iterations is captured once into a HIP graph and
+    // replayed, the kernels reading the step from device memory
+    EDX_TRY(iterate(0, 0, 1, st));
+    int i = 1, checked = 0;
+    Dbuf stepb;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+    if (graphs_enabled() && !edx::g_prof && maxit >= 3) {
+        EDX_TRY(stepb.alloc(sizeof(int), "minres step"));
+        EDX_HIP(hipMemcpy(stepb.p, &i, sizeof(int), hipMemcpyHostToDevice));
+        BMinres sg = st;
+        sg.istep = stepb.as<int>();
+        EDX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        const hipError_t eb = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        int rc = eb == hipSuccess ? EDX_OK
+                                  : edx::hip_fail(eb, "hipStreamBeginCapture", __FILE__, __LINE__);
+        if (!rc) {
+            {
+                edx::StreamScope scope(cap);
+                rc = iterate(0, 1, 0, sg);
+                if (!rc) rc = iterate(0, 0, 1, sg);
+            }
+            hipGraph_t graph = nullptr;
+            const hipError_t ee = hipStreamEndCapture(cap, &graph);
+            if (!rc && ee != hipSuccess) rc = edx::hip_fail(ee, "hipStreamEndCapture", __FILE__, __LINE__);
+            if (!rc) {
+                const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (ei != hipSuccess) {
+                    exec = nullptr;
+                    rc = edx::hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+                }
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+        if (rc) {
+            (void)hipStreamDestroy(cap);
+            return rc;
+        }
+    }
+    int rc = EDX_OK;
+    while (i < maxit && rc == EDX_OK) {
+        if (exec && i + 2 <= maxit) {
+            const hipError_t el = hipGraphLaunch(exec, s);
+            if (el != hipSuccess) rc = edx::hip_fail(el, "hipGraphLaunch", __FILE__, __LINE__);
+            i += 2;
+        } else {
+            rc = iterate(i, i & 1, (i & 1) ^ 1, st);
+            i += 1;
+        }
+        if (!rc && i - checked >= 16) {
             std::vector<int> r(K);
-            EDX_HIP(hipMemcpy(r.data(), st.run + (size_t)(i + 1) * K, K * sizeof(int),
-                              hipMemcpyDeviceToHost));
-            checked = i + 1;
+            const hipError_t em = hipMemcpy(r.data(), st.run + (size_t)i * K, K * sizeof(int),
+                                            hipMemcpyDeviceToHost);
+            if (em != hipSuccess) {
+                rc = edx::hip_fail(em, "hipMemcpy", __FILE__, __LINE__);
+                break;
+            }
+            checked = i;
             bool any = false;
             for (int k = 0; k < K; ++k) any |= r[k] != 0;
             if (!any) break;
         }
     }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (cap) (void)hipStreamDestroy(cap);
+    EDX_TRY(rc);
     // true residuals |(sigma_k - H) x_k - b_k|
     ELAUNCH(k_cm_to_il, n, K, n, K, Bcm, b);
     if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, x, x32.as<float2>());

@@ -22,9 +22,17 @@ void *dmalloc(size_t bytes, const char *what);
 void dfree(void *p);
 int64_t mem_in_use();
 
-// The stream all library work is queued on (the legacy default stream, so it
-// is ordered with torch's default stream and with synchronous hipMemcpy).
-inline hipStream_t stream() { return 0; }
+// The stream all library work is queued on: the legacy default stream, so it
+// is ordered with torch's default stream and with synchronous hipMemcpy --
+// except while a solver captures its iterations into a HIP graph, when the
+// launches of this thread go to the capture stream (StreamScope).
+extern thread_local hipStream_t g_launch_stream;
+inline hipStream_t stream() { return g_launch_stream; }
+struct StreamScope {
+    hipStream_t saved;
+    explicit StreamScope(hipStream_t s) : saved(g_launch_stream) { g_launch_stream = s; }
+    ~StreamScope() { g_launch_stream = saved; }
+};
 int current_device();
 int num_cus();
 

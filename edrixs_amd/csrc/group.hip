@@ -286,11 +286,6 @@ int group_step(edx_group *g, hipStream_t s) {
     return EDX_OK;
 }
 
-bool graphs_enabled() {
-    const char *off = getenv("EDX_NO_GRAPH");
-    return !(off && off[0] == '1');
-}
-
 // capture GRAPH_STEPS steps for the current membership
 int group_capture(edx_group *g) {
     if (g->exec) return EDX_OK;

@@ -14,6 +14,7 @@ namespace edx {
 static thread_local std::string g_last_error;
 static thread_local int g_device = 0;
 static thread_local int g_corehole_layout = EDX_COREHOLE_REFERENCE;
+thread_local hipStream_t g_launch_stream = 0;
 
 int corehole_layout() { return g_corehole_layout; }
 

@@ -60,6 +60,7 @@ struct SpmvEpi {
     const double2 *dvec;    // nullptr -> no dot
     double2 *partials;      // one per block when dvec != nullptr
     const int *run;         // nullptr or flag: skip the launch when *run == 0
+    int xmap;               // gridDim.x % 8 == 0: XCD-contiguous block order (set by the launcher)
 };
 
 // block-level sum of a complex value (256 threads), result valid in thread 0
@@ -95,7 +96,12 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
     if (e.run && *e.run == 0) return;
     using acc_t = ACC;
     const int lig = threadIdx.x & (G - 1);
-    const int64_t group = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+    // xmap: the workgroups the hardware places on one XCD (blockIdx.x % 8) take
+    // one contiguous eighth of the row blocks, so neighbouring rows, which
+    // gather nearby x, share that XCD's L2
+    const int64_t bid = e.xmap ? (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
+                               : (int64_t)blockIdx.x;
+    const int64_t group = (bid * 256 + threadIdx.x) / G;
     const int64_t ngroups = (int64_t)gridDim.x * (256 / G);
     const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
     const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
@@ -303,6 +309,7 @@ double spmv_bytes(const edx_csr *a, int precision, int nrhs, bool z, bool dvec) 
 int spmv_blocks(const edx_csr *a, int G) {
     int64_t blocks = (a->m * G + 255) / 256;
     if (blocks < 1) blocks = 1;
+    if (blocks >= 64) blocks = (blocks + 7) / 8 * 8;   // XCD-contiguous order needs a multiple of 8
     return (int)blocks;
 }
 
@@ -313,7 +320,17 @@ int64_t spmv_partials_len(const edx_csr *a) { return (int64_t)spmv_blocks(a, 64)
 // Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
 // C64D and complex64 (already converted) for C64.  Returns the block count used.
 
-int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e, int lanes,
+// XCD-contiguous block order of the SpMV/SpMM launches (EDX_NO_XCDMAP=1 turns
+// it off)
+bool xcd_map_enabled() {
+    static const bool on = [] {
+        const char *off = getenv("EDX_NO_XCDMAP");
+        return !(off && off[0] == '1');
+    }();
+    return on;
+}
+
+int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e_in, int lanes,
                 int *nblocks_out) {
     if (precision == EDX_C64D && !a->d_diag)
         return set_error(EDX_ERR_ARG, "split (c64d) form not prepared");
@@ -329,6 +346,8 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     int nb = spmv_blocks(a, G);
     if (nblocks_out) *nblocks_out = nb;
     if (a->m == 0) return EDX_OK;
+    SpmvEpi e = e_in;
+    e.xmap = nb % 8 == 0 && xcd_map_enabled();
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
                                spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
 #define EDX_SPMV_CASE(GG)                                                                     \

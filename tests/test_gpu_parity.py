@@ -993,3 +993,37 @@ def test_interleaved_corehole_layout_is_the_permuted_matrix(core):
     assert abs(n1 - n2) < 1e-12 * n1
     with pytest.raises(Exception, match="unknown core-hole layout"):
         core._lib.call("edx_set_corehole_layout", 7, None)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64d"])
+def test_batched_minres_graph_replay_equals_eager(core, monkeypatch, precision):
+    # the iterations after the first are one captured odd/even pair replayed;
+    # results must equal the eager launches bit for bit, call after call, with
+    # eager work between the calls, for K on both SpMM schedules
+    m = H.small_siam(nbath=2, seed=4)
+    fn = H.fock_words(m.norb, m.nocc + 1)
+    bn = core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    rng = np.random.default_rng(10)
+    for K in (3, 12):
+        sig = -20.0 + 3.0 * np.arange(K) + 0.4j
+        B = _to_dev(core, rng.normal(size=(K, Hn.n)) + 1j * rng.normal(size=(K, Hn.n)))
+        monkeypatch.setenv("EDX_NO_GRAPH", "1")
+        Xe, ite, rese = core.minres_shifted_batch(Hn, sig, B, tol=1e-10, maxit=777,
+                                                  precision=precision)
+        monkeypatch.delenv("EDX_NO_GRAPH")
+        for rep in range(3):
+            Xg, itg, resg = core.minres_shifted_batch(Hn, sig, B, tol=1e-10, maxit=777,
+                                                      precision=precision)
+            assert np.array_equal(itg, ite)
+            assert np.array_equal(resg, rese)
+            assert bool((Xg == Xe).all())
+            _ = Hn.matvec(B[0].contiguous())     # eager work between the replayed calls
+        # an odd iteration cap ends with one eager iteration after the replays
+        Xo, ito, _ = core.minres_shifted_batch(Hn, sig, B, tol=1e-14, maxit=8,
+                                               precision=precision)
+        monkeypatch.setenv("EDX_NO_GRAPH", "1")
+        Xo2, ito2, _ = core.minres_shifted_batch(Hn, sig, B, tol=1e-14, maxit=8,
+                                                 precision=precision)
+        monkeypatch.delenv("EDX_NO_GRAPH")
+        assert np.array_equal(ito, ito2) and bool((Xo == Xo2).all())
