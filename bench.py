@@ -738,6 +738,8 @@ def main():
              "lanczos_iters": r["info"]["lanczos_iters"],
              "rixs_iters_max": max(r["info"]["rixs_iters"]),
              "rixs_krylov_steps_max": max(r["info"]["rixs_krylov_steps"]),
+             "rixs_krylov_steps": r["info"]["rixs_krylov_steps"],
+             "rixs_iters": r["info"]["rixs_iters"],
              "rixs_cf_change_max": r["info"]["rixs_cf_change_max"],
              "xas_krylov_steps_max": max(r["info"]["xas_krylov_steps"]),
              "xas_cf_change_max": r["info"]["xas_cf_change_max"],

@@ -148,6 +148,146 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
     }
 }
 
+// What a lane of the pipelined SpMV keeps per prefetched nonzero: the value
+// itself, or for the indexed form its 16-bit number (looked up when used).
+template <class VP>
+struct PipeVal {
+    using type = typename std::remove_cv<typename std::remove_pointer<VP>::type>::type;
+    __device__ static type load(VP v, int64_t j) { return v[j]; }
+    __device__ static type zero() { return type{}; }
+    __device__ static type value(VP, type h) { return h; }
+};
+template <>
+struct PipeVal<IdxPtr> {
+    using type = uint16_t;
+    __device__ static type load(IdxPtr v, int64_t j) { return v.idx[j]; }
+    __device__ static type zero() { return 0; }
+    __device__ static double2 value(IdxPtr v, type h) { return v.tab[h]; }
+};
+
+// Pipelined single-vector SpMV, the same product and epilogue as k_spmv: a
+// capped grid (gridDim.x a multiple of 8) walks the rows grid-stride, the
+// workgroups of one XCD (blockIdx.x % 8) over one contiguous eighth of them,
+// G lanes per row; each lane holds the column and value numbers of up to U
+// nonzeros of its current row and loads those of its next row before it
+// gathers x for the current one, so the index loads of one row overlap the
+// gathers of the previous.  Lane l of a row holds its nonzeros l, l + G, ...,
+// l + (U-1) G (index loads through the cache); nonzeros beyond U * G in a row
+// are read in place.
+template <int G, int U, class VP, class XT, class ACC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ cols, VP vals,
+                                                   const XT *__restrict__ x,
+                                                   const double2 *__restrict__ diag, SpmvEpi e) {
+    if (e.run && *e.run == 0) return;
+    using PV = PipeVal<VP>;
+    using H = typename PV::type;
+    using acc_t = ACC;
+    const int lig = threadIdx.x & (G - 1);
+    constexpr int64_t gpb = 256 / G;   // row groups per workgroup
+    const int64_t per_x = gridDim.x / 8, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
+    const int64_t rows_x = (m + 7) / 8;
+    const int64_t rbeg = xcd * rows_x, rend = min(m, rbeg + rows_x);
+    const int64_t stride = per_x * gpb;
+    const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
+    const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
+    double2 dot = make_double2(0.0, 0.0);
+    int64_t r = rbeg + w * gpb + threadIdx.x / G;
+    int c[U];
+    H h[U];
+    int64_t b = 0, end = 0;
+    // the epilogue's operands of the row, loaded with its indices by lane 0:
+    // diag[r] * x[r] (split forms), the addend z[r], the dot vector d[r]
+    double2 pd = make_double2(0, 0), pz = pd, pv = pd;
+    XT px{};
+    auto epi_load = [&](int64_t row, double2 &d_, XT &x_, double2 &z_, double2 &v_) {
+        if (lig != 0 || row >= rend) return;
+        if constexpr (SPLIT) {
+            d_ = diag[row];
+            x_ = x[row];
+        }
+        if (e.z) z_ = e.z[row];
+        if (e.dvec) v_ = e.dvec[row];
+    };
+    if (r < rend) {
+        b = rowptr[r];
+        end = rowptr[r + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t j = b + lig + u * G;
+        c[u] = j < end ? cols[j] : -1;
+        h[u] = j < end ? PV::load(vals, j) : PV::zero();
+    }
+    epi_load(r, pd, px, pz, pv);
+    while (__any(r < rend)) {
+        const int64_t rn = r + stride;
+        int cn[U];
+        H hn[U];
+        int64_t bn = 0, endn = 0;
+        if (rn < rend) {
+            bn = rowptr[rn];
+            endn = rowptr[rn + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = bn + lig + u * G;
+            cn[u] = j < endn ? cols[j] : -1;
+            hn[u] = j < endn ? PV::load(vals, j) : PV::zero();
+        }
+        double2 nd = make_double2(0, 0), nz = nd, nv = nd;
+        XT nx{};
+        epi_load(rn, nd, nx, nz, nv);
+        acc_t sr = 0, si = 0;
+        if (r < rend) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (c[u] >= 0) {
+                    const auto a = PV::value(vals, h[u]);
+                    const XT xv = x[c[u]];
+                    sr += (acc_t)a.x * (acc_t)xv.x - (acc_t)a.y * (acc_t)xv.y;
+                    si += (acc_t)a.x * (acc_t)xv.y + (acc_t)a.y * (acc_t)xv.x;
+                }
+            }
+            for (int64_t j = b + lig + U * G; j < end; j += G) {
+                const auto a = nt_load2(vals + j);
+                const XT xv = x[nt_load(cols + j)];
+                sr += (acc_t)a.x * (acc_t)xv.x - (acc_t)a.y * (acc_t)xv.y;
+                si += (acc_t)a.x * (acc_t)xv.y + (acc_t)a.y * (acc_t)xv.x;
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, G);
+            si += __shfl_xor(si, off, G);
+        }
+        if (r < rend && lig == 0) {
+            double2 ax = make_double2((double)sr, (double)si);
+            if constexpr (SPLIT) ax = cadd(ax, cmul(pd, px));
+            double2 out = cmul(c0, ax);
+            if (e.z) out = cadd(out, cmul(c1, pz));
+            e.y[r] = out;
+            if (e.dvec) dot = cadd(dot, cconjmul(pv, out));
+        }
+        r = rn;
+        b = bn;
+        end = endn;
+        pd = nd;
+        px = nx;
+        pz = nz;
+        pv = nv;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = cn[u];
+            h[u] = hn[u];
+        }
+    }
+    if (e.dvec) {
+        double2 s = block_sum_256(dot);
+        if (threadIdx.x == 0) e.partials[blockIdx.x] = s;
+    }
+}
+
 // Multi-RHS: Y[:, k] = c0 * A X[:, k] + c1 * Z[:, k], column-major blocks of
 // nrhs columns.  Each lane group handles one row and NR right-hand sides at a
 // time, so every value / column index read serves NR products.
@@ -313,9 +453,17 @@ int spmv_blocks(const edx_csr *a, int G) {
     return (int)blocks;
 }
 
+// Pipelined schedules (k_spmv_pipe), selected through the lanes setting:
+// code 100 + 10 G + B is G lanes per row on a grid of B workgroups per CU.
+constexpr int PIPE_MAX_BPC = 4;
+bool is_pipe_code(int code) { return code == 143 || code == 144 || code == 184; }
+int pipe_blocks(int bpc) { return (num_cus() * bpc + 7) / 8 * 8; }
+
 // Length of a partials buffer that serves every single-vector SpMV launch of
 // the matrix: one entry per block of the widest launch plus the folded sums.
-int64_t spmv_partials_len(const edx_csr *a) { return (int64_t)spmv_blocks(a, 64) + FOLD_N + 64; }
+int64_t spmv_partials_len(const edx_csr *a) {
+    return (int64_t)std::max(spmv_blocks(a, 64), pipe_blocks(PIPE_MAX_BPC)) + FOLD_N + 64;
+}
 
 // Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
 // C64D and complex64 (already converted) for C64.  Returns the block count used.
@@ -343,13 +491,54 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
         else if (precision == EDX_C128I) G = a->spmv_lanes128i ? a->spmv_lanes128i : lanes_for_split(a);
         else G = a->spmv_lanes ? a->spmv_lanes : lanes_for(a);
     }
-    int nb = spmv_blocks(a, G);
+    const bool pipe = is_pipe_code(G);
+    int nb = pipe ? pipe_blocks(G % 10) : spmv_blocks(a, G);
     if (nblocks_out) *nblocks_out = nb;
     if (a->m == 0) return EDX_OK;
     SpmvEpi e = e_in;
     e.xmap = nb % 8 == 0 && xcd_map_enabled();
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
                                spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
+    if (pipe) {
+#define EDX_PIPE_CASE(CODE, GG, UU)                                                               \
+    case CODE:                                                                                    \
+        if (precision == EDX_C128)                                                                \
+            hipLaunchKernelGGL((k_spmv_pipe<GG, UU, const double2 *, double2, double, false>),     \
+                               dim3(nb), dim3(256), 0, stream(), a->m, a->d_rowptr, a->d_cols,     \
+                               (const double2 *)a->d_vals, (const double2 *)x, nullptr, e);        \
+        else if (precision == EDX_C64D)                                                           \
+            hipLaunchKernelGGL((k_spmv_pipe<GG, UU, const float2 *, double2, double, true>),       \
+                               dim3(nb), dim3(256), 0, stream(), a->m, a->d_off_rowptr,            \
+                               a->d_off_cols, (const float2 *)a->d_off_vals32, (const double2 *)x, \
+                               a->d_diag, e);                                                      \
+        else if (precision == EDX_C128I)                                                          \
+            hipLaunchKernelGGL((k_spmv_pipe<GG, UU, IdxPtr, double2, double, true>), dim3(nb),     \
+                               dim3(256), 0, stream(), a->m, a->d_off_rowptr, a->d_off_cols,       \
+                               IdxPtr{a->d_off_idx, a->d_vtab}, (const double2 *)x, a->d_diag, e); \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_spmv_pipe<GG, UU, const float2 *, float2, float, false>),        \
+                               dim3(nb), dim3(256), 0, stream(), a->m, a->d_rowptr, a->d_cols,     \
+                               (const float2 *)a->d_vals32, (const float2 *)x, nullptr, e);        \
+        break;
+        switch (G) {
+            EDX_PIPE_CASE(143, 4, 8)
+            EDX_PIPE_CASE(144, 4, 8)
+            EDX_PIPE_CASE(184, 8, 4)
+            default: return set_error(EDX_ERR_ARG, "unsupported SpMV schedule %d", G);
+        }
+#undef EDX_PIPE_CASE
+        EDX_LAUNCH_CHECK();
+        prof_stop(pe);
+        if (e.dvec && nb > FOLD_N) {
+            hipLaunchKernelGGL(k_fold_partials, dim3(FOLD_N), dim3(256), 0, stream(), e.partials,
+                               nb, e.run);
+            hipLaunchKernelGGL(k_move_folded, dim3(1), dim3(FOLD_N), 0, stream(), e.partials, nb,
+                               e.run);
+            EDX_LAUNCH_CHECK();
+            if (nblocks_out) *nblocks_out = FOLD_N;
+        }
+        return EDX_OK;
+    }
 #define EDX_SPMV_CASE(GG)                                                                     \
     case GG:                                                                                  \
         if (precision == EDX_C128)                                                            \

@@ -1027,3 +1027,45 @@ def test_batched_minres_graph_replay_equals_eager(core, monkeypatch, precision):
                                                  precision=precision)
         monkeypatch.delenv("EDX_NO_GRAPH")
         assert np.array_equal(ito, ito2) and bool((Xo == Xo2).all())
+
+
+@pytest.mark.parametrize("code", [143, 144, 184])
+def test_pipelined_spmv_schedules_match_oracle(core, code):
+    # the pipelined grid-stride schedules (chosen by the autotuner on large
+    # matrices) forced on a small one, in every value form, with rows longer
+    # than the U * G nonzeros a lane prefetches (dense random terms)
+    from edrixs_amd import siam
+    m = H.small_siam(nbath=4, seed=2)
+    fn = H.fock_words(m.norb, m.nocc + 1)
+    bn = core.Basis.from_words(fn, m.norb)
+    A = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    R = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    rng = np.random.default_rng(6)
+    n12, N12 = 12, 6
+    ht = siam.emat_terms(H.random_hermitian_emat(n12, rng, density=1.0))
+    ut = siam.umat_terms(H.random_hermitian_umat(n12, rng, nnz=400))
+    f12 = H.fock_words(n12, N12)
+    B = core.build_ham_i(core.Basis.from_words(f12, n12), ht, ut)
+    RB = o.build_ham_i(f12, ht, ut)
+    assert np.diff(RB.rowptr).max() > 64          # beyond the prefetched window
+    for Amat, Rmat in ((A, R), (B, RB)):
+        x = rng.normal(size=Amat.n) + 1j * rng.normal(size=Amat.n)
+        y0 = rng.normal(size=Amat.m) + 1j * rng.normal(size=Amat.m)
+        ref = o.matvec(Rmat, x, y0)
+        scale = np.abs(Rmat.vals).max() * np.abs(x).max() * np.diff(Rmat.rowptr).max()
+        for form, tol in (("c128", 1e-14), ("c128i", 1e-14), ("c64d", 2e-6), ("c64", 2e-6)):
+            Amat.prepare(form)
+            Amat.spmv_lanes(form, set=code)
+            assert Amat.spmv_lanes(form) == code
+            y = Amat.spmv(_to_dev(core, x), _to_dev(core, y0), precision=form).cpu().numpy()
+            assert np.abs(y - ref).max() < tol * scale, (form, code)
+    # a Krylov recurrence (fused epilogue and dot partials) on the forced schedule
+    A.spmv_lanes("c128", set=8)
+    v0 = _to_dev(core, rng.normal(size=A.n) + 0j)
+    a1, b1, _ = core.build_krylov(A, v0, 40)
+    A.spmv_lanes("c128", set=code)
+    a2, b2, _ = core.build_krylov(A, v0, 40)
+    np.testing.assert_allclose(a2, a1, rtol=1e-12)
+    np.testing.assert_allclose(b2, b1, rtol=1e-12, atol=1e-12 * np.abs(a1).max())
+    with pytest.raises(Exception, match="SpMV schedule"):
+        A.spmv_lanes("c128", set=145)
