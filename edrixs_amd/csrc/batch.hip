@@ -348,21 +348,35 @@ __global__ void k_bdiv_sqrt_norm(int64_t n, int K, double2 *v, const double *nor
 
 // ------------------------------------------------------- batched Krylov
 
+// The recurrence keeps its vectors unnormalized: the stored u_i is the
+// residual w_i of the previous step, with v_i = u_i / s_i and s_i = beta_i
+// (s_1 = 1 after the start vector is normalized).  The product of a step is
+// y = (1/s_i) H u_i - (beta_i / s_{i-1}) u_{i-1} = H v_i - beta_i v_{i-1}, its
+// dot <u_i|y> / s_i = alpha_i, and the residual y - (alpha_i / s_i) u_i is the
+// next stored vector: no pass divides a vector by beta.
 struct BKrylov {
     double *alpha;     // [maxn * K], alpha[(i-1)*K + k]
     double *beta;      // [(maxn+1) * K]
     double *norm;      // [K]
     int *neff;         // [K]
     int *run;          // [(maxn+1) * K]
-    double2 *c1;       // [K]  -beta_{i-1}
+    double2 *c1;       // [K]  -beta_i / s_{i-1}: coefficient of u_{i-1} in the product
     double2 *sums;     // [K]
+    double2 *c0;       // [K]  1 / s_i: scale of H u_i in the product
+    double *acoef;     // [K]  alpha_i / s_i: coefficient of u_i in the residual
+    double *scur;      // [K]  s_i
 };
+
+// doubles of the per-RHS tail of the scalar block (norm .. scur) per RHS
+constexpr int BK_TAIL = 1 + 2 + 2 + 2 + 1 + 1;
 
 __global__ void k_bk_init(BKrylov s, int K, const double2 *sums) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     s.norm[k] = sums[k].x;
     s.c1[k] = make_double2(-0.0, -0.0);
+    s.c0[k] = make_double2(1.0, 0.0);
+    s.scur[k] = 1.0;
     s.neff[k] = 0;
     s.run[k] = sums[k].x > 0.0 ? 1 : 0;
 }
@@ -371,7 +385,10 @@ __global__ void k_bk_alpha(BKrylov s, int K, int i, const double2 *sums) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     if (!s.run[(i - 1) * K + k]) return;
-    s.alpha[(int64_t)(i - 1) * K + k] = sums[k].x;
+    const double inv = s.c0[k].x;
+    const double a = sums[k].x * inv;
+    s.alpha[(int64_t)(i - 1) * K + k] = a;
+    s.acoef[k] = a * inv;
     s.neff[k] = i;
 }
 
@@ -401,18 +418,10 @@ __global__ void k_bk_beta(BKrylov s, int K, int i, const double2 *sums) {
     }
     const double b = sqrt(sums[k].x);
     s.beta[(int64_t)i * K + k] = b;
-    s.c1[k] = make_double2(-b, -0.0);
+    s.c1[k] = make_double2(-b / s.scur[k], -0.0);
+    s.scur[k] = b;
+    s.c0[k] = make_double2(b > 0.0 ? 1.0 / b : 0.0, 0.0);
     s.run[i * K + k] = fabs(b) < 1e-10 ? 0 : 1;   // src/lanczos.f90:290-293
-}
-
-__global__ void k_bdiv_beta(int64_t n, int K, double2 *v, const double *beta_i, const int *run_i) {
-    ELEM_SETUP
-    if (!kv || !run_i[k]) return;
-    const double d = beta_i[k];
-    for (int64_t r = rstart; r < n; r += rstride) {
-        const double2 t = v[r * K + k];
-        v[r * K + k] = make_double2(t.x / d, t.y / d);
-    }
 }
 
 // ------------------------------------------------------- batched MINRES
@@ -949,18 +958,22 @@ namespace {
 int kb_layout(edx_krylov_batch *q, int cap, const void *old_sc, const void *old_flags, int old_cap) {
     const int K = q->K;
     Dbuf sc, flags;
-    const size_t nsc = (size_t)cap * K + (size_t)(cap + 1) * K + K;   // alpha, beta, norm
-    EDX_TRY(sc.alloc(nsc * sizeof(double) + (size_t)2 * K * sizeof(double2), "batched scalars"));
+    // alpha, beta, then the per-RHS tail norm, c1, sums, c0, acoef, scur
+    const size_t nsc = (size_t)cap * K + (size_t)(cap + 1) * K + (size_t)BK_TAIL * K;
+    EDX_TRY(sc.alloc(nsc * sizeof(double), "batched scalars"));
     EDX_TRY(flags.alloc(((size_t)(cap + 1) * K + K) * sizeof(int), "batched flags"));
     hipStream_t s = edx::stream();
-    EDX_HIP(hipMemsetAsync(sc.p, 0, nsc * sizeof(double) + (size_t)2 * K * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(sc.p, 0, nsc * sizeof(double), s));
     EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(cap + 1) * K + K) * sizeof(int), s));
     BKrylov st;
     st.alpha = sc.as<double>();
     st.beta = st.alpha + (size_t)cap * K;
     st.norm = st.beta + (size_t)(cap + 1) * K;
-    st.c1 = (double2 *)(st.norm + K);
+    st.c1 = (double2 *)(st.norm + K);    // 16-byte aligned: 2 (cap + 1) K doubles precede it
     st.sums = st.c1 + K;
+    st.c0 = st.sums + K;
+    st.acoef = (double *)(st.c0 + K);
+    st.scur = st.acoef + K;
     st.run = flags.as<int>();
     st.neff = st.run + (size_t)(cap + 1) * K;
     if (old_sc) {
@@ -969,7 +982,7 @@ int kb_layout(edx_krylov_batch *q, int cap, const void *old_sc, const void *old_
                                hipMemcpyDeviceToDevice, s));
         EDX_HIP(hipMemcpyAsync(st.beta, o.beta, (size_t)(old_cap + 1) * K * sizeof(double),
                                hipMemcpyDeviceToDevice, s));
-        EDX_HIP(hipMemcpyAsync(st.norm, o.norm, K * sizeof(double) + 2 * K * sizeof(double2),
+        EDX_HIP(hipMemcpyAsync(st.norm, o.norm, (size_t)BK_TAIL * K * sizeof(double),
                                hipMemcpyDeviceToDevice, s));
         EDX_HIP(hipMemcpyAsync(st.run, o.run, (size_t)(old_cap + 1) * K * sizeof(int),
                                hipMemcpyDeviceToDevice, s));
@@ -1036,9 +1049,10 @@ int kb_advance(edx_krylov_batch *q, int nsteps) {
     for (int i = q->done + 1; i <= end; ++i) {
         const int curr = q->curr, prev = q->prev;
         if (q->precision == EDX_C64) ELAUNCH(k_il_to_c64, n, K, n, K, v[curr], q->x32.as<float2>());
-        BEpi e{};
+        BEpi e{};   // y = (1/s_i) H u_i - (beta_i/s_{i-1}) u_{i-1}; partial <u_i|y>
         e.y = v[prev];
         e.z = v[prev];
+        e.c0 = st.c0;
         e.c1 = st.c1;
         e.dvec = v[curr];
         e.partials = q->part.as<double2>();
@@ -1046,22 +1060,18 @@ int kb_advance(edx_krylov_batch *q, int nsteps) {
         EDX_TRY(edx::bspmm(q->h, q->precision, K, v[curr], q->x32.as<float2>(), e, &nb));
         hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nb, K, st.sums);
         hipLaunchKernelGGL(k_bk_alpha, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
+        // u_{i+1} = y - (alpha_i/s_i) u_i, kept unnormalized (s_{i+1} = beta_{i+1})
         if (single) {   // one vector of a large matrix: the plain vector kernels
             hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
-                               st.alpha + (i - 1), q->part.as<double2>(), nullptr);
+                               st.acoef, q->part.as<double2>(), nullptr);
             hipLaunchKernelGGL(k_breduce, dim3(1), dim3(256), 0, s, q->part.as<double2>(), nbv, 1,
                                st.sums);
-            hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
-            hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[prev], st.beta + i,
-                               st.run + i);
         } else {
-            ELAUNCH(k_baxpy_norm, n, K, n, K, v[prev], v[curr], st.alpha + (size_t)(i - 1) * K,
-                    q->part.as<double2>());
+            ELAUNCH(k_baxpy_norm, n, K, n, K, v[prev], v[curr], st.acoef, q->part.as<double2>());
             hipLaunchKernelGGL(k_breduce, dim3(K), dim3(256), 0, s, q->part.as<double2>(), nbe, K,
                                st.sums);
-            hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
-            ELAUNCH(k_bdiv_beta, n, K, n, K, v[prev], st.beta + (size_t)i * K, st.run + (size_t)i * K);
         }
+        hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, K, i, st.sums);
         EDX_LAUNCH_CHECK();
         std::swap(q->curr, q->prev);
         q->done = i;

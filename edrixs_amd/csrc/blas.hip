@@ -86,12 +86,16 @@ struct KrylovDev {
     double2 sum;       // scratch for a reduced scalar
     double norm;       // <v0|v0>
     double sqrt_norm;
+    double scur;       // scale of the current vector when it is kept unnormalized
+    double acoef;      // alpha_i / scur
     int neff;
 };
 
 __global__ void k_krylov_init(KrylovDev *st, int *run) {
     st->coef[0] = make_double2(1.0, 0.0);
     st->coef[1] = make_double2(-0.0, 0.0);
+    st->scur = 1.0;
+    st->acoef = 0.0;
     st->neff = 0;
     run[0] = 1;
 }

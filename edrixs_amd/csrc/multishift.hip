@@ -103,7 +103,8 @@ __global__ __launch_bounds__(256) void k_ms_scalars(int i, const double *alpha, 
 // Q and out are mP x K column-major.
 __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *prow,
                                                    const int32_t *pcol, const double2 *pval,
-                                                   const double2 *v, const double *beta_prev,
+                                                   const double2 *v, const double *vscale,
+                                                   const double *beta_prev,
                                                    const MsShift *sh, int K, double2 *Q,
                                                    double2 *out, const int *run, const MsCtl *ctl) {
     if (!run[0] || ctl->nactive == 0) return;
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
     }
     __syncthreads();
     const double bp = *beta_prev;
+    const double vs = *vscale;   // v_i = vscale * u_i (the stored vector is unnormalized)
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
         double2 w;
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
         } else {
             w = v[r];
         }
+        w = make_double2(w.x * vs, w.y * vs);
         for (int k = 0; k < K; ++k) {
             if (!s_act[k]) continue;
             const int64_t o = (int64_t)k * mP + r;
@@ -133,6 +136,22 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
             Q[o] = q;
             out[o] = cadd(out[o], cmul(s_z[k], q));
         }
+    }
+}
+
+// The Lanczos vectors are kept unnormalized: u_i with v_i = u_i / s_i and
+// s_i = beta_i (s_1 = 1).  alpha_i = <u_i | y> / s_i for the product
+// y = (1/s_i) H u_i - (beta_i / s_{i-1}) u_{i-1}, and u_{i+1} = y - (alpha_i / s_i) u_i.
+__global__ __launch_bounds__(VB) void k_ms_alpha(const double2 *partials, int nb, KrylovDev *st,
+                                                 double *alpha, int i, const int *run) {
+    if (!run[i - 1]) return;
+    double2 s = reduce_partials(partials, nb);
+    if (threadIdx.x == 0) {
+        const double inv = st->coef[0].x;
+        const double a = s.x * inv;
+        alpha[i - 1] = a;
+        st->acoef = a * inv;
+        st->neff = i;
     }
 }
 
@@ -147,7 +166,9 @@ __global__ __launch_bounds__(VB) void k_ms_beta(const double2 *partials, int nb,
     if (threadIdx.x == 0) {
         const double b = sqrt(s.x);
         beta[i] = b;
-        st->coef[1] = make_double2(-b, -0.0);
+        st->coef[1] = make_double2(-b / st->scur, -0.0);
+        st->scur = b;
+        st->coef[0] = make_double2(b > 0.0 ? 1.0 / b : 0.0, 0.0);
         run[i] = (fabs(b) < 1e-10 || ctl->nactive == 0) ? 0 : 1;
     }
 }
@@ -240,7 +261,7 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         e.run = run + (i - 1);
         int nb = 0;
         EDX_TRY(apply(h, precision, v[curr], x32.as<float2>(), e, &nb, run + (i - 1)));
-        hipLaunchKernelGGL(k_krylov_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
+        hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
                            d_alpha, i, run);
         hipLaunchKernelGGL(k_ms_scalars, dim3(1), dim3(256), 0, s, i, d_alpha, d_beta, S, sh, K,
                            ctl, run);
@@ -249,14 +270,14 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
             const int64_t mp = P ? P->m : n;
             hipLaunchKernelGGL(k_ms_update, dim3(vec_blocks(mp)), dim3(256), 0, s, mp,
                                P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
-                               P ? P->d_vals : nullptr, v[curr], d_beta + (i - 1), sh, K,
+                               P ? P->d_vals : nullptr, v[curr], &S->coef[0].x,
+                               d_beta + (i - 1), sh, K,
                                qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl);
         }
         hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
-                           d_alpha + (i - 1), part.as<double2>(), run + (i - 1));
+                           &S->acoef, part.as<double2>(), run + (i - 1));
         hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S, d_beta,
                            i, run, ctl);
-        hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[prev], d_beta + i, run + i);
         EDX_LAUNCH_CHECK();
         std::swap(curr, prev);
         if (i - checked >= 16 && i < maxit) {
