@@ -133,9 +133,13 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
         }
     }
     std::vector<BSched> cand = {BSched{0, 8}, BSched{0, 16}};
-    if (K >= 8)
+    if (K >= 8) {
         for (int npl : {1, 2, 4, 8})
             if (16 * npl <= ((K + 15) / 16) * 16) cand.push_back(BSched{1, npl});
+        for (int bpc : {4, 8})   // pipelined RHS-parallel, workgroups per CU
+            if (bsched_blocks(a, BSched{2, bpc}) <= bspmm_blocks(a->m))
+                cand.push_back(BSched{2, bpc});
+    }
     const size_t nk = (size_t)a->n * K;
     double2 *X = (double2 *)dmalloc(nk * sizeof(double2), "tune X");
     float2 *X32 = precision == EDX_C64 ? (float2 *)dmalloc(nk * sizeof(float2), "tune X32") : nullptr;

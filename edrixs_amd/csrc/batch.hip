@@ -52,6 +52,116 @@ __device__ __forceinline__ double2 epilogue(const BEpi &e, int k, int64_t o, dou
     return out;
 }
 
+__device__ __forceinline__ uint16_t shfl16(uint16_t v, int src) {
+    return (uint16_t)__shfl((int)v, src, 16);
+}
+__device__ __forceinline__ float2 shfl16(float2 v, int src) {
+    return make_float2(__shfl(v.x, src, 16), __shfl(v.y, src, 16));
+}
+__device__ __forceinline__ double2 shfl16(double2 v, int src) {
+    return make_double2(__shfl(v.x, src, 16), __shfl(v.y, src, 16));
+}
+
+// Pipelined RHS-parallel schedule: 16-lane groups own rows and lane l owns
+// right-hand side blockIdx.y * 16 + l, as in k_spmm_rl, on a capped grid whose
+// workgroups of one XCD (blockIdx.x % 8) walk one contiguous eighth of the
+// rows.  Lane l holds the column and value numbers of nonzeros l and l + 16 of
+// its group's row (one coalesced load per row instead of a broadcast per
+// nonzero), hands them to the group by shuffles while each lane gathers its own
+// right-hand side, and loads those of the group's next row first.  Nonzeros
+// beyond 32 in a row are read in place.
+template <class VP, class XT, class ACC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ cols, VP vals,
+                                                  const double2 *__restrict__ diag,
+                                                  const XT *__restrict__ X,
+                                                  const double2 *__restrict__ X64, BEpi e) {
+    using PV = PipeVal<VP>;
+    using H = typename PV::type;
+    __shared__ double2 red[16 * 16];
+    const int l = threadIdx.x & 15;
+    const int grp = threadIdx.x >> 4;
+    const int k = blockIdx.y * 16 + l;
+    const bool kv = k < K;
+    const int64_t per_x = gridDim.x / 8, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
+    const int64_t rows_x = (m + 7) / 8;
+    const int64_t rbeg = xcd * rows_x, rend = min(m, rbeg + rows_x);
+    const int64_t stride = per_x * 16;
+    double2 dot = make_double2(0, 0);
+    int64_t r = rbeg + w * 16 + grp;
+    int64_t b = 0, end = 0;
+    if (r < rend) {
+        b = rowptr[r];
+        end = rowptr[r + 1];
+    }
+    int ca = b + l < end ? cols[b + l] : 0, cb = b + 16 + l < end ? cols[b + 16 + l] : 0;
+    H ha = b + l < end ? PV::load(vals, b + l) : PV::zero();
+    H hb = b + 16 + l < end ? PV::load(vals, b + 16 + l) : PV::zero();
+    while (__any(r < rend)) {
+        const int64_t rn = r + stride;
+        int64_t bn = 0, endn = 0;
+        if (rn < rend) {
+            bn = rowptr[rn];
+            endn = rowptr[rn + 1];
+        }
+        const int can = bn + l < endn ? cols[bn + l] : 0;
+        const int cbn = bn + 16 + l < endn ? cols[bn + 16 + l] : 0;
+        const H han = bn + l < endn ? PV::load(vals, bn + l) : PV::zero();
+        const H hbn = bn + 16 + l < endn ? PV::load(vals, bn + 16 + l) : PV::zero();
+        if (r < rend) {
+            ACC sr = 0, si = 0;
+            const int cnt = (int)(end - b);
+            const int nf = cnt < 32 ? cnt : 32;
+            for (int t = 0; t < nf; ++t) {
+                const int src = t & 15;
+                const int c = __shfl(t < 16 ? ca : cb, src, 16);
+                const auto a = PV::value(vals, shfl16(t < 16 ? ha : hb, src));
+                if (kv) {
+                    const XT xv = X[(int64_t)c * K + k];
+                    sr += (ACC)a.x * (ACC)xv.x - (ACC)a.y * (ACC)xv.y;
+                    si += (ACC)a.x * (ACC)xv.y + (ACC)a.y * (ACC)xv.x;
+                }
+            }
+            for (int64_t j = b + 32; j < end; ++j) {
+                const auto a = nt_load2(vals + j);
+                const int64_t c = nt_load(cols + j);
+                if (kv) {
+                    const XT xv = X[c * K + k];
+                    sr += (ACC)a.x * (ACC)xv.x - (ACC)a.y * (ACC)xv.y;
+                    si += (ACC)a.x * (ACC)xv.y + (ACC)a.y * (ACC)xv.x;
+                }
+            }
+            if (kv) {
+                const int64_t o = r * K + k;
+                double2 hx = make_double2((double)sr, (double)si);
+                if (SPLIT) hx = cadd(hx, cmul(diag[r], X64[o]));
+                const double2 out = epilogue(e, k, o, hx, X64);
+                e.y[o] = out;
+                if (e.dvec) dot = cadd(dot, cconjmul(e.dvec[o], out));
+            }
+        }
+        r = rn;
+        b = bn;
+        end = endn;
+        ca = can;
+        cb = cbn;
+        ha = han;
+        hb = hbn;
+    }
+    if (e.dvec) {
+        red[grp * 16 + l] = dot;
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            const int kk = blockIdx.y * 16 + threadIdx.x;
+            if (kk < K) {
+                double2 s = make_double2(0, 0);
+                for (int g = 0; g < 16; ++g) s = cadd(s, red[g * 16 + threadIdx.x]);
+                e.partials[(int64_t)blockIdx.x * K + kk] = s;
+            }
+        }
+    }
+}
+
 // Row-split schedule: G lanes per row, KT right-hand sides per pass (blockIdx.y).
 template <int G, int KT, class VP, class XT, class ACC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_spmm_il(int64_t m, int K, const int64_t *__restrict__ rowptr,
@@ -678,14 +788,56 @@ struct BSched {
 };
 int tune_spmv(edx_csr *a, int precision);
 int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched);
+int bsched_blocks(const edx_csr *a, BSched sched);
 
 // One launch of the multi-RHS product with an explicit schedule: row-split
 // (sched.rl == 0: sched.p lanes per row, tiles of BKT right-hand sides) or
 // RHS-parallel (sched.rl == 1: tiles of 16 * sched.p right-hand sides).
+// EDX_FORCE_BSCHED="rl:p" replaces the chosen multi-RHS schedule of K >= 8
+// products (rl 0 row-split with p lanes, 1 RHS-parallel with tiles of 16 p, 2
+// pipelined RHS-parallel with p workgroups per CU): for tests that exercise
+// every schedule on small matrices.
+void forced_bsched(int K, BSched *sched) {
+    const char *f = getenv("EDX_FORCE_BSCHED");
+    int rl = 0, p = 0;
+    if (K >= 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && rl >= 0 && rl <= 2 && p > 0)
+        *sched = BSched{rl, p};
+}
+
+// Workgroups (and dot partials per right-hand side) of a schedule's launch.
+// The pipelined schedule needs a multiple of 8 and at most bspmm_blocks(m),
+// the partials every caller provides.
+int bsched_blocks(const edx_csr *a, BSched sched) {
+    if (sched.rl == 2) return (edx::num_cus() * sched.p + 7) / 8 * 8;
+    return bspmm_blocks(a->m);
+}
+
 int bspmm_launch(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
                  const BEpi &e, BSched sched) {
     const int nb = bspmm_blocks(a->m);
-    if (sched.rl) {
+    if (sched.rl == 2) {   // pipelined RHS-parallel, sched.p workgroups per CU
+        const int nbp = bsched_blocks(a, sched);
+        if (nbp > nb)
+            return set_error(EDX_ERR_ARG, "pipelined SpMM needs %d workgroups, %d available", nbp,
+                             nb);
+        dim3 grid((unsigned)nbp, (unsigned)((K + 15) / 16));
+        if (precision == EDX_C128)
+            hipLaunchKernelGGL((k_spmm_rlp<const double2 *, double2, double, false>), grid,
+                               dim3(256), 0, stream(), a->m, K, a->d_rowptr, a->d_cols,
+                               (const double2 *)a->d_vals, nullptr, X64, X64, e);
+        else if (precision == EDX_C64)
+            hipLaunchKernelGGL((k_spmm_rlp<const float2 *, float2, float, false>), grid, dim3(256),
+                               0, stream(), a->m, K, a->d_rowptr, a->d_cols,
+                               (const float2 *)a->d_vals32, nullptr, X32, X64, e);
+        else if (precision == EDX_C128I)
+            hipLaunchKernelGGL((k_spmm_rlp<IdxPtr, double2, double, true>), grid, dim3(256), 0,
+                               stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,
+                               IdxPtr{a->d_off_idx, a->d_vtab}, a->d_diag, X64, X64, e);
+        else
+            hipLaunchKernelGGL((k_spmm_rlp<const float2 *, double2, double, true>), grid,
+                               dim3(256), 0, stream(), a->m, K, a->d_off_rowptr, a->d_off_cols,
+                               (const float2 *)a->d_off_vals32, a->d_diag, X64, X64, e);
+    } else if (sched.rl) {
         const int npl = sched.p;
         dim3 grid((unsigned)nb, (unsigned)((K + 16 * npl - 1) / (16 * npl)));
         const int xmap = nb % 8 == 0 && xcd_map_enabled();
@@ -780,11 +932,12 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
         return spmv_launch(a, precision, precision == EDX_C64 ? (const void *)X32 : (const void *)X64,
                            se, 0, nblocks_out);
     }
-    const int nb = bspmm_blocks(a->m);
-    if (nblocks_out) *nblocks_out = nb;
+    if (nblocks_out) *nblocks_out = bspmm_blocks(a->m);
     if (a->m == 0 || K == 0) return EDX_OK;
     BSched sched = bspmm_rule(a, K);
     if (a->m >= TUNE_MIN_ROWS) EDX_TRY(tune_bspmm(a, precision, K, &sched));
+    forced_bsched(K, &sched);
+    if (nblocks_out) *nblocks_out = bsched_blocks(a, sched);
     hipEvent_t pe = prof_start(precision, sched.rl ? EDX_KERNEL_SPMM_RL : EDX_KERNEL_SPMM_IL,
                                spmv_bytes_k(a, precision, K, e.z != nullptr, e.dvec != nullptr));
     EDX_TRY(bspmm_launch(a, precision, K, X64, X32, e, sched));

@@ -1069,3 +1069,32 @@ def test_pipelined_spmv_schedules_match_oracle(core, code):
     np.testing.assert_allclose(b2, b1, rtol=1e-12, atol=1e-12 * np.abs(a1).max())
     with pytest.raises(Exception, match="SpMV schedule"):
         A.spmv_lanes("c128", set=145)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c128i", "c64d"])
+def test_pipelined_rhs_parallel_spmm_matches_other_schedules(core, monkeypatch, precision):
+    # the pipelined RHS-parallel SpMM (chosen by the autotuner for large
+    # matrices) forced on a small one: a 16-vector Krylov batch and a 12-vector
+    # shifted MINRES against the default schedules, rows longer than the 32
+    # nonzeros a lane group prefetches included
+    m, fn, bn, A, R = _siam_hn(core)
+    A.prepare(precision)
+    assert np.diff(R.rowptr).max() > 32
+    rng = np.random.default_rng(21)
+    V = _to_dev(core, rng.normal(size=(16, A.n)) + 1j * rng.normal(size=(16, A.n)))
+    monkeypatch.delenv("EDX_FORCE_BSCHED", raising=False)
+    a0, b0, n0 = core.build_krylov_batch(A, V, 60, precision=precision)
+    monkeypatch.setenv("EDX_FORCE_BSCHED", "2:4")
+    a1, b1, n1 = core.build_krylov_batch(A, V, 60, precision=precision)
+    # 60 steps of the plain recurrence amplify the summation-order rounding
+    for k in range(16):
+        np.testing.assert_allclose(a1[k], a0[k], rtol=1e-10)
+        np.testing.assert_allclose(b1[k], b0[k], rtol=1e-10, atol=1e-10 * np.abs(a0[k]).max())
+    sig = -20.0 + 2.5 * np.arange(12) + 0.4j
+    B = V[:12].contiguous()
+    X1, it1, _ = core.minres_shifted_batch(A, sig, B, tol=1e-10, maxit=600, precision=precision)
+    monkeypatch.delenv("EDX_FORCE_BSCHED")
+    X0, it0, _ = core.minres_shifted_batch(A, sig, B, tol=1e-10, maxit=600, precision=precision)
+    assert np.abs(it1.astype(int) - it0.astype(int)).max() <= 2
+    err = (X1 - X0).abs().max().item() / X0.abs().max().item()
+    assert err < 1e-8
