@@ -116,3 +116,58 @@ def rixs_siam(model, ominc, eloss, pol_vectors, workdir=".", gamma_c=0.5, gamma_
                 per_om.append(p)
             poles.append(per_om)
     return rixs, poles
+
+
+def search_gs_sector(norb, hop, coul, imp_orbs, precision="c128i", tol=1e-10, maxiter=500,
+                     log=None):
+    """Particle-number sector of the ground state: the ``do_ed=0`` search of
+    ``ed_siam_fort`` (edrixs/solvers.py:2156-2263), in memory on the device.
+
+    Starts at N = norb // 2 and walks the sectors N+1, N+2, ... and N-1, N-2,
+    ... in the reference's order: a direction stops at the first sector whose
+    ground energy is above the best so far, and a sector that improves on it
+    stops the other direction.  Each sector's Hamiltonian is built on the
+    device from the valence terms (hop, coul) and its lowest eigenvalue found
+    with the Lanczos solver in ``precision``.  Returns (n_gs, e_gs, table) with
+    table a list of (N, E0, impurity occupation) sorted by energy, as the
+    reference writes to search_result.dat; the impurity occupation is the trace
+    of the ground state's density matrix over the orbitals ``imp_orbs``.  With
+    ``log`` (a writable file) each sector is printed as it is evaluated, like
+    search_gs.log."""
+    from . import core
+
+    imp = np.asarray(list(imp_orbs), dtype=int)
+
+    def sector(n):
+        b = core.Basis.combinations(norb, n)
+        H = core.build_ham_i(b, hop, coul)
+        ev, vecs, _ = core.ground_state(H, nev=1, precision=precision, maxiter=maxiter, tol=tol)
+        occ = float(np.real(np.trace(core.denmat(b, norb, vecs[0].contiguous())[np.ix_(imp, imp)])))
+        row = (n, float(ev[0]), occ)
+        if log is not None:
+            print("%d %.10f %.10f" % row, file=log, flush=True)
+        return row
+
+    n0 = norb // 2
+    res = [sector(n0)]
+    n_gs, e_gs = n0, res[0][1]
+    up = down = True
+    for i in range(norb // 2):
+        if up:
+            r = sector(n0 + i + 1)
+            res.append(r)
+            if r[1] > e_gs:
+                up = False
+            else:
+                down = False
+                n_gs, e_gs = r[0], r[1]
+        if down:
+            r = sector(n0 - i - 1)
+            res.append(r)
+            if r[1] > e_gs:
+                down = False
+            else:
+                up = False
+                n_gs, e_gs = r[0], r[1]
+    res.sort(key=lambda t: t[1])
+    return n_gs, e_gs, res

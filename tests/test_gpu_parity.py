@@ -1098,3 +1098,24 @@ def test_pipelined_rhs_parallel_spmm_matches_other_schedules(core, monkeypatch, 
     assert np.abs(it1.astype(int) - it0.astype(int)).max() <= 2
     err = (X1 - X0).abs().max().item() / X0.abs().max().item()
     assert err < 1e-8
+
+
+def test_gs_sector_search_matches_dense_sectors(core):
+    # do_ed=0 sector search on a 12-orbital impurity model with the bath
+    # shifted down (the ground sector is not the half-filled one), against the
+    # dense ground energies of every sector
+    from edrixs_amd import solvers, siam
+    m = siam.SiamModel(nbath=2, seed=3, bath_range=(-6.0, -4.0))
+    hop, coul = m.hop_i, m.coul_i
+    e_dense = {}
+    for n in range(0, m.norb + 1):
+        fock = H.fock_words(m.norb, n)
+        e_dense[n] = float(np.linalg.eigvalsh(o.build_ham_i(fock, hop, coul).todense())[0])
+    n_best = min(e_dense, key=e_dense.get)
+    n_gs, e_gs, table = solvers.search_gs_sector(m.norb, hop, coul, imp_orbs=range(10))
+    assert n_gs == n_best and n_gs != m.norb // 2
+    assert abs(e_gs - e_dense[n_best]) < 1e-9
+    for n, e, occ in table:
+        assert abs(e - e_dense[n]) < 1e-9
+        assert -1e-9 <= occ <= 10 + 1e-9
+    assert table[0][0] == n_gs and [t[1] for t in table] == sorted(t[1] for t in table)
