@@ -9,8 +9,10 @@
 //
 // Each member is a build_krylov_mp recurrence (src/lanczos.f90:266-298) on one
 // vector.  Its rows map to a contiguous range of the SpMV grid (32 rows per
-// block, 8 lanes per row) and its elements to a contiguous range of the vector
-// grid; a block finds its member by binary search over the range starts.  All
+// block, 8 lanes per row; a member of G_PIPE_MIN_ROWS rows or more gets a
+// fixed sub-grid that walks its rows in the pipelined schedule instead) and
+// its elements to a contiguous range of the vector grid; a block finds its
+// member by binary search over the range starts.  All
 // reductions run over the member's own blocks in a fixed order, so a member's
 // numbers are the same whichever other members share the launches: a request
 // gets the same spectrum alone and in any mix.
@@ -26,6 +28,8 @@ namespace {
 constexpr int GG = 8;                    // lanes per row
 constexpr int G_ROWS = 256 / GG;         // rows per SpMV block
 constexpr int G_VEC = 1024;              // vector elements per vector block
+constexpr int64_t G_PIPE_MIN_ROWS = 65536;   // members this large take the pipelined SpMV
+constexpr int GP_G = 4, GP_U = 8;            // its lanes per row and prefetched nonzeros per lane
 
 struct GState {
     double2 c1;      // -beta of the last step (addend scale of the next product)
@@ -47,6 +51,7 @@ struct GMember {
     GState *st;
     int64_t blk0, nblk;      // SpMV blocks
     int64_t vblk0, nvblk;    // vector blocks
+    int pipe;                // SpMV blocks form a pipelined sub-grid (blk0, nblk multiples of 8)
 };
 
 __device__ __forceinline__ int g_find(const GMember *mem, int nmem, int64_t bid, bool vec) {
@@ -75,17 +80,128 @@ __device__ __forceinline__ IdxPtr g_vals<IdxPtr>(const GMember &M) {
     return IdxPtr{M.idx, M.tab};
 }
 
+// A large member's product y = H x + c1 y in the pipelined schedule of
+// k_spmv_pipe: the member's nblk workgroups (a multiple of 8, starting at a
+// multiple of 8) split its rows into XCD-contiguous eighths and walk them
+// grid-stride; each lane loads its next row's GP_U column/value numbers and
+// epilogue operands before it gathers x for the current row.  Returns this
+// thread's share of <x|y>.
+template <class VP, bool SPLIT>
+__device__ double2 g_spmv_pipe(const GMember &M, double2 c1, const double2 *__restrict__ x,
+                               double2 *y, int64_t l) {
+    using PV = PipeVal<VP>;
+    using Hd = typename PV::type;
+    const VP vals = g_vals<VP>(M);
+    const int lig = threadIdx.x & (GP_G - 1);
+    constexpr int64_t gpb = 256 / GP_G;
+    const int64_t per_x = M.nblk / 8, xcd = l % 8, w = l / 8;
+    const int64_t rows_x = (M.m + 7) / 8;
+    const int64_t rbeg = xcd * rows_x, rend = min(M.m, rbeg + rows_x);
+    const int64_t stride = per_x * gpb;
+    double2 dot = make_double2(0, 0);
+    int64_t r = rbeg + w * gpb + threadIdx.x / GP_G;
+    int c[GP_U];
+    Hd h[GP_U];
+    int64_t b = 0, end = 0;
+    double2 pd = make_double2(0, 0), px = pd, py = pd;
+    if (r < rend) {
+        b = M.rowptr[r];
+        end = M.rowptr[r + 1];
+        if (lig == 0) {
+            if (SPLIT) pd = M.diag[r];
+            px = x[r];
+            py = y[r];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < GP_U; ++u) {
+        const int64_t j = b + lig + u * GP_G;
+        c[u] = j < end ? M.cols[j] : -1;
+        h[u] = j < end ? PV::load(vals, j) : PV::zero();
+    }
+    while (__any(r < rend)) {
+        const int64_t rn = r + stride;
+        int cn[GP_U];
+        Hd hn[GP_U];
+        int64_t bn = 0, endn = 0;
+        double2 nd = make_double2(0, 0), nx = nd, ny = nd;
+        if (rn < rend) {
+            bn = M.rowptr[rn];
+            endn = M.rowptr[rn + 1];
+            if (lig == 0) {
+                if (SPLIT) nd = M.diag[rn];
+                nx = x[rn];
+                ny = y[rn];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GP_U; ++u) {
+            const int64_t j = bn + lig + u * GP_G;
+            cn[u] = j < endn ? M.cols[j] : -1;
+            hn[u] = j < endn ? PV::load(vals, j) : PV::zero();
+        }
+        double sr = 0, si = 0;
+        if (r < rend) {
+#pragma unroll
+            for (int u = 0; u < GP_U; ++u) {
+                if (c[u] >= 0) {
+                    const auto a = PV::value(vals, h[u]);
+                    const double2 xv = x[c[u]];
+                    sr += (double)a.x * xv.x - (double)a.y * xv.y;
+                    si += (double)a.x * xv.y + (double)a.y * xv.x;
+                }
+            }
+            for (int64_t j = b + lig + GP_U * GP_G; j < end; j += GP_G) {
+                const auto a = nt_load2(vals + j);
+                const double2 xv = x[nt_load(M.cols + j)];
+                sr += (double)a.x * xv.x - (double)a.y * xv.y;
+                si += (double)a.x * xv.y + (double)a.y * xv.x;
+            }
+        }
+#pragma unroll
+        for (int off = GP_G / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, GP_G);
+            si += __shfl_xor(si, off, GP_G);
+        }
+        if (r < rend && lig == 0) {
+            double2 ax = make_double2(sr, si);
+            if constexpr (SPLIT) ax = cadd(ax, cmul(pd, px));
+            const double2 out = cadd(ax, cmul(c1, py));
+            y[r] = out;
+            dot = cadd(dot, cconjmul(px, out));
+        }
+        r = rn;
+        b = bn;
+        end = endn;
+        pd = nd;
+        px = nx;
+        py = ny;
+#pragma unroll
+        for (int u = 0; u < GP_U; ++u) {
+            c[u] = cn[u];
+            h[u] = hn[u];
+        }
+    }
+    return dot;
+}
+
 template <class VP, bool SPLIT>
 __global__ __launch_bounds__(256) void k_gspmv(const GMember *mem, int nmem, double2 *partials) {
     const int64_t bid = blockIdx.x;
     const GMember M = mem[g_find(mem, nmem, bid, false)];
     const GState st = *M.st;
-    if (!st.run) {   // uniform over the block
+    // stopped member, or a padding block after it (uniform over the block)
+    if (!st.run || bid - M.blk0 >= M.nblk) {
         if (threadIdx.x == 0) partials[bid] = make_double2(0, 0);
         return;
     }
     const double2 *x = (st.steps & 1) ? M.v1 : M.v0;
     double2 *y = (st.steps & 1) ? M.v0 : M.v1;
+    if (M.pipe) {
+        const double2 dot = block_sum_256(g_spmv_pipe<VP, SPLIT>(M, st.c1, x, y, bid - M.blk0));
+        if (threadIdx.x == 0) partials[bid] = dot;
+        return;
+    }
     const VP vals = g_vals<VP>(M);
     const int lig = threadIdx.x & (GG - 1);
     const int64_t r = (bid - M.blk0) * G_ROWS + threadIdx.x / GG;
@@ -334,7 +450,13 @@ int group_layout(edx_group *g) {
         d.alpha = m.ab;
         d.beta = m.ab + g->cap;
         d.st = m.st;
-        d.nblk = (h->m + G_ROWS - 1) / G_ROWS;
+        d.pipe = h->m >= G_PIPE_MIN_ROWS;
+        if (d.pipe) {   // a sub-grid of 4 workgroups per CU starting at a multiple of 8
+            blk = (blk + 7) / 8 * 8;
+            d.nblk = (edx::num_cus() * 4 + 7) / 8 * 8;
+        } else {
+            d.nblk = (h->m + G_ROWS - 1) / G_ROWS;
+        }
         d.nvblk = (h->m + G_VEC - 1) / G_VEC;
         if (d.nblk < 1) d.nblk = 1;
         if (d.nvblk < 1) d.nvblk = 1;

@@ -694,6 +694,53 @@ def test_group_members_match_oracle_and_are_mix_independent(core, precision):
     g2.close()
 
 
+@pytest.mark.parametrize("precision", ["c128", "c64d"])
+def test_group_pipelined_member_matches_oracle_and_is_mix_independent(core, precision, monkeypatch):
+    # H_n of 68,640 rows (at least 65,536: the member's product runs on its own
+    # pipelined sub-grid; rows of up to 35 nonzeros, beyond the 32 a lane group
+    # prefetches) beside small members that keep the one-group-per-row product
+    m, fn, bn, A, R = _siam_hn(core, nbath=6, seed=2)
+    assert A.n >= 65536 and np.diff(R.rowptr).max() > 32
+    m2 = H.small_siam(nbath=3, seed=5)
+    f2 = H.fock_words(m2.norb, m2.nocc)
+    A2 = core.build_ham_i(core.Basis.from_words(f2, m2.norb), m2.hop_i, m2.coul_i)
+    rng = np.random.default_rng(43)
+    va = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    vb = _to_dev(core, rng.normal(size=A2.n) + 1j * rng.normal(size=A2.n))
+    g1 = core.KrylovGroup(precision, max_steps=200)
+    a = g1.add(A, _to_dev(core, va))
+    g1.advance(40)
+    al1, be1, n1, run1 = g1.coefficients(a)
+    g1.close()
+    assert len(al1) == 40 and run1
+    ra, rb, rn = o.build_krylov(R, va, 40)
+    tol = 1e-9 if precision == "c128" else 1e-5
+    np.testing.assert_allclose(al1[:20], ra[:20], rtol=tol, atol=tol)
+    np.testing.assert_allclose(be1[:20], rb[:20], rtol=tol, atol=tol)
+    assert abs(n1 - rn) <= 1e-12 * rn
+
+    def mix():
+        # a small member first, so the large one's sub-grid starts after padding blocks
+        g = core.KrylovGroup(precision, max_steps=200)
+        b = g.add(A2, vb)
+        g.advance(5)
+        a2 = g.add(A, _to_dev(core, va))
+        c = g.add(A2, vb)
+        g.advance(17)
+        g.remove(b)
+        g.advance(23)
+        out = g.coefficients(a2), g.coefficients(c)
+        g.close()
+        return out
+
+    (al2, be2, n2, _), cg = mix()
+    assert np.array_equal(al2, al1) and np.array_equal(be2, be1) and n2 == n1
+    monkeypatch.setenv("EDX_NO_GRAPH", "1")
+    (al3, be3, n3, _), ce = mix()
+    assert np.array_equal(al3, al1) and np.array_equal(be3, be1) and n3 == n1
+    assert len(ce[0]) == 40 and np.array_equal(ce[0], cg[0]) and np.array_equal(ce[1], cg[1])
+
+
 def test_group_breakdown_and_capacity(core):
     m, fn, bn, A, R = _siam_hn(core, nbath=2, seed=4)
     ev = np.linalg.eigh(R.todense())[1][:, 3]
