@@ -15,6 +15,7 @@ The functions mirror the Fortran routines they replace:
 ``CSRMatrix.spmv``          ``matvec_csr``       src/spmv.f90:73-103 (y += A x)
 ``lanczos_eigs``            ``diag_ham_lanczos`` src/lanczos.f90:6-167
 ``block_lanczos_eigs``      ``diag_ham_lanczos`` src/lanczos.f90:6-167 (8-vector blocks)
+``trlan_eigs``              ``diag_ham_arpack``  src/arpack.f90:2-112 (thick restart)
 ``build_krylov``            ``build_krylov_mp``  src/lanczos.f90:210-303
 ``minres``                  ``pminres_csr``      src/linsys.f90:3-128
 ``denmat``                  density matrix       src/ed_driver.f90:219-248
@@ -290,6 +291,23 @@ def lanczos_eigs(H, nev, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None
     _lib.call("edx_lanczos_eigs", H._h, _prec(precision), nev, maxiter, float(tol),
               int(seed), int(bool(deflate)), nvec, _ptr(ev), _dptr(vecs), ctypes.byref(it))
     return ev, vecs[:nvec], it.value
+
+
+def trlan_eigs(H, nev, ncv=None, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None,
+               precision="c128"):
+    """Lowest nev eigenpairs by thick-restart Lanczos (the ARPACK path,
+    ``diag_ham_arpack`` src/arpack.f90:2-112).  ncv: basis size (at least
+    2 nev + 16 is used); maxiter: restarts.  Returns (evals,
+    evecs (nvec x n device tensor), products with H, converged values)."""
+    T = torch()
+    nvec = nev if nvec is None else nvec
+    ncv = nev + 2 if ncv is None else ncv
+    ev = np.zeros(nev)
+    vecs = T.zeros((max(nvec, 1), H.n), dtype=T.complex128, device=device())
+    it, nc = ctypes.c_int(), ctypes.c_int()
+    _lib.call("edx_trlan_eigs", H._h, _prec(precision), nev, ncv, maxiter, float(tol), int(seed),
+              int(bool(deflate)), nvec, _ptr(ev), _dptr(vecs), ctypes.byref(it), ctypes.byref(nc))
+    return ev, vecs[:nvec], it.value, nc.value
 
 
 def lanczos_refine(H, start, nev, maxiter=200, tol=1e-10, nvec=None, precision="c128"):

@@ -367,14 +367,20 @@ int ed_driver() {   // src/ed_driver.f90:1-281
                 hv[(size_t)j * ndim + r] = make_double2(Z[(size_t)r * nvector + j].real(),
                                                         Z[(size_t)r * nvector + j].imag());
         EDX_HIP(hipMemcpy(vecs.p, hv.data(), hv.size() * sizeof(double2), hipMemcpyHostToDevice));
-    } else {   // Lanczos (ed_solver=1; ed_solver=2 is served by the same solver)
+    } else {   // Lanczos (ed_solver=1) or, for ed_solver=2, thick-restart Lanczos with
+               // ARPACK's ncv / maxiter / tol meaning (src/ed_driver.f90:154-172)
         int nev = std::max(neval, nvector);
         std::vector<double> ev(nev);
         Dev tmp;
         EDX_TRY(dev_alloc(tmp, ndim * nev, "eigenvectors"));
-        int iters = 0;
-        EDX_TRY(edx_lanczos_eigs(H.a, EDX_C128, nev, cfg.maxiter, cfg.eigval_tol, lanczos_seed(),
-                                 lanczos_deflate(), nev, ev.data(), tmp.p, &iters));
+        int iters = 0, nconv = 0;
+        if (solver == 2)
+            EDX_TRY(edx_trlan_eigs(H.a, EDX_C128, nev, std::max(cfg.ncv, 1), cfg.maxiter,
+                                   cfg.eigval_tol, lanczos_seed(), lanczos_deflate(), nev, ev.data(),
+                                   tmp.p, &iters, &nconv));
+        else
+            EDX_TRY(edx_lanczos_eigs(H.a, EDX_C128, nev, cfg.maxiter, cfg.eigval_tol, lanczos_seed(),
+                                     lanczos_deflate(), nev, ev.data(), tmp.p, &iters));
         for (int k = 0; k < neval; ++k) evals[k] = ev[k];
         if (nvector)
             EDX_HIP(hipMemcpy(vecs.p, tmp.p, (size_t)ndim * nvector * sizeof(double2),

@@ -352,6 +352,7 @@ struct edx_group {
     double2 *vpartials = nullptr;
     size_t vpartials_cap = 0;
     int64_t nblk = 0, nvblk = 0;
+    int64_t pipe_nblk = 0;   // SpMV sub-grid of a large member (0: not yet known)
     ~edx_group() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -425,6 +426,21 @@ int group_capture(edx_group *g) {
     return EDX_OK;
 }
 
+// SpMV sub-grid of a large member: up to 4 workgroups per CU, no more than
+// fit at once (the kernel's register use sets that per form), rounded to 8
+int64_t pipe_subgrid(int precision) {
+    int per_cu = 0;
+    hipError_t e;
+    if (precision == EDX_C64D)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gspmv<const float2 *, true>, 256, 0);
+    else if (precision == EDX_C128I)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gspmv<IdxPtr, true>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gspmv<const double2 *, false>, 256, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    return ((int64_t)edx::num_cus() * std::min(per_cu, 4) + 7) / 8 * 8;
+}
+
 int group_layout(edx_group *g) {
     if (!g->dirty) return EDX_OK;
     if (g->exec) {
@@ -451,9 +467,10 @@ int group_layout(edx_group *g) {
         d.beta = m.ab + g->cap;
         d.st = m.st;
         d.pipe = h->m >= G_PIPE_MIN_ROWS;
-        if (d.pipe) {   // a sub-grid of 4 workgroups per CU starting at a multiple of 8
+        if (d.pipe) {   // a resident sub-grid starting at a multiple of 8
+            if (!g->pipe_nblk) g->pipe_nblk = pipe_subgrid(g->precision);
             blk = (blk + 7) / 8 * 8;
-            d.nblk = (edx::num_cus() * 4 + 7) / 8 * 8;
+            d.nblk = g->pipe_nblk;
         } else {
             d.nblk = (h->m + G_ROWS - 1) / G_ROWS;
         }

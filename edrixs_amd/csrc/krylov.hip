@@ -15,6 +15,7 @@
 
 namespace edx {
 int tridiag_ql(int n, const double *d, const double *e, double *w, double *z, double *zlast);
+int herm_eig_lowest(int n, std::complex<double> *A, double *w, int nvec, std::complex<double> *Z);
 }
 
 namespace {
@@ -329,6 +330,109 @@ int lanczos_run(Lanczos &L, int first, int nev, int maxiter, double tol, std::ve
     return EDX_OK;
 }
 
+// Thick-restart Lanczos (K. Wu and H. Simon, SIAM J. Matrix Anal. Appl. 22,
+// 602 (2000)) over basis columns [first, first + m]: the role of ARPACK's
+// implicitly restarted Lanczos in diag_ham_arpack (src/arpack.f90:2-112,
+// which='SR').  A cycle extends the basis to m fully reorthogonalized Lanczos
+// vectors; the projected matrix T is the arrow matrix of the kept Ritz values
+// and their couplings, followed by the tridiagonal of the new steps.  Ritz
+// value i has converged when |beta_m y_{m,i}| <= tol max(eps^(2/3), |theta_i|)
+// (ARPACK's test).  Unconverged, the cycle keeps the `keep` lowest Ritz vectors
+// V y_i as the new first columns, the residual w / beta_m as the next, and
+// T = diag(theta) with couplings beta_m y_{m,i}.  At the end the `take` lowest
+// Ritz vectors are written to X (n x take) and their values to ritz.
+int trl_run(Lanczos &L, int first, int nev, int m, int max_cycles, double tol,
+            std::vector<double> &ritz, double2 *X, double2 *tmp, double *dS, int &take,
+            int &iters, int &cycles, int &nconv) {
+    using cd = std::complex<double>;
+    hipStream_t s = edx::stream();
+    const int64_t n = L.n;
+    m = std::min<int64_t>(m, n - first);
+    std::vector<double> T((size_t)m * m, 0.0);
+    int k = 0;   // kept Ritz vectors at the front of the run's columns
+    const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+    for (cycles = 1;; ++cycles) {
+        int me = m;   // columns of this cycle (fewer on an invariant subspace)
+        double beta = 0.0;
+        for (int j = k; j < m; ++j) {
+            double2 *vj = L.basis + (int64_t)(first + j) * L.ld;
+            SpmvEpi e{};
+            e.y = L.w;
+            EDX_TRY(apply(L.h, L.precision, vj, L.x32, e, nullptr, nullptr));
+            double2 a;
+            EDX_TRY(host_dot(L, vj, L.w, &a));
+            double alpha = a.x;
+            // three-term part inside a tridiagonal stretch; after a restart the
+            // arrow couplings go with the Gram-Schmidt passes
+            const bool tri = j > k;
+            const double2 *vprev = tri ? L.basis + (int64_t)(first + j - 1) * L.ld : nullptr;
+            hipLaunchKernelGGL(k_axpby_sub, dim3(L.nbv), dim3(VB), 0, s, n, L.w, vj, alpha, vprev,
+                               tri ? T[(size_t)(j - 1) * m + j] : 0.0);
+            EDX_LAUNCH_CHECK();
+            std::vector<double2> h1;
+            EDX_TRY(orthogonalize(L, first + j + 1, L.w, &h1));
+            EDX_TRY(orthogonalize(L, first + j + 1, L.w, nullptr));
+            alpha += h1[first + j].x;
+            double2 nb2;
+            EDX_TRY(host_dot(L, L.w, L.w, &nb2));
+            beta = sqrt(fabs(nb2.x));
+            ++iters;
+            T[(size_t)j * m + j] = alpha;
+            if (beta < 1e-10) {
+                me = j + 1;
+                break;
+            }
+            if (j + 1 < m) {
+                T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = beta;
+                hipLaunchKernelGGL(k_copy_div, dim3(L.nbv), dim3(VB), 0, s, n, L.w, beta,
+                                   L.basis + (int64_t)(first + j + 1) * L.ld);
+                EDX_LAUNCH_CHECK();
+            }
+        }
+        // Ritz pairs of the me x me projected matrix
+        std::vector<cd> A((size_t)me * me), Z((size_t)me * me);
+        for (int i = 0; i < me; ++i)
+            for (int j = 0; j < me; ++j) A[(size_t)i * me + j] = T[(size_t)i * m + j];
+        ritz.assign(me, 0.0);
+        EDX_CHECK(edx::herm_eig_lowest(me, A.data(), ritz.data(), me, Z.data()) == 0, EDX_ERR_NUMERIC,
+                  "projected eigenproblem did not converge");
+        const int want = std::min(nev, me);
+        nconv = 0;
+        for (int i = 0; i < want; ++i)
+            if (fabs(beta * Z[(size_t)(me - 1) * me + i].real()) <=
+                tol * std::max(eps23, fabs(ritz[i])))
+                ++nconv;
+        const bool done = nconv >= want || beta < 1e-10 || cycles >= max_cycles || me < m;
+        // keep: the wanted values and half the rest of the cycle (at least one new vector)
+        const int keep = done ? want : std::min(m - 1, want + (m - want) / 2);
+        std::vector<double> S((size_t)keep * me);   // column i: Ritz vector i's coefficients
+        for (int i = 0; i < keep; ++i)
+            for (int c = 0; c < me; ++c) S[(size_t)i * me + c] = Z[(size_t)c * me + i].real();
+        EDX_HIP(hipMemcpy(dS, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice));
+        const double2 *V = L.basis + (int64_t)first * L.ld;
+        if (done) {
+            hipLaunchKernelGGL(k_combine, dim3(L.nbv), dim3(VB), 0, s, n, V, L.ld, me, dS, keep, X, n);
+            EDX_LAUNCH_CHECK();
+            EDX_HIP(hipStreamSynchronize(s));
+            take = keep;
+            return EDX_OK;
+        }
+        hipLaunchKernelGGL(k_combine, dim3(L.nbv), dim3(VB), 0, s, n, V, L.ld, me, dS, keep, tmp, n);
+        EDX_LAUNCH_CHECK();
+        EDX_HIP(hipMemcpyAsync(L.basis + (int64_t)first * L.ld, tmp, (size_t)n * keep * sizeof(double2),
+                               hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_copy_div, dim3(L.nbv), dim3(VB), 0, s, n, L.w, beta,
+                           L.basis + (int64_t)(first + keep) * L.ld);
+        EDX_LAUNCH_CHECK();
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int i = 0; i < keep; ++i) {
+            T[(size_t)i * m + i] = ritz[i];
+            T[(size_t)i * m + keep] = T[(size_t)keep * m + i] = beta * Z[(size_t)(me - 1) * me + i].real();
+        }
+        k = keep;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -367,20 +471,28 @@ __global__ void k_sum_columns(int64_t n, const double2 *S, int64_t ld, int k, do
     }
 }
 
+// Lowest nev eigenpairs.  ncv == 0: one Lanczos run of up to maxiter steps
+// per round; ncv > 0: thick-restart runs (trl_run) over a basis of m >= ncv
+// columns and at most maxiter cycles.  With deflate, further rounds start from
+// random vectors orthogonal to the locked pairs until a round finds nothing
+// below the nev-th value (degenerate levels a single start vector misses).
 int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double tol, uint64_t seed,
                  int deflate, int nvec, double *evals, void *d_evecs, int *niter_out,
-                 const double2 *start, int nstart) {
+                 const double2 *start, int nstart, int ncv = 0, int *nconv_out = nullptr) {
     EDX_TRY(check_square(h, precision));
     EDX_CHECK(nev >= 1 && maxiter >= 1 && evals && nvec >= 0 && nvec <= nev &&
-                  (nvec == 0 || d_evecs),
-              EDX_ERR_ARG, "bad argument (nev=%d maxiter=%d nvec=%d)", nev, maxiter, nvec);
+                  (nvec == 0 || d_evecs) && ncv >= 0,
+              EDX_ERR_ARG, "bad argument (nev=%d maxiter=%d nvec=%d ncv=%d)", nev, maxiter, nvec, ncv);
     const int64_t n = h->n;
     if (nev > n) nev = (int)n;
     if (nvec > nev) nvec = nev;
-    if (maxiter > n) maxiter = (int)n;
+    // thick restart: at least 2 nev + 16 columns, so a cycle adds enough new directions
+    // for clustered spectra (ARPACK's default ncv = nev + 2 stalls there)
+    const int m_trl = ncv > 0 ? (int)std::min<int64_t>(n, std::max(ncv, 2 * nev + 16)) : 0;
+    if (!ncv && maxiter > n) maxiter = (int)n;
     const int max_rounds = deflate ? 2 * nev + 2 : 1;
-    const int cap_cols = nev + maxiter + 2;
-    DevBuf basis, x32, w, part, hv, tmpS, evec_all;
+    const int cap_cols = nev + (ncv ? m_trl : maxiter) + 2;
+    DevBuf basis, x32, w, part, hv, tmpS, evec_all, trl_tmp;
     EDX_TRY(basis.alloc((size_t)n * cap_cols * sizeof(double2), "lanczos basis"));
     if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "lanczos x32"));
     EDX_TRY(w.alloc(n * sizeof(double2), "lanczos w"));
@@ -389,11 +501,12 @@ int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double t
     EDX_TRY(hv.alloc((size_t)cap_cols * sizeof(double2), "coefficients"));
     EDX_TRY(tmpS.alloc((size_t)cap_cols * cap_cols * sizeof(double), "ritz vectors"));
     EDX_TRY(evec_all.alloc((size_t)n * nev * sizeof(double2), "eigenvectors"));
+    if (ncv) EDX_TRY(trl_tmp.alloc((size_t)n * m_trl * sizeof(double2), "thick-restart vectors"));
     Lanczos L{h, precision, n, basis.as<double2>(), n, cap_cols, x32.as<float2>(), w.as<double2>(),
               part.as<double2>(), hv.as<double2>(), nbv};
     hipStream_t s = edx::stream();
     std::vector<double> locked_vals;   // eigenvalues of locked columns [0, nlock)
-    int nlock = 0, iters = 0;
+    int nlock = 0, iters = 0, nconv = 0;
     for (int round = 0; round < max_rounds; ++round) {
         double2 *v0 = L.basis + (int64_t)nlock * n;
         if (round == 0 && start)
@@ -409,17 +522,25 @@ int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double t
         if (sqrt(nn.x) < 1e-12) break;   // nothing left outside the locked space
         hipLaunchKernelGGL(k_scale_real, dim3(nbv), dim3(VB), 0, s, n, v0, 1.0 / sqrt(nn.x));
         std::vector<double> ritz, S;
-        int m = 0;
-        EDX_TRY(lanczos_run(L, nlock, nev, maxiter, tol, ritz, S, m, iters));
-        // Ritz vectors of this run for its lowest values
-        const int take = std::min(nev, m);
-        std::vector<double> Sk((size_t)m * take);
-        for (int j = 0; j < take; ++j)
-            for (int c = 0; c < m; ++c) Sk[(size_t)j * m + c] = S[(size_t)j * m + c];
-        EDX_HIP(hipMemcpy(tmpS.p, Sk.data(), Sk.size() * sizeof(double), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_combine, dim3(nbv), dim3(VB), 0, s, n, L.basis + (int64_t)nlock * n, n,
-                           m, tmpS.as<double>(), take, evec_all.as<double2>(), n);
-        EDX_LAUNCH_CHECK();
+        int take = 0;
+        if (ncv) {
+            int cycles = 0, nc = 0;
+            EDX_TRY(trl_run(L, nlock, nev, m_trl, maxiter, tol, ritz, evec_all.as<double2>(),
+                            trl_tmp.as<double2>(), tmpS.as<double>(), take, iters, cycles, nc));
+            if (round == 0) nconv = nc;
+        } else {
+            int m = 0;
+            EDX_TRY(lanczos_run(L, nlock, nev, maxiter, tol, ritz, S, m, iters));
+            // Ritz vectors of this run for its lowest values
+            take = std::min(nev, m);
+            std::vector<double> Sk((size_t)m * take);
+            for (int j = 0; j < take; ++j)
+                for (int c = 0; c < m; ++c) Sk[(size_t)j * m + c] = S[(size_t)j * m + c];
+            EDX_HIP(hipMemcpy(tmpS.p, Sk.data(), Sk.size() * sizeof(double), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_combine, dim3(nbv), dim3(VB), 0, s, n, L.basis + (int64_t)nlock * n,
+                               n, m, tmpS.as<double>(), take, evec_all.as<double2>(), n);
+            EDX_LAUNCH_CHECK();
+        }
         // merge: locked (already lowest of previous rounds) with this run's Ritz values
         std::vector<std::pair<double, int>> cand;   // (value, source: <0 locked k, >=0 new j)
         for (int k = 0; k < nlock; ++k) cand.push_back({locked_vals[k], -1 - k});
@@ -457,6 +578,7 @@ int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double t
                                hipMemcpyDeviceToDevice, s));
     EDX_HIP(hipStreamSynchronize(s));
     if (niter_out) *niter_out = iters;
+    if (nconv_out) *nconv_out = nconv;
     return EDX_OK;
 }
 
@@ -469,6 +591,14 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
                      int *niter_out) {
     return lanczos_impl(h, precision, nev, maxiter, tol, seed, deflate, nvec, evals, d_evecs,
                         niter_out, nullptr, 0);
+}
+
+int edx_trlan_eigs(const edx_csr *h, int precision, int nev, int ncv, int maxiter, double tol,
+                   uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
+                   int *nmatvec_out, int *nconv_out) {
+    EDX_CHECK(ncv >= 1, EDX_ERR_ARG, "ncv must be positive (got %d)", ncv);
+    return lanczos_impl(h, precision, nev, maxiter, tol, seed, deflate, nvec, evals, d_evecs,
+                        nmatvec_out, nullptr, 0, ncv, nconv_out);
 }
 
 int edx_build_krylov_reorth(const edx_csr *h, int precision, const void *d_v0, int maxn,

@@ -212,6 +212,20 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
 int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, double tol,
                        int nstart, const void *d_start, int nvec, double *evals, void *d_evecs,
                        int *niter_out);
+/* Lowest nev eigenpairs by thick-restart Lanczos (replaces diag_ham_arpack,
+ * src/arpack.f90:2-112, which='SR', the ed_solver=2 path): a basis of
+ * m = max(ncv, 2 nev + 16) fully reorthogonalized vectors, restarted
+ * at most maxiter times, each restart keeping the wanted Ritz vectors and half
+ * of the others (Wu & Simon 2000).  Converged when every wanted Ritz residual
+ * |beta_m y_m| <= tol * max(eps^(2/3), |theta|), ARPACK's test.  deflate != 0
+ * adds restarts from vectors orthogonal to the pairs found, for degenerate
+ * levels.  evals[nev]; d_evecs n x nvec complex128 column-major (NULL when
+ * nvec == 0); nmatvec_out: products with H (ARPACK's iparam(3) counts
+ * restarts instead); nconv_out: converged values of the first run (iparam(5)).
+ * Keeps about nev + 2m vectors of 16 n bytes. */
+int edx_trlan_eigs(const edx_csr *h, int precision, int nev, int ncv, int maxiter, double tol,
+                   uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
+                   int *nmatvec_out, int *nconv_out);
 /* Lowest nev eigenpairs by block Lanczos (replaces diag_ham_lanczos,
  * src/lanczos.f90:6-167, for several states in one Krylov space): blocks of
  * 8 vectors, one 8-RHS SpMM per step, full reorthogonalization of each block

@@ -404,6 +404,33 @@ def test_full_diag_path_matches_reference_test(core, tmp_path):
     np.testing.assert_allclose(e_gpu, e, atol=1e-9)
 
 
+def test_arpack_path_matches_reference_test(core, tmp_path):
+    """The case of edrixs/tests/test_fortran.py:8-42 through ed_solver=2 (the
+    ARPACK path, here thick-restart Lanczos with ncv = 150 and deflated
+    restarts): all 45 eigenvalues of d^2, whose levels are up to 9-fold."""
+    from edrixs_amd import fedrixs, iostream, siam
+    emat = siam.cf_cubic_d(1.5)
+    umat = siam.umat_slater_d(4.0, 2.0, 0.5)
+    fock = H.fock_words(10, 2)
+    e = np.linalg.eigvalsh(o.dense_operator(10, emat, umat)[np.ix_(fock, fock)])
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        with open("config.in", "w") as f:
+            f.write("&control\ned_solver     =   2\nnum_val_orbs  =   20\nncv           =   150\n"
+                    "neval         =   45\nnvector       =   45\nnum_gs        =   45\n"
+                    "maxiter       =   1000\neigval_tol    =   1E-10\nmin_ndim      =   10\n"
+                    "idump         =   .true.\n&end\n")
+        iostream.write_fock_dec_by_N(10, 2)
+        iostream.write_emat(emat, "hopping_i.in")
+        iostream.write_umat(umat, "coulomb_i.in")
+        fedrixs.ed_fsolver(0, 0, 1)
+        _, e_gpu = np.loadtxt("eigvals.dat", unpack=True)
+    finally:
+        os.chdir(cwd)
+    np.testing.assert_allclose(e_gpu, e, atol=1e-9)
+
+
 def test_missing_input_raises(core, tmp_path):
     from edrixs_amd import EdxError, fedrixs
     cwd = os.getcwd()
@@ -990,6 +1017,69 @@ def test_block_lanczos_matches_single_vector_lanczos(core):
     tiny = core.build_ham_i(core.Basis.from_words(H.fock_words(10, 1), 10), ht, ut)
     with pytest.raises(Exception, match="at least 16 rows"):
         core.block_lanczos_eigs(tiny, 1)
+
+
+def test_trlan_matches_dense_many_states(core):
+    m = H.small_siam(nbath=2, seed=5)          # dim 924
+    fi = H.fock_words(m.norb, m.nocc)
+    A = core.build_ham_i(core.Basis.from_words(fi, m.norb), m.hop_i, m.coul_i)
+    Hd = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    e = np.linalg.eigvalsh(Hd)
+    # ARPACK's default ncv = nev + 2 (raised to 2 nev + 16 = 96) and a larger basis; the
+    # 40 lowest levels of this model lie within 2.7 of each other, 3.4 % of the spectrum
+    for ncv in (42, 120):
+        ev, vecs, nmv, nconv = core.trlan_eigs(A, 40, ncv=ncv, maxiter=400, tol=1e-12, seed=1)
+        np.testing.assert_allclose(ev, e[:40], atol=1e-9)
+        assert nconv == 40
+        V = vecs.cpu().numpy()
+        np.testing.assert_allclose(V.conj() @ V.T, np.eye(40), atol=1e-10)
+        assert np.abs(Hd @ V.T - V.T * ev).max() < 1e-8 and nmv > 0
+    # few states, no deflation, eigenvectors only for the first two
+    ev, vecs, _, nconv = core.trlan_eigs(A, 3, ncv=5, maxiter=400, tol=1e-12, seed=2,
+                                         deflate=False, nvec=2)
+    np.testing.assert_allclose(ev, e[:3], atol=1e-10)
+    assert vecs.shape[0] == 2 and nconv == 3
+    with pytest.raises(Exception, match="ncv"):
+        core.trlan_eigs(A, 3, ncv=0)
+
+
+def test_trlan_degenerate_levels_and_invariant_subspaces(core):
+    from edrixs_amd import siam
+    ht = siam.emat_terms(siam.cf_cubic_d(1.5))
+    ut = siam.umat_terms(siam.umat_slater_d(4.0, 2.0, 0.5))
+    # d^3 (dim 120): 4-, 4- and 6-fold lowest levels; deflated restarts find every copy
+    fock = H.fock_words(10, 3)
+    A = core.build_ham_i(core.Basis.from_words(fock, 10), ht, ut)
+    Hd = o.build_ham_i(fock, ht, ut).todense()
+    e = np.linalg.eigvalsh(Hd)
+    ev, vecs, _, _ = core.trlan_eigs(A, 14, ncv=30, maxiter=300, tol=1e-12, seed=3)
+    np.testing.assert_allclose(ev, e[:14], atol=1e-10)
+    V = vecs.cpu().numpy()
+    np.testing.assert_allclose(V.conj() @ V.T, np.eye(14), atol=1e-10)
+    assert np.abs(Hd @ V.T - V.T * ev).max() < 1e-8
+    # d^2 without Coulomb terms (dim 45, levels 15-, 24- and 6-fold): every
+    # Krylov space is invariant after three steps
+    ut0 = siam.umat_terms(np.zeros((10, 10, 10, 10), complex))
+    fock2 = H.fock_words(10, 2)
+    A2 = core.build_ham_i(core.Basis.from_words(fock2, 10), ht, ut0)
+    e2 = np.linalg.eigvalsh(o.build_ham_i(fock2, ht, ut0).todense())
+    ev2, vecs2, _, _ = core.trlan_eigs(A2, 12, ncv=20, maxiter=100, tol=1e-12, seed=4)
+    np.testing.assert_allclose(ev2, e2[:12], atol=1e-10)
+    W = vecs2.cpu().numpy()
+    np.testing.assert_allclose(W.conj() @ W.T, np.eye(12), atol=1e-10)
+
+
+def test_trlan_matches_single_vector_lanczos(core):
+    m = H.small_siam(nbath=6, seed=7)          # 16 orbitals, N=8: dim 12,870
+    fi = H.fock_words(m.norb, m.nocc)
+    A = core.build_ham_i(core.Basis.from_words(fi, m.norb), m.hop_i, m.coul_i)
+    ev1, v1, _ = core.lanczos_eigs(A, 10, maxiter=1500, tol=1e-11, seed=2, deflate=True)
+    ev, vecs, nmv, nconv = core.trlan_eigs(A, 10, ncv=30, maxiter=500, tol=1e-12, seed=2)
+    np.testing.assert_allclose(ev, ev1, atol=1e-9)
+    assert nconv == 10
+    for k in range(10):
+        y = A.matvec(vecs[k].contiguous())
+        assert float((y - ev[k] * vecs[k]).norm()) < 1e-8
 
 
 def test_interleaved_corehole_layout_is_the_permuted_matrix(core):
