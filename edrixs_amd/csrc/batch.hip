@@ -776,6 +776,20 @@ __global__ void k_vt_index(int64_t m, const int64_t *rowptr, const int32_t *cols
     }
 }
 
+// compact the table: slot s (fingerprint set) -> entry remap[s], so the
+// nvtab distinct values sit in nvtab consecutive entries
+__global__ void k_vt_compact(const unsigned long long *fp, const double2 *tab, const uint16_t *remap,
+                             double2 *out) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < VTAB_SLOTS; s += gridDim.x * blockDim.x)
+        if (fp[s]) out[remap[s]] = tab[s];
+}
+
+__global__ void k_vt_renumber(int64_t nnz, const uint16_t *remap, uint16_t *idx) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * blockDim.x)
+        idx[j] = remap[idx[j]];
+}
+
 }  // namespace
 
 namespace edx {
@@ -1040,8 +1054,38 @@ int csr_add_indexed(edx_csr *a) {
         rc = set_error(EDX_ERR_HIP, "indexed form: download failed");
         goto fail;
     }
-    a->nvtab = 0;
-    for (unsigned long long f : hfp) a->nvtab += f != 0ull;
+    {   // renumber the values 0 .. nvtab-1 in slot order: a dense table of nvtab entries
+        std::vector<uint16_t> remap(VTAB_SLOTS, 0);
+        int64_t k = 0;
+        for (int sl = 0; sl < VTAB_SLOTS; ++sl)
+            if (hfp[sl]) remap[sl] = (uint16_t)k++;
+        a->nvtab = k;
+        auto *d_remap = (uint16_t *)dmalloc(VTAB_SLOTS * sizeof(uint16_t), "value renumbering");
+        auto *tabc = (double2 *)dmalloc((size_t)std::max<int64_t>(k, 1) * sizeof(double2), "value table");
+        if (!d_remap || !tabc) {
+            dfree(d_remap);
+            dfree(tabc);
+            rc = EDX_ERR_NOMEM;
+            goto fail;
+        }
+        bool ok = hipMemcpy(d_remap, remap.data(), VTAB_SLOTS * sizeof(uint16_t), hipMemcpyHostToDevice) ==
+                  hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(k_vt_compact, dim3(VTAB_SLOTS / 256), dim3(256), 0, stream(), fp, tab, d_remap,
+                               tabc);
+            if (a->off_nnz)
+                hipLaunchKernelGGL(k_vt_renumber, dim3(elem_blocks(a->off_nnz)), dim3(256), 0, stream(),
+                                   a->off_nnz, d_remap, idx);
+            ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(stream()) == hipSuccess;
+        }
+        dfree(d_remap);
+        dfree(tab);
+        tab = tabc;
+        if (!ok) {
+            rc = set_error(EDX_ERR_HIP, "indexed form: renumbering failed");
+            goto fail;
+        }
+    }
     dfree(fp);
     dfree(flag);
     a->d_vtab = tab;

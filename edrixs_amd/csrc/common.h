@@ -89,7 +89,7 @@ struct edx_csr {
     // indexed form (EDX_C128I): the split form's off-diagonal pattern with, per
     // nonzero, the number of its exact value in a table of distinct values
     uint16_t *d_off_idx;
-    double2 *d_vtab;     // VTAB_SLOTS entries (hash-addressed; unused slots zero)
+    double2 *d_vtab;     // nvtab entries: value number k -> its exact value
     int64_t nvtab;       // distinct off-diagonal values
     // index layout of the core-hole space on the rows and/or columns that are
     // in it (edx_corehole_layout); 0 for matrices without one

@@ -61,7 +61,19 @@ struct SpmvEpi {
     double2 *partials;      // one per block when dvec != nullptr
     const int *run;         // nullptr or flag: skip the launch when *run == 0
     int xmap;               // gridDim.x % 8 == 0: XCD-contiguous block order (set by the launcher)
+    int lds_tab;            // indexed form: table entries staged in LDS (0: read from global)
 };
+
+// largest value table of the indexed form that the kernels stage in LDS
+constexpr int LDS_TAB_MAX = 2048;
+
+inline bool lds_tab_enabled() {
+    static const bool on = [] {
+        const char *v = std::getenv("EDX_NO_LDS_TAB");
+        return !(v && *v && *v != '0');
+    }();
+    return on;
+}
 
 // block-level sum of a complex value (256 threads), result valid in thread 0
 __device__ __forceinline__ double2 block_sum_256(double2 v) {
@@ -183,6 +195,21 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__r
     using PV = PipeVal<VP>;
     using H = typename PV::type;
     using acc_t = ACC;
+    // indexed form: the value table in LDS, so value lookups stay off the vector cache
+    extern __shared__ double2 s_tab[];
+    constexpr bool IDX = std::is_same<VP, IdxPtr>::value;
+    if constexpr (IDX) {
+        if (e.lds_tab) {
+            for (int i = threadIdx.x; i < e.lds_tab; i += 256) s_tab[i] = vals.tab[i];
+            __syncthreads();
+        }
+    }
+    auto value_of = [&](H hv) {
+        if constexpr (IDX) {
+            if (e.lds_tab) return s_tab[hv];
+        }
+        return PV::value(vals, hv);
+    };
     const int lig = threadIdx.x & (G - 1);
     constexpr int64_t gpb = 256 / G;   // row groups per workgroup
     const int64_t per_x = gridDim.x / 8, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
@@ -243,14 +270,14 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__r
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (c[u] >= 0) {
-                    const auto a = PV::value(vals, h[u]);
+                    const auto a = value_of(h[u]);
                     const XT xv = x[c[u]];
                     sr += (acc_t)a.x * (acc_t)xv.x - (acc_t)a.y * (acc_t)xv.y;
                     si += (acc_t)a.x * (acc_t)xv.y + (acc_t)a.y * (acc_t)xv.x;
                 }
             }
             for (int64_t j = b + lig + U * G; j < end; j += G) {
-                const auto a = nt_load2(vals + j);
+                const auto a = value_of(PV::load(vals, j));
                 const XT xv = x[nt_load(cols + j)];
                 sr += (acc_t)a.x * (acc_t)xv.x - (acc_t)a.y * (acc_t)xv.y;
                 si += (acc_t)a.x * (acc_t)xv.y + (acc_t)a.y * (acc_t)xv.x;
@@ -497,6 +524,9 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     if (a->m == 0) return EDX_OK;
     SpmvEpi e = e_in;
     e.xmap = nb % 8 == 0 && xcd_map_enabled();
+    e.lds_tab = precision == EDX_C128I && pipe && a->nvtab <= LDS_TAB_MAX && lds_tab_enabled()
+                    ? (int)a->nvtab
+                    : 0;
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
                                spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
     if (pipe) {
@@ -513,8 +543,9 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
                                a->d_diag, e);                                                      \
         else if (precision == EDX_C128I)                                                          \
             hipLaunchKernelGGL((k_spmv_pipe<GG, UU, IdxPtr, double2, double, true>), dim3(nb),     \
-                               dim3(256), 0, stream(), a->m, a->d_off_rowptr, a->d_off_cols,       \
-                               IdxPtr{a->d_off_idx, a->d_vtab}, (const double2 *)x, a->d_diag, e); \
+                               dim3(256), e.lds_tab * sizeof(double2), stream(), a->m,             \
+                               a->d_off_rowptr, a->d_off_cols, IdxPtr{a->d_off_idx, a->d_vtab},    \
+                               (const double2 *)x, a->d_diag, e);                                  \
         else                                                                                      \
             hipLaunchKernelGGL((k_spmv_pipe<GG, UU, const float2 *, float2, float, false>),        \
                                dim3(nb), dim3(256), 0, stream(), a->m, a->d_rowptr, a->d_cols,     \
