@@ -784,6 +784,12 @@ __global__ void k_vt_compact(const unsigned long long *fp, const double2 *tab, c
         if (fp[s]) out[remap[s]] = tab[s];
 }
 
+__global__ void k_vt_pack(int64_t nnz, const int32_t *cols, const uint16_t *idx, uint32_t *pk) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * blockDim.x)
+        pk[j] = (uint32_t)cols[j] | ((uint32_t)idx[j] << PK_COL_BITS);
+}
+
 __global__ void k_vt_renumber(int64_t nnz, const uint16_t *remap, uint16_t *idx) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz;
          j += (int64_t)gridDim.x * blockDim.x)
@@ -1090,6 +1096,20 @@ int csr_add_indexed(edx_csr *a) {
     dfree(flag);
     a->d_vtab = tab;
     a->d_off_idx = idx;
+    // few values and columns: one 32-bit word per nonzero for the single-vector product
+    if (a->n < (int64_t(1) << PK_COL_BITS) && a->nvtab <= (int64_t(1) << (32 - PK_COL_BITS)) &&
+        a->off_nnz) {
+        auto *pk = (uint32_t *)dmalloc((size_t)a->off_nnz * sizeof(uint32_t), "packed indices");
+        if (pk) {   // optional: without the memory the unpacked form serves
+            hipLaunchKernelGGL(k_vt_pack, dim3(elem_blocks(a->off_nnz)), dim3(256), 0, stream(),
+                               a->off_nnz, a->d_off_cols, idx, pk);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream()) != hipSuccess) {
+                dfree(pk);
+                return set_error(EDX_ERR_HIP, "indexed form: packing failed");
+            }
+            a->d_off_pk = pk;
+        }
+    }
     return EDX_OK;
 fail:
     dfree(fp);

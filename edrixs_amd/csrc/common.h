@@ -91,6 +91,8 @@ struct edx_csr {
     uint16_t *d_off_idx;
     double2 *d_vtab;     // nvtab entries: value number k -> its exact value
     int64_t nvtab;       // distinct off-diagonal values
+    // packed indexed form (n < 2^24, nvtab <= 256): column | value number << 24
+    uint32_t *d_off_pk;
     // index layout of the core-hole space on the rows and/or columns that are
     // in it (edx_corehole_layout); 0 for matrices without one
     int corehole_layout;

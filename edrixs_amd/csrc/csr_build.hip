@@ -637,6 +637,7 @@ void edx_csr_free(edx_csr *a) {
     edx::dfree(a->d_off_vals32);
     edx::dfree(a->d_off_idx);
     edx::dfree(a->d_vtab);
+    edx::dfree(a->d_off_pk);
     delete a;
 }
 

@@ -41,6 +41,15 @@ __device__ __forceinline__ double2 nt_load2(IdxPtr p) {
     return p.tab[__builtin_nontemporal_load(p.idx)];
 }
 
+// The packed indexed form: one 32-bit word per nonzero, its column in the low
+// PK_COL_BITS bits and its value number above them (matrices with fewer than
+// 2^24 columns and at most 256 distinct off-diagonal values).
+constexpr int PK_COL_BITS = 24;
+struct PkPtr {
+    const uint32_t *pk;
+    const double2 *tab;
+};
+
 template <class A>
 struct AccT;
 template <>
@@ -162,19 +171,44 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
 
 // What a lane of the pipelined SpMV keeps per prefetched nonzero: the value
 // itself, or for the indexed form its 16-bit number (looked up when used).
+// fetch: the column and the entry of nonzero j when ok (else column -1).
+// indexed: the value comes from a table of distinct values (the kernels may
+// stage it in LDS).
 template <class VP>
 struct PipeVal {
     using type = typename std::remove_cv<typename std::remove_pointer<VP>::type>::type;
+    static constexpr bool indexed = false;
     __device__ static type load(VP v, int64_t j) { return v[j]; }
     __device__ static type zero() { return type{}; }
+    __device__ static void fetch(const int32_t *cols, VP v, int64_t j, bool ok, int &c, type &h) {
+        c = ok ? cols[j] : -1;
+        h = ok ? v[j] : zero();
+    }
     __device__ static type value(VP, type h) { return h; }
 };
 template <>
 struct PipeVal<IdxPtr> {
     using type = uint16_t;
+    static constexpr bool indexed = true;
     __device__ static type load(IdxPtr v, int64_t j) { return v.idx[j]; }
     __device__ static type zero() { return 0; }
+    __device__ static void fetch(const int32_t *cols, IdxPtr v, int64_t j, bool ok, int &c, type &h) {
+        c = ok ? cols[j] : -1;
+        h = ok ? v.idx[j] : (uint16_t)0;
+    }
     __device__ static double2 value(IdxPtr v, type h) { return v.tab[h]; }
+};
+template <>
+struct PipeVal<PkPtr> {
+    using type = uint32_t;
+    static constexpr bool indexed = true;
+    __device__ static type zero() { return 0; }
+    __device__ static void fetch(const int32_t *, PkPtr v, int64_t j, bool ok, int &c, type &h) {
+        const uint32_t w = ok ? v.pk[j] : 0u;
+        c = ok ? (int)(w & ((1u << PK_COL_BITS) - 1)) : -1;
+        h = w >> PK_COL_BITS;
+    }
+    __device__ static double2 value(PkPtr v, type h) { return v.tab[h]; }
 };
 
 // Pipelined single-vector SpMV, the same product and epilogue as k_spmv: a
@@ -197,7 +231,7 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__r
     using acc_t = ACC;
     // indexed form: the value table in LDS, so value lookups stay off the vector cache
     extern __shared__ double2 s_tab[];
-    constexpr bool IDX = std::is_same<VP, IdxPtr>::value;
+    constexpr bool IDX = PV::indexed;
     if constexpr (IDX) {
         if (e.lds_tab) {
             for (int i = threadIdx.x; i < e.lds_tab; i += 256) s_tab[i] = vals.tab[i];
@@ -243,8 +277,7 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__r
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int64_t j = b + lig + u * G;
-        c[u] = j < end ? cols[j] : -1;
-        h[u] = j < end ? PV::load(vals, j) : PV::zero();
+        PV::fetch(cols, vals, j, j < end, c[u], h[u]);
     }
     epi_load(r, pd, px, pz, pv);
     while (__any(r < rend)) {
@@ -259,8 +292,7 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__r
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t j = bn + lig + u * G;
-            cn[u] = j < endn ? cols[j] : -1;
-            hn[u] = j < endn ? PV::load(vals, j) : PV::zero();
+            PV::fetch(cols, vals, j, j < endn, cn[u], hn[u]);
         }
         double2 nd = make_double2(0, 0), nz = nd, nv = nd;
         XT nx{};
@@ -277,8 +309,11 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__r
                 }
             }
             for (int64_t j = b + lig + U * G; j < end; j += G) {
-                const auto a = value_of(PV::load(vals, j));
-                const XT xv = x[nt_load(cols + j)];
+                int cj;
+                H hj;
+                PV::fetch(cols, vals, j, true, cj, hj);
+                const auto a = value_of(hj);
+                const XT xv = x[cj];
                 sr += (acc_t)a.x * (acc_t)xv.x - (acc_t)a.y * (acc_t)xv.y;
                 si += (acc_t)a.x * (acc_t)xv.y + (acc_t)a.y * (acc_t)xv.x;
             }
@@ -527,8 +562,11 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     e.lds_tab = precision == EDX_C128I && pipe && a->nvtab <= LDS_TAB_MAX && lds_tab_enabled()
                     ? (int)a->nvtab
                     : 0;
+    // the packed indexed form reads 4 bytes of index per nonzero instead of 6
+    const bool packed = precision == EDX_C128I && pipe && a->d_off_pk;
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
-                               spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr));
+                               spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr) -
+                                   (packed ? 2.0 * (double)a->off_nnz : 0.0));
     if (pipe) {
 #define EDX_PIPE_CASE(CODE, GG, UU)                                                               \
     case CODE:                                                                                    \
@@ -541,6 +579,11 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
                                dim3(nb), dim3(256), 0, stream(), a->m, a->d_off_rowptr,            \
                                a->d_off_cols, (const float2 *)a->d_off_vals32, (const double2 *)x, \
                                a->d_diag, e);                                                      \
+        else if (precision == EDX_C128I && a->d_off_pk)                                           \
+            hipLaunchKernelGGL((k_spmv_pipe<GG, UU, PkPtr, double2, double, true>), dim3(nb),      \
+                               dim3(256), e.lds_tab * sizeof(double2), stream(), a->m,             \
+                               a->d_off_rowptr, nullptr, PkPtr{a->d_off_pk, a->d_vtab},            \
+                               (const double2 *)x, a->d_diag, e);                                  \
         else if (precision == EDX_C128I)                                                          \
             hipLaunchKernelGGL((k_spmv_pipe<GG, UU, IdxPtr, double2, double, true>), dim3(nb),     \
                                dim3(256), e.lds_tab * sizeof(double2), stream(), a->m,             \
