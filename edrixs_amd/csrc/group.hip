@@ -44,6 +44,8 @@ struct GMember {
     const void *vals;        // double2 (complex128 form) or float2 (split form off-diagonal)
     const uint16_t *idx;     // indexed form: value numbers ...
     const double2 *tab;      // ... into this table
+    const uint32_t *pk;      // packed indexed form (column | value number << 24), or null
+    int ntab;                // table entries staged in LDS by the pipelined product (0: none)
     const double2 *diag;     // split and indexed forms: diagonal
     int64_t m;
     double2 *v0, *v1;
@@ -87,11 +89,16 @@ __device__ __forceinline__ IdxPtr g_vals<IdxPtr>(const GMember &M) {
 // epilogue operands before it gathers x for the current row.  Returns this
 // thread's share of <x|y>.
 template <class VP, bool SPLIT>
-__device__ double2 g_spmv_pipe(const GMember &M, double2 c1, const double2 *__restrict__ x,
-                               double2 *y, int64_t l) {
+__device__ double2 g_spmv_pipe(const GMember &M, VP vals, const double2 *s_tab, double2 c1,
+                               const double2 *__restrict__ x, double2 *y, int64_t l) {
     using PV = PipeVal<VP>;
     using Hd = typename PV::type;
-    const VP vals = g_vals<VP>(M);
+    auto value_of = [&](Hd hv) {
+        if constexpr (PV::indexed) {
+            if (M.ntab) return s_tab[hv];
+        }
+        return PV::value(vals, hv);
+    };
     const int lig = threadIdx.x & (GP_G - 1);
     constexpr int64_t gpb = 256 / GP_G;
     const int64_t per_x = M.nblk / 8, xcd = l % 8, w = l / 8;
@@ -116,8 +123,7 @@ __device__ double2 g_spmv_pipe(const GMember &M, double2 c1, const double2 *__re
 #pragma unroll
     for (int u = 0; u < GP_U; ++u) {
         const int64_t j = b + lig + u * GP_G;
-        c[u] = j < end ? M.cols[j] : -1;
-        h[u] = j < end ? PV::load(vals, j) : PV::zero();
+        PV::fetch(M.cols, vals, j, j < end, c[u], h[u]);
     }
     while (__any(r < rend)) {
         const int64_t rn = r + stride;
@@ -137,23 +143,25 @@ __device__ double2 g_spmv_pipe(const GMember &M, double2 c1, const double2 *__re
 #pragma unroll
         for (int u = 0; u < GP_U; ++u) {
             const int64_t j = bn + lig + u * GP_G;
-            cn[u] = j < endn ? M.cols[j] : -1;
-            hn[u] = j < endn ? PV::load(vals, j) : PV::zero();
+            PV::fetch(M.cols, vals, j, j < endn, cn[u], hn[u]);
         }
         double sr = 0, si = 0;
         if (r < rend) {
 #pragma unroll
             for (int u = 0; u < GP_U; ++u) {
                 if (c[u] >= 0) {
-                    const auto a = PV::value(vals, h[u]);
+                    const auto a = value_of(h[u]);
                     const double2 xv = x[c[u]];
                     sr += (double)a.x * xv.x - (double)a.y * xv.y;
                     si += (double)a.x * xv.y + (double)a.y * xv.x;
                 }
             }
             for (int64_t j = b + lig + GP_U * GP_G; j < end; j += GP_G) {
-                const auto a = nt_load2(vals + j);
-                const double2 xv = x[nt_load(M.cols + j)];
+                int cj;
+                Hd hj;
+                PV::fetch(M.cols, vals, j, true, cj, hj);
+                const auto a = value_of(hj);
+                const double2 xv = x[cj];
                 sr += (double)a.x * xv.x - (double)a.y * xv.y;
                 si += (double)a.x * xv.y + (double)a.y * xv.x;
             }
@@ -198,7 +206,25 @@ __global__ __launch_bounds__(256) void k_gspmv(const GMember *mem, int nmem, dou
     const double2 *x = (st.steps & 1) ? M.v1 : M.v0;
     double2 *y = (st.steps & 1) ? M.v0 : M.v1;
     if (M.pipe) {
-        const double2 dot = block_sum_256(g_spmv_pipe<VP, SPLIT>(M, st.c1, x, y, bid - M.blk0));
+        // indexed forms: the member's value table in LDS, the packed words where they exist
+        extern __shared__ double2 s_tab[];
+        if constexpr (PipeVal<VP>::indexed) {
+            if (M.ntab) {
+                for (int i = threadIdx.x; i < M.ntab; i += 256) s_tab[i] = M.tab[i];
+                __syncthreads();
+            }
+        }
+        const int64_t l = bid - M.blk0;
+        double2 dot;
+        if constexpr (std::is_same<VP, IdxPtr>::value) {
+            if (M.pk)
+                dot = g_spmv_pipe<PkPtr, SPLIT>(M, PkPtr{M.pk, M.tab}, s_tab, st.c1, x, y, l);
+            else
+                dot = g_spmv_pipe<VP, SPLIT>(M, g_vals<VP>(M), s_tab, st.c1, x, y, l);
+        } else {
+            dot = g_spmv_pipe<VP, SPLIT>(M, g_vals<VP>(M), s_tab, st.c1, x, y, l);
+        }
+        dot = block_sum_256(dot);
         if (threadIdx.x == 0) partials[bid] = dot;
         return;
     }
@@ -353,6 +379,7 @@ struct edx_group {
     size_t vpartials_cap = 0;
     int64_t nblk = 0, nvblk = 0;
     int64_t pipe_nblk = 0;   // SpMV sub-grid of a large member (0: not yet known)
+    int lds_tab = 0;         // LDS entries of the SpMV launch: the largest staged member table
     ~edx_group() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -389,8 +416,8 @@ int group_step(edx_group *g, hipStream_t s) {
         hipLaunchKernelGGL((k_gspmv<const float2 *, true>), dim3((unsigned)g->nblk), dim3(256), 0,
                            s, g->d_mem, nmem, g->partials);
     else if (g->precision == EDX_C128I)
-        hipLaunchKernelGGL((k_gspmv<IdxPtr, true>), dim3((unsigned)g->nblk), dim3(256), 0, s,
-                           g->d_mem, nmem, g->partials);
+        hipLaunchKernelGGL((k_gspmv<IdxPtr, true>), dim3((unsigned)g->nblk), dim3(256),
+                           g->lds_tab * sizeof(double2), s, g->d_mem, nmem, g->partials);
     else
         hipLaunchKernelGGL((k_gspmv<const double2 *, false>), dim3((unsigned)g->nblk), dim3(256),
                            0, s, g->d_mem, nmem, g->partials);
@@ -449,6 +476,7 @@ int group_layout(edx_group *g) {
     }
     std::vector<GMember> hm(g->mems.size());
     int64_t blk = 0, vblk = 0;
+    int lds_tab = 0;
     for (size_t j = 0; j < g->mems.size(); ++j) {
         const GroupMem &m = g->mems[j];
         const edx_csr *h = m.h;
@@ -459,6 +487,7 @@ int group_layout(edx_group *g) {
         d.vals = g->precision == EDX_C64D ? (const void *)h->d_off_vals32 : (const void *)h->d_vals;
         d.idx = h->d_off_idx;
         d.tab = h->d_vtab;
+        d.pk = g->precision == EDX_C128I ? h->d_off_pk : nullptr;
         d.diag = split ? h->d_diag : nullptr;
         d.m = h->m;
         d.v0 = m.v;
@@ -467,6 +496,10 @@ int group_layout(edx_group *g) {
         d.beta = m.ab + g->cap;
         d.st = m.st;
         d.pipe = h->m >= G_PIPE_MIN_ROWS;
+        d.ntab = d.pipe && g->precision == EDX_C128I && h->nvtab <= LDS_TAB_MAX && lds_tab_enabled()
+                     ? (int)h->nvtab
+                     : 0;
+        lds_tab = std::max(lds_tab, d.ntab);
         if (d.pipe) {   // a resident sub-grid starting at a multiple of 8
             if (!g->pipe_nblk) g->pipe_nblk = pipe_subgrid(g->precision);
             blk = (blk + 7) / 8 * 8;
@@ -484,6 +517,7 @@ int group_layout(edx_group *g) {
     }
     g->nblk = blk;
     g->nvblk = vblk;
+    g->lds_tab = lds_tab;
     EDX_TRY(grow((void **)&g->d_mem, &g->d_mem_cap, std::max<size_t>(hm.size(), 1) * sizeof(GMember),
                  "group members"));
     EDX_TRY(grow((void **)&g->partials, &g->partials_cap, (size_t)std::max<int64_t>(blk, 1) * sizeof(double2),
