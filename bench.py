@@ -761,7 +761,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"c128i": "f64 (complex128): exact values through a table of the distinct "
-                           "off-diagonal values (uint16 index per nonzero), f64 diagonal",
+                           "off-diagonal values (value number per nonzero, packed with its "
+                           "column in one 32-bit word), f64 diagonal",
                   "c64d": "c64 off-diagonal values, f64 diagonal/vectors/accumulation/scalars",
                   "c64": "c64 values and vectors, f32 accumulation, f64 scalars",
                   "c128": "f64 (complex128)"}[main_form],
