@@ -44,7 +44,9 @@ enum edx_status {
  *             from a few hundred terms has a few dozen), complex128 vectors,
  *             FP64 accumulation.  Preparing it fails (EDX_ERR_CAPACITY) when
  *             the distinct off-diagonal values do not fit the 65536-slot
- *             table (in practice: more than a few tens of thousands). */
+ *             table (in practice: more than a few tens of thousands).  With
+ *             fewer than 2^24 columns and at most 256 values a packed copy
+ *             holds column and value number in one 32-bit word per nonzero. */
 enum edx_precision { EDX_C128 = 0, EDX_C64 = 1, EDX_C64D = 2, EDX_C128I = 3 };
 
 /* Term records, layout-compatible with T_tensor2 / T_tensor4
