@@ -315,6 +315,16 @@ int lanczos_deflate() {
     return s ? std::atoi(s) : 1;
 }
 
+// Form of the products with a square operator: the exact indexed form where
+// its distinct off-diagonal values fit the table (the same FP64 operator,
+// diagonal added apart, faster products), else complex128 CSR;
+// EDX_FSOLVER_C128=1 keeps complex128 CSR throughout.
+int exact_form(edx_csr *a) {
+    const char *s = std::getenv("EDX_FSOLVER_C128");
+    if (s && *s && *s != '0') return EDX_C128;
+    return edx_csr_prepare(a, EDX_C128I) == EDX_OK ? EDX_C128I : EDX_C128;
+}
+
 // ------------------------------------------------------------ ED driver
 
 int ed_driver() {   // src/ed_driver.f90:1-281
@@ -375,11 +385,11 @@ int ed_driver() {   // src/ed_driver.f90:1-281
         EDX_TRY(dev_alloc(tmp, ndim * nev, "eigenvectors"));
         int iters = 0, nconv = 0;
         if (solver == 2)
-            EDX_TRY(edx_trlan_eigs(H.a, EDX_C128, nev, std::max(cfg.ncv, 1), cfg.maxiter,
+            EDX_TRY(edx_trlan_eigs(H.a, exact_form(H.a), nev, std::max(cfg.ncv, 1), cfg.maxiter,
                                    cfg.eigval_tol, lanczos_seed(), lanczos_deflate(), nev, ev.data(),
                                    tmp.p, &iters, &nconv));
         else
-            EDX_TRY(edx_lanczos_eigs(H.a, EDX_C128, nev, cfg.maxiter, cfg.eigval_tol, lanczos_seed(),
+            EDX_TRY(edx_lanczos_eigs(H.a, exact_form(H.a), nev, cfg.maxiter, cfg.eigval_tol, lanczos_seed(),
                                      lanczos_deflate(), nev, ev.data(), tmp.p, &iters));
         for (int k = 0; k < neval; ++k) evals[k] = ev[k];
         if (nvector)
@@ -451,7 +461,7 @@ int xas_driver() {   // src/xas_driver.f90:1-166
         EDX_TRY(edx_spmv(D.a, EDX_C128, gs.p, phi.p));
         int neff = 0;
         double norm = 0.0;
-        EDX_TRY(edx_build_krylov(Hn.a, EDX_C128, phi.p, cfg.nkryl, &neff, alpha.data(), beta.data(),
+        EDX_TRY(edx_build_krylov(Hn.a, exact_form(Hn.a), phi.p, cfg.nkryl, &neff, alpha.data(), beta.data(),
                                  &norm));
         EDX_TRY(write_krylov("xas_poles." + std::to_string(igs), neff, alpha.data(), beta.data(),
                              norm, eig));
@@ -504,12 +514,12 @@ int rixs_driver() {   // src/rixs_driver.f90:1-246
                                 EDX_C128, &A.a));
         int iters = 0;
         double res = 0.0;
-        EDX_TRY(edx_minres(A.a, EDX_C128, phi.p, x.p, cfg.linsys_tol, cfg.linsys_max, &iters, &res));
+        EDX_TRY(edx_minres(A.a, exact_form(A.a), phi.p, x.p, cfg.linsys_tol, cfg.linsys_max, &iters, &res));
         EDX_HIP(hipMemset(phif.p, 0, ndim_f * sizeof(double2)));
         EDX_TRY(edx_spmv(Df.a, EDX_C128, x.p, phif.p));
         int neff = 0;
         double norm = 0.0;
-        EDX_TRY(edx_build_krylov(Hf.a, EDX_C128, phif.p, cfg.nkryl, &neff, alpha.data(), beta.data(),
+        EDX_TRY(edx_build_krylov(Hf.a, exact_form(Hf.a), phif.p, cfg.nkryl, &neff, alpha.data(), beta.data(),
                                  &norm));
         EDX_TRY(write_krylov("rixs_poles." + std::to_string(igs), neff, alpha.data(), beta.data(),
                              norm, eig));

@@ -319,16 +319,21 @@ def test_cf_kernel_matches_oracle(core):
 
 # ------------------------------------------------------- drop-in end to end
 
-def test_fsolver_pipeline_matches_oracle(core, tmp_path):
+def test_fsolver_pipeline_matches_oracle(core, tmp_path, monkeypatch):
     """ed -> xas -> rixs -> opavg through the files, each output checked."""
     from edrixs_amd import iostream, solvers, spectra
     m = H.small_siam(nbath=2, seed=11)            # norb 12, N 6: dims 924 / 6*792
     wd = str(tmp_path)
+    # the drivers' products in complex128 CSR instead of the exact indexed form
+    monkeypatch.setenv("EDX_FSOLVER_C128", "1")
+    evals128, _ = solvers.ed_siam(m, wd, neval=2, nvector=2, eigval_tol=1e-12, min_ndim=10)
+    monkeypatch.delenv("EDX_FSOLVER_C128")
     evals, dm = solvers.ed_siam(m, wd, neval=2, nvector=2, eigval_tol=1e-12, min_ndim=10)
     fi = H.fock_words(m.norb, m.nocc)
     Hd = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
     e, V = np.linalg.eigh(Hd)
     np.testing.assert_allclose(evals, e[:2], atol=2e-10)        # f20.10 text + solver
+    np.testing.assert_allclose(evals128, e[:2], atol=2e-10)
     for k in range(2):
         np.testing.assert_allclose(np.trace(dm[k]).real, m.nocc, atol=1e-8)
     # XAS with the ground state written by the ED step
