@@ -81,7 +81,8 @@ int tune_spmv(edx_csr *a, int precision) {
     }
     prof_saved = g_prof;   // tuning launches stay out of the live profile
     g_prof = false;
-    for (int G : {4, 8, 16, 32, 64, 143, 144, 184}) {
+    for (int G : {4, 8, 16, 32, 64, 143, 144, 145, 184}) {
+        if (G == 145 && !(precision == EDX_C128I && a->d_off_pk)) continue;
         // the epilogue of a Krylov step: y = A x + z, with the dot <d|y>
         SpmvEpi e{};
         e.y = (double2 *)y;
@@ -213,8 +214,8 @@ int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes) {
     EDX_CHECK(set == 0 || set == 4 || set == 8 || set == 16 || set == 32 || set == 64 ||
                   edx::is_pipe_code(set),
               EDX_ERR_ARG,
-              "SpMV schedule must be 4, 8, 16, 32 or 64 lanes per row, or the pipelined 143, 144 "
-              "or 184 (got %d)", set);
+              "SpMV schedule must be 4, 8, 16, 32 or 64 lanes per row, or the pipelined 143, 144, "
+              "145 (packed indexed form) or 184 (got %d)", set);
     int *slot = lanes_slot(a, precision);
     if (set) *slot = set;
     if (lanes) *lanes = *slot;

@@ -220,8 +220,8 @@ struct PipeVal<PkPtr> {
 // gathers of the previous.  Lane l of a row holds its nonzeros l, l + G, ...,
 // l + (U-1) G (index loads through the cache); nonzeros beyond U * G in a row
 // are read in place.
-template <int G, int U, class VP, class XT, class ACC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_spmv_pipe(int64_t m, const int64_t *__restrict__ rowptr,
+template <int G, int U, class VP, class XT, class ACC, bool SPLIT, int WPE = 1>
+__global__ __launch_bounds__(256, WPE) void k_spmv_pipe(int64_t m, const int64_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ cols, VP vals,
                                                    const XT *__restrict__ x,
                                                    const double2 *__restrict__ diag, SpmvEpi e) {
@@ -517,8 +517,10 @@ int spmv_blocks(const edx_csr *a, int G) {
 
 // Pipelined schedules (k_spmv_pipe), selected through the lanes setting:
 // code 100 + 10 G + B is G lanes per row on a grid of B workgroups per CU.
-constexpr int PIPE_MAX_BPC = 4;
-bool is_pipe_code(int code) { return code == 143 || code == 144 || code == 184; }
+// 145 (packed indexed form only): the kernel built for five resident
+// workgroups per CU, which its 96 registers allow without spills.
+constexpr int PIPE_MAX_BPC = 5;
+bool is_pipe_code(int code) { return code == 143 || code == 144 || code == 145 || code == 184; }
 int pipe_blocks(int bpc) { return (num_cus() * bpc + 7) / 8 * 8; }
 
 // Length of a partials buffer that serves every single-vector SpMV launch of
@@ -594,7 +596,14 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
                                dim3(nb), dim3(256), 0, stream(), a->m, a->d_rowptr, a->d_cols,     \
                                (const float2 *)a->d_vals32, (const float2 *)x, nullptr, e);        \
         break;
-        switch (G) {
+        if (G == 145) {
+            if (!(precision == EDX_C128I && a->d_off_pk))
+                return set_error(EDX_ERR_ARG, "SpMV schedule 145 needs the packed indexed form");
+            hipLaunchKernelGGL((k_spmv_pipe<4, 8, PkPtr, double2, double, true, 5>), dim3(nb),
+                               dim3(256), e.lds_tab * sizeof(double2), stream(), a->m,
+                               a->d_off_rowptr, nullptr, PkPtr{a->d_off_pk, a->d_vtab},
+                               (const double2 *)x, a->d_diag, e);
+        } else switch (G) {
             EDX_PIPE_CASE(143, 4, 8)
             EDX_PIPE_CASE(144, 4, 8)
             EDX_PIPE_CASE(184, 8, 4)

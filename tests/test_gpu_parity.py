@@ -1210,7 +1210,34 @@ def test_pipelined_spmv_schedules_match_oracle(core, code):
     np.testing.assert_allclose(a2, a1, rtol=1e-12)
     np.testing.assert_allclose(b2, b1, rtol=1e-12, atol=1e-12 * np.abs(a1).max())
     with pytest.raises(Exception, match="SpMV schedule"):
-        A.spmv_lanes("c128", set=145)
+        A.spmv_lanes("c128", set=146)
+
+
+def test_packed_indexed_schedule_145(core):
+    # the pipelined product built for five workgroups per CU, which only the
+    # packed indexed form (column | value number in one word) may take
+    m = H.small_siam(nbath=4, seed=2)
+    fn = H.fock_words(m.norb, m.nocc + 1)
+    bn = core.Basis.from_words(fn, m.norb)
+    A = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    R = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    rng = np.random.default_rng(31)
+    x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    y0 = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    ref = o.matvec(R, x, y0)
+    scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
+    A.prepare("c128i")
+    A.spmv_lanes("c128i", set=145)
+    y = A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128i").cpu().numpy()
+    assert np.abs(y - ref).max() < 1e-14 * scale
+    a1, b1, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+    A.spmv_lanes("c128i", set=144)
+    a2, b2, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+    np.testing.assert_allclose(a1, a2, rtol=1e-12)
+    np.testing.assert_allclose(b1, b2, rtol=1e-12, atol=1e-12 * np.abs(a2).max())
+    A.spmv_lanes("c128", set=145)
+    with pytest.raises(Exception, match="packed indexed form"):
+        A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128")
 
 
 @pytest.mark.parametrize("precision", ["c128", "c128i", "c64d"])
