@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t m, const int64_t *__restri
 }
 
 // What a lane of the pipelined SpMV keeps per prefetched nonzero: the value
-// itself, or for the indexed form its 16-bit number (looked up when used).
+// itself, or for the indexed forms its value number (looked up when used).
 // fetch: the column and the entry of nonzero j when ok (else column -1).
 // indexed: the value comes from a table of distinct values (the kernels may
 // stage it in LDS).
