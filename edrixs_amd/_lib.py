@@ -95,6 +95,8 @@ _SIGS = {
     "edx_set_corehole_layout": (_I, [_I, ctypes.POINTER(_I)]),
     "edx_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,
                               ctypes.POINTER(_I)]),
+    "edx_lanczos_ground": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _P, _P,
+                                ctypes.POINTER(_I)]),
     "edx_trlan_eigs": (_I, [_P, _I, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,
                             ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "edx_lanczos_refine": (_I, [_P, _I, _I, _I, _D, _I, _P, _I, _P, _P, ctypes.POINTER(_I)]),

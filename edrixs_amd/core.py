@@ -14,6 +14,7 @@ The functions mirror the Fortran routines they replace:
 ``build_transop_f``         ``build_transop_f``  src/ham.f90:915-1115
 ``CSRMatrix.spmv``          ``matvec_csr``       src/spmv.f90:73-103 (y += A x)
 ``lanczos_eigs``            ``diag_ham_lanczos`` src/lanczos.f90:6-167
+``lanczos_ground``          ``diag_ham_lanczos`` src/lanczos.f90:6-167 (two passes, no basis)
 ``block_lanczos_eigs``      ``diag_ham_lanczos`` src/lanczos.f90:6-167 (8-vector blocks)
 ``trlan_eigs``              ``diag_ham_arpack``  src/arpack.f90:2-112 (thick restart)
 ``build_krylov``            ``build_krylov_mp``  src/lanczos.f90:210-303
@@ -293,6 +294,22 @@ def lanczos_eigs(H, nev, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None
     return ev, vecs[:nvec], it.value
 
 
+def lanczos_ground(H, nev=1, maxiter=500, tol=1e-10, seed=1, nvec=None, precision="c128"):
+    """Lowest nev eigenpairs by the reference's two-pass Lanczos
+    (``diag_ham_lanczos`` src/lanczos.f90:6-167): no stored basis, no
+    reorthogonalization, Ritz vectors from a second run of the recurrence.
+    One copy of a degenerate level.  Returns (evals, evecs (nvec x n device
+    tensor), steps)."""
+    T = torch()
+    nvec = nev if nvec is None else nvec
+    ev = np.zeros(nev)
+    vecs = T.zeros((max(nvec, 1), H.n), dtype=T.complex128, device=device())
+    it = ctypes.c_int()
+    _lib.call("edx_lanczos_ground", H._h, _prec(precision), nev, maxiter, float(tol), int(seed),
+              nvec, _ptr(ev), _dptr(vecs), ctypes.byref(it))
+    return ev, vecs[:nvec], it.value
+
+
 def trlan_eigs(H, nev, ncv=None, maxiter=500, tol=1e-10, seed=1, deflate=True, nvec=None,
                precision="c128"):
     """Lowest nev eigenpairs by thick-restart Lanczos (the ARPACK path,
@@ -343,8 +360,12 @@ def ground_state(H, nev=1, precision="c128", maxiter=500, tol=1e-10, seed=1, def
     """Lowest eigenpairs.  With precision 'c64' or 'c64d' the Lanczos products
     round values to complex64 and the result is finished by an FP64 Lanczos
     started from those eigenvectors; 'c128' and 'c128i' are exact and need no
-    refinement.  Returns (evals, evecs, iterations)."""
+    refinement.  One state without deflation takes the reference's two-pass
+    Lanczos (no stored basis); several states, or deflated restarts, the
+    fully reorthogonalized solver.  Returns (evals, evecs, iterations)."""
     if _prec(precision) in (EDX_C128, EDX_C128I):   # exact products: no refinement
+        if nev == 1 and not deflate:
+            return lanczos_ground(H, 1, maxiter, tol, seed, 1, precision)
         return lanczos_eigs(H, nev, maxiter, tol, seed, deflate, nev, precision)
     ev, vecs, it = lanczos_eigs(H, nev, maxiter, max(tol, 1e-6), seed, deflate, nev, precision)
     ev2, vecs2, it2 = lanczos_refine(H, vecs, nev, refine_maxiter, tol, nev, "c128")

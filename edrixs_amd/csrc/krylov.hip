@@ -582,6 +582,126 @@ int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double t
     return EDX_OK;
 }
 
+// The reference's two-pass procedure (diag_ham_lanczos, src/lanczos.f90:6-167)
+// without a stored basis: pass 1 runs the three-term recurrence with no
+// reorthogonalization from the seeded start vector, keeps alpha and beta, and
+// every 10 steps (and at the last) tests the nev lowest Ritz pairs of the
+// tridiagonal matrix by their residual estimates |beta_m y_{m,k}| < tol;
+// pass 2 runs the same recurrence again from the same start and accumulates
+// x_k = sum_j y_{j,k} v_j.  Three vectors of n plus nvec outputs; one copy of
+// each level (the Krylov space of one vector holds one).
+int lanczos_rerun_impl(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                       uint64_t seed, int nvec, double *evals, void *d_evecs, int *niter_out) {
+    EDX_TRY(check_square(h, precision));
+    EDX_CHECK(nev >= 1 && maxiter >= 1 && evals && nvec >= 0 && nvec <= nev &&
+                  (nvec == 0 || d_evecs),
+              EDX_ERR_ARG, "bad argument (nev=%d maxiter=%d nvec=%d)", nev, maxiter, nvec);
+    const int64_t n = h->n;
+    if (nev > n) nev = (int)n;
+    if (nvec > nev) nvec = nev;
+    if (maxiter > n) maxiter = (int)n;
+    const int nbv = vec_blocks(n);
+    DevBuf vb, x32, part;
+    EDX_TRY(vb.alloc((size_t)n * 3 * sizeof(double2), "lanczos vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "lanczos x32"));
+    EDX_TRY(part.alloc(((size_t)nbv + 64) * sizeof(double2), "partials"));
+    Lanczos L{h, precision, n, nullptr, n, 0, x32.as<float2>(), nullptr, part.as<double2>(), nullptr,
+              nbv};
+    hipStream_t s = edx::stream();
+    double2 *buf[3] = {vb.as<double2>(), vb.as<double2>() + n, vb.as<double2>() + 2 * n};
+    double2 *X = (double2 *)d_evecs;
+    // v_1: the seeded start vector, normalised
+    auto start = [&](double2 *v) -> int {
+        hipLaunchKernelGGL(k_random_real, dim3(nbv), dim3(VB), 0, s, n, seed, v);
+        EDX_LAUNCH_CHECK();
+        double2 nn;
+        EDX_TRY(host_dot(L, v, v, &nn));
+        EDX_CHECK(nn.x > 0, EDX_ERR_NUMERIC, "zero start vector");
+        hipLaunchKernelGGL(k_scale_real, dim3(nbv), dim3(VB), 0, s, n, v, 1.0 / sqrt(nn.x));
+        EDX_LAUNCH_CHECK();
+        return EDX_OK;
+    };
+    // w = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}; alpha from the device when
+    // known < 0, else given
+    auto step = [&](double2 *vprev, double2 *vj, double2 *w, double bprev, double &alpha,
+                    bool measure) -> int {
+        SpmvEpi e{};
+        e.y = w;
+        EDX_TRY(apply(L.h, L.precision, vj, L.x32, e, nullptr, nullptr));
+        if (measure) {
+            double2 a;
+            EDX_TRY(host_dot(L, vj, w, &a));
+            alpha = a.x;
+        }
+        hipLaunchKernelGGL(k_axpby_sub, dim3(nbv), dim3(VB), 0, s, n, w, vj, alpha, vprev, bprev);
+        EDX_LAUNCH_CHECK();
+        return EDX_OK;
+    };
+    std::vector<double> al, be, ritz;
+    int m = 0;
+    {   // pass 1
+        EDX_TRY(start(buf[1]));
+        double2 *v[3] = {buf[0], buf[1], buf[2]};   // v[0] = v_{j-1}, v[1] = v_j, v[2] = w
+        for (int j = 0; j < maxiter; ++j) {
+            double alpha = 0.0;
+            EDX_TRY(step(j > 0 ? v[0] : nullptr, v[1], v[2], j > 0 ? be[j - 1] : 0.0, alpha, true));
+            double2 nb2;
+            EDX_TRY(host_dot(L, v[2], v[2], &nb2));
+            const double beta = sqrt(fabs(nb2.x));
+            al.push_back(alpha);
+            be.push_back(beta);
+            m = j + 1;
+            const bool last = beta < 1e-10 || m == maxiter;
+            if (last || (m >= nev && m % 10 == 0)) {
+                std::vector<double> wv(m), zl(m);
+                EDX_CHECK(edx::tridiag_ql(m, al.data(), be.data(), wv.data(), nullptr, zl.data()) == 0,
+                          EDX_ERR_NUMERIC, "tridiagonal eigensolver did not converge");
+                int nconv = 0;
+                for (int k = 0; k < std::min(nev, m); ++k)
+                    if (fabs(beta * zl[k]) < tol) ++nconv;
+                if (nconv >= std::min(nev, m) || last) break;
+            }
+            // v_{j+1} = w / beta over v_{j-1}, which the next step no longer needs
+            hipLaunchKernelGGL(k_copy_div, dim3(nbv), dim3(VB), 0, s, n, v[2], beta, v[0]);
+            EDX_LAUNCH_CHECK();
+            std::swap(v[0], v[1]);
+        }
+    }
+    ritz.assign(m, 0.0);
+    std::vector<double> S((size_t)m * m);   // row k: eigenvector k of the tridiagonal matrix
+    EDX_CHECK(edx::tridiag_ql(m, al.data(), be.data(), ritz.data(), S.data(), nullptr) == 0,
+              EDX_ERR_NUMERIC, "tridiagonal eigensolver did not converge");
+    for (int k = 0; k < nev; ++k) evals[k] = k < m ? ritz[k] : 0.0;
+    if (nvec > 0) {   // pass 2
+        EDX_HIP(hipMemsetAsync(X, 0, (size_t)n * nvec * sizeof(double2), s));
+        EDX_TRY(start(buf[1]));
+        double2 *v[3] = {buf[0], buf[1], buf[2]};   // v[0] = v_{j-1}, v[1] = v_j, v[2] = w
+        for (int j = 0; j < m; ++j) {
+            for (int k = 0; k < nvec && k < m; ++k)   // x_k += y_{j,k} v_j
+                hipLaunchKernelGGL(k_axpby_sub, dim3(nbv), dim3(VB), 0, s, n, X + (int64_t)k * n, v[1],
+                                   -S[(size_t)k * m + j], nullptr, 0.0);
+            EDX_LAUNCH_CHECK();
+            if (j + 1 == m) break;
+            double alpha = al[j];
+            EDX_TRY(step(j > 0 ? v[0] : nullptr, v[1], v[2], j > 0 ? be[j - 1] : 0.0, alpha, false));
+            hipLaunchKernelGGL(k_copy_div, dim3(nbv), dim3(VB), 0, s, n, v[2], be[j], v[0]);
+            EDX_LAUNCH_CHECK();
+            std::swap(v[0], v[1]);   // v_j becomes v_{j-1}, the new vector v_{j+1} sits in v[1]
+        }
+        for (int k = 0; k < nvec && k < m; ++k) {   // unit norm
+            double2 nn;
+            EDX_TRY(host_dot(L, X + (int64_t)k * n, X + (int64_t)k * n, &nn));
+            if (nn.x > 0)
+                hipLaunchKernelGGL(k_scale_real, dim3(nbv), dim3(VB), 0, s, n, X + (int64_t)k * n,
+                                   1.0 / sqrt(nn.x));
+        }
+        EDX_LAUNCH_CHECK();
+    }
+    EDX_HIP(hipStreamSynchronize(s));
+    if (niter_out) *niter_out = m;
+    return EDX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -591,6 +711,11 @@ int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, doub
                      int *niter_out) {
     return lanczos_impl(h, precision, nev, maxiter, tol, seed, deflate, nvec, evals, d_evecs,
                         niter_out, nullptr, 0);
+}
+
+int edx_lanczos_ground(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                       uint64_t seed, int nvec, double *evals, void *d_evecs, int *niter_out) {
+    return lanczos_rerun_impl(h, precision, nev, maxiter, tol, seed, nvec, evals, d_evecs, niter_out);
 }
 
 int edx_trlan_eigs(const edx_csr *h, int precision, int nev, int ncv, int maxiter, double tol,

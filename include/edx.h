@@ -207,6 +207,17 @@ void edx_mf_free(edx_mfop *op);
 int edx_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxiter, double tol,
                      uint64_t seed, int deflate, int nvec, double *evals, void *d_evecs,
                      int *niter_out);
+/* Lowest nev eigenpairs by the reference's two-pass procedure
+ * (diag_ham_lanczos, src/lanczos.f90:6-167): the three-term recurrence with
+ * no reorthogonalization from the same seeded start vector as
+ * edx_lanczos_eigs, tested every 10 steps (converged when the nev lowest
+ * Ritz residual estimates |beta_m y_m| are below tol), then run again from
+ * the same start to accumulate the Ritz vectors.  Keeps three vectors of n
+ * instead of the basis; finds one copy of a degenerate level (use
+ * edx_lanczos_eigs with deflate for multiplets).  evals[nev]; d_evecs n x nvec
+ * complex128 column-major (NULL when nvec == 0); niter_out: steps of pass 1. */
+int edx_lanczos_ground(const edx_csr *h, int precision, int nev, int maxiter, double tol,
+                       uint64_t seed, int nvec, double *evals, void *d_evecs, int *niter_out);
 /* Same solver started from the normalised sum of nstart given vectors
  * (d_start, n x nstart column-major) instead of a random vector, without
  * deflated restarts: the FP64 refinement that finishes a ground state found

@@ -1024,6 +1024,36 @@ def test_block_lanczos_matches_single_vector_lanczos(core):
         core.block_lanczos_eigs(tiny, 1)
 
 
+@pytest.mark.parametrize("precision", ["c128", "c128i"])
+def test_two_pass_lanczos_ground_state(core, precision):
+    # the reference's procedure (no basis, no reorthogonalization, Ritz vector
+    # from a second run of the recurrence) against dense eigh and against the
+    # fully reorthogonalized solver from the same start vector
+    m = H.small_siam(nbath=2, seed=5)          # dim 924
+    fi = H.fock_words(m.norb, m.nocc)
+    A = core.build_ham_i(core.Basis.from_words(fi, m.norb), m.hop_i, m.coul_i)
+    Hd = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    e, V = np.linalg.eigh(Hd)
+    ev, vecs, steps = core.lanczos_ground(A, 1, maxiter=600, tol=1e-10, seed=1, precision=precision)
+    assert abs(ev[0] - e[0]) < 1e-10 and 10 <= steps < 600
+    x = vecs[0].cpu().numpy()
+    assert abs(abs(np.vdot(V[:, 0], x)) - 1.0) < 1e-9
+    assert np.linalg.norm(Hd @ x - ev[0] * x) < 1e-7
+    evn, _, _ = core.lanczos_ground(A, 1, maxiter=600, tol=1e-10, seed=1, nvec=0, precision=precision)
+    assert evn[0] == ev[0]
+    # 12,870 rows: same energy and state as the reorthogonalized solver
+    m2 = H.small_siam(nbath=6, seed=7)
+    f2 = H.fock_words(m2.norb, m2.nocc)
+    B = core.build_ham_i(core.Basis.from_words(f2, m2.norb), m2.hop_i, m2.coul_i)
+    e1, v1, _ = core.lanczos_eigs(B, 1, maxiter=800, tol=1e-11, seed=2, deflate=False,
+                                  precision=precision)
+    e2, v2, _ = core.ground_state(B, 1, precision=precision, maxiter=800, tol=1e-11, seed=2)
+    assert abs(e1[0] - e2[0]) < 1e-10
+    assert abs(abs(complex((v1[0].conj() * v2[0]).sum().item())) - 1.0) < 1e-9
+    y = B.matvec(v2[0].contiguous())
+    assert float((y - e2[0] * v2[0]).norm()) < 1e-7
+
+
 def test_trlan_matches_dense_many_states(core):
     m = H.small_siam(nbath=2, seed=5)          # dim 924
     fi = H.fock_words(m.norb, m.nocc)
