@@ -1,5 +1,6 @@
 """Host-side logic that needs no device: serving request generation and
-reports, the bench's reading of measured traffic, precision names."""
+reports, the bench's reading of measured traffic and its one-GPU guard,
+precision names."""
 import math
 import os
 import sys
@@ -66,3 +67,10 @@ def test_precision_names():
     assert core._prec("c64") == 1 and core._prec("mixed") == 2 and core._prec("indexed") == 3
     with pytest.raises(ValueError):
         core._prec("fp16")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr and not r.stdout.strip()
