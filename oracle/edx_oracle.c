@@ -15,6 +15,10 @@
  *                         with the sorted insert / duplicate sum of
  *                         src/m_types.f90:59-121 (init_col, insert_into_row)
  *   orc_matvec_csr        src/spmv.f90:73-103 (y += A x)
+ *
+ * The builds and products split their rows over OpenMP threads
+ * (orc_num_threads); each row is computed as the reference computes it, so
+ * the results are the same for any thread count.
  *   orc_build_krylov      src/lanczos.f90:210-303 (build_krylov_mp)
  *   orc_lanczos           src/lanczos.f90:6-167 (diag_ham_lanczos), start vector
  *                         supplied by the caller instead of random_number
@@ -28,6 +32,7 @@
  * (0-based global columns), vals double[2*nnz] (re, im interleaved).
  */
 #include <complex.h>
+#include <omp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -129,6 +134,82 @@ static void acc_push_row(orc_csr_acc *a, const orc_row *r)
  * diagonal if ham_sign<0 and with_omega), and the column lookup follow those lines.
  * Returns the CSR through malloc'd arrays that the caller frees with orc_free.
  */
+typedef struct {
+    int rowkind, colkind, nv, nc;
+    int64_t nleft, nright;
+    const int64_t *fock_left, *fock_right;
+    int64_t nhopp, ncoul;
+    const orc_t2 *hop;
+    const orc_t4 *coul;
+    double omega_re, omega_im, ham_sign;
+    int with_omega;
+    int64_t fock_core[64], full_core;
+} orc_build_args;
+
+/* Row r of the build into row (cleared first). */
+static void build_row(const orc_build_args *A, int64_t r, orc_row *row)
+{
+    const int nv = A->nv, nc = A->nc;
+    const int64_t nleft = A->nleft, nright = A->nright;
+    row->n = 0;
+    int64_t base;
+    if (A->rowkind == ROW_PLAIN) base = A->fock_left[r];
+    else if (A->rowkind == ROW_COREHOLE) {
+        int64_t core_indx = r / nleft, val_indx = r % nleft;
+        base = A->fock_core[core_indx] * ((int64_t)1 << nv) + A->fock_left[val_indx];
+    } else base = A->full_core * ((int64_t)1 << nv) + A->fock_left[r];
+
+    for (int64_t t = 0; t < A->nhopp + A->ncoul; ++t) {
+        int ops[4]; char kinds[4]; int nop; double vre, vim;
+        if (t < A->nhopp) {
+            ops[0] = A->hop[t].ind1; kinds[0] = '-';
+            ops[1] = A->hop[t].ind2; kinds[1] = '+';
+            nop = 2; vre = A->hop[t].re; vim = A->hop[t].im;
+        } else {
+            const orc_t4 *c = &A->coul[t - A->nhopp];
+            ops[0] = c->ind1; kinds[0] = '-';
+            ops[1] = c->ind2; kinds[1] = '-';
+            ops[2] = c->ind3; kinds[2] = '+';
+            ops[3] = c->ind4; kinds[3] = '+';
+            nop = 4; vre = c->re; vim = c->im;
+        }
+        int64_t old = base, nw = 0; int tot_sign = 1, sgn, ok = 1;
+        for (int q = 0; q < nop; ++q) {
+            int occ = (old >> (ops[q] - 1)) & 1;
+            if ((kinds[q] == '-' && !occ) || (kinds[q] == '+' && occ)) { ok = 0; break; }
+            orc_make_newfock(kinds[q], ops[q], old, &nw, &sgn);
+            tot_sign *= sgn;
+            old = nw;
+        }
+        if (!ok) continue;
+        int64_t jcfg;
+        if (A->colkind == COL_PLAIN) {
+            jcfg = orc_binary_search(nright, A->fock_right, nw);
+        } else {
+            int64_t new_core = nw / ((int64_t)1 << nv);
+            int64_t new_val = nw - new_core * ((int64_t)1 << nv);
+            if (A->colkind == COL_VALONLY) {
+                jcfg = orc_binary_search(nright, A->fock_right, new_val);
+            } else {
+                int64_t k = orc_binary_search(nc, A->fock_core, new_core);
+                int64_t l = orc_binary_search(nright, A->fock_right, new_val);
+                jcfg = (k < 0 || l < 0) ? -1 : k * nright + l;
+            }
+        }
+        if (jcfg < 0) continue;
+        /* val * tot_sign * ham_sign, evaluated left to right on each part */
+        double re = vre * (double)tot_sign * A->ham_sign;
+        double im = vim * (double)tot_sign * A->ham_sign;
+        row_insert(row, (int32_t)jcfg, re, im);
+    }
+    if (A->with_omega && A->ham_sign < 0) /* src/ham.f90:156-176 */
+        row_insert(row, (int32_t)r, A->omega_re, A->omega_im);
+}
+
+/* Threads the oracle's row-parallel loops use (OpenMP; rows are computed
+ * independently, so results do not depend on it). */
+int orc_num_threads(void) { return omp_get_max_threads(); }
+
 int orc_build(int rowkind, int64_t nleft, const int64_t *fock_left,
               int colkind, int64_t nright, const int64_t *fock_right,
               int num_val_orbs, int num_core_orbs,
@@ -138,81 +219,60 @@ int orc_build(int rowkind, int64_t nleft, const int64_t *fock_left,
               int64_t *m_out, int64_t *nnz_out,
               int64_t **rowptr_out, int32_t **cols_out, double **vals_out)
 {
-    const int nc = num_core_orbs, nv = num_val_orbs;
-    int64_t fock_core[64];
+    orc_build_args A;
+    memset(&A, 0, sizeof(A));
+    A.rowkind = rowkind; A.colkind = colkind;
+    A.nv = num_val_orbs; A.nc = num_core_orbs;
+    A.nleft = nleft; A.nright = nright;
+    A.fock_left = fock_left; A.fock_right = fock_right;
+    A.nhopp = nhopp; A.ncoul = ncoul; A.hop = hop; A.coul = coul;
+    A.omega_re = omega_re; A.omega_im = omega_im; A.ham_sign = ham_sign;
+    A.with_omega = with_omega;
+    const int nc = num_core_orbs;
     for (int i = 1; i <= nc; ++i)   /* src/ham.f90:419-421 */
-        fock_core[i - 1] = ((int64_t)1 << nc) - 1 - ((int64_t)1 << (nc - i));
-    const int64_t full_core = ((int64_t)1 << nc) - 1;
+        A.fock_core[i - 1] = ((int64_t)1 << nc) - 1 - ((int64_t)1 << (nc - i));
+    A.full_core = ((int64_t)1 << nc) - 1;
     int64_t m_total = (rowkind == ROW_COREHOLE) ? nleft * nc : nleft;
     if (row_end < 0 || row_end > m_total) row_end = m_total;
     if (row_begin < 0) row_begin = 0;
     int64_t m = row_end - row_begin;
+    if (m < 0) m = 0;
 
-    orc_csr_acc acc = {0};
-    acc.rowptr = (int64_t *)malloc((size_t)(m + 1) * sizeof(int64_t));
-    acc.rowptr[0] = 0;
-    orc_row row = {0};
-
-    for (int64_t r = row_begin; r < row_end; ++r) {
-        row.n = 0;
-        int64_t base;
-        if (rowkind == ROW_PLAIN) base = fock_left[r];
-        else if (rowkind == ROW_COREHOLE) {
-            int64_t core_indx = r / nleft, val_indx = r % nleft;
-            base = fock_core[core_indx] * ((int64_t)1 << nv) + fock_left[val_indx];
-        } else base = full_core * ((int64_t)1 << nv) + fock_left[r];
-
-        for (int64_t t = 0; t < nhopp + ncoul; ++t) {
-            int ops[4]; char kinds[4]; int nop; double vre, vim;
-            if (t < nhopp) {
-                ops[0] = hop[t].ind1; kinds[0] = '-';
-                ops[1] = hop[t].ind2; kinds[1] = '+';
-                nop = 2; vre = hop[t].re; vim = hop[t].im;
-            } else {
-                const orc_t4 *c = &coul[t - nhopp];
-                ops[0] = c->ind1; kinds[0] = '-';
-                ops[1] = c->ind2; kinds[1] = '-';
-                ops[2] = c->ind3; kinds[2] = '+';
-                ops[3] = c->ind4; kinds[3] = '+';
-                nop = 4; vre = c->re; vim = c->im;
-            }
-            int64_t old = base, nw = 0; int tot_sign = 1, sgn, ok = 1;
-            for (int q = 0; q < nop; ++q) {
-                int occ = (old >> (ops[q] - 1)) & 1;
-                if ((kinds[q] == '-' && !occ) || (kinds[q] == '+' && occ)) { ok = 0; break; }
-                orc_make_newfock(kinds[q], ops[q], old, &nw, &sgn);
-                tot_sign *= sgn;
-                old = nw;
-            }
-            if (!ok) continue;
-            int64_t jcfg;
-            if (colkind == COL_PLAIN) {
-                jcfg = orc_binary_search(nright, fock_right, nw);
-            } else {
-                int64_t new_core = nw / ((int64_t)1 << nv);
-                int64_t new_val = nw - new_core * ((int64_t)1 << nv);
-                if (colkind == COL_VALONLY) {
-                    jcfg = orc_binary_search(nright, fock_right, new_val);
-                } else {
-                    int64_t k = orc_binary_search(nc, fock_core, new_core);
-                    int64_t l = orc_binary_search(nright, fock_right, new_val);
-                    jcfg = (k < 0 || l < 0) ? -1 : k * nright + l;
-                }
-            }
-            if (jcfg < 0) continue;
-            /* val * tot_sign * ham_sign, evaluated left to right on each part */
-            double re = vre * (double)tot_sign * ham_sign;
-            double im = vim * (double)tot_sign * ham_sign;
-            row_insert(&row, (int32_t)jcfg, re, im);
+    /* contiguous row chunks, one per thread, joined in row order */
+    int nt = omp_get_max_threads();
+    if (nt > 1 && m / 2048 < nt) nt = (int)(m / 2048) > 1 ? (int)(m / 2048) : 1;
+    orc_csr_acc *accs = (orc_csr_acc *)calloc((size_t)nt, sizeof(orc_csr_acc));
+    int64_t *rowptr = (int64_t *)malloc((size_t)(m + 1) * sizeof(int64_t));
+    #pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num();
+        const int64_t b = row_begin + m * t / nt, e = row_begin + m * (t + 1) / nt;
+        orc_row row = {0};
+        for (int64_t r = b; r < e; ++r) {
+            build_row(&A, r, &row);
+            acc_push_row(&accs[t], &row);
+            rowptr[r - row_begin + 1] = row.n;
         }
-        if (with_omega && ham_sign < 0) /* src/ham.f90:156-176 */
-            row_insert(&row, (int32_t)r, omega_re, omega_im);
-        acc_push_row(&acc, &row);
-        acc.rowptr[r - row_begin + 1] = acc.nnz;
+        free(row.e);
     }
-    free(row.e);
-    *m_out = m; *nnz_out = acc.nnz;
-    *rowptr_out = acc.rowptr; *cols_out = acc.cols; *vals_out = acc.vals;
+    rowptr[0] = 0;
+    for (int64_t i = 0; i < m; ++i) rowptr[i + 1] += rowptr[i];
+    const int64_t nnz = rowptr[m];
+    int32_t *cols = (int32_t *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
+    double *vals = (double *)malloc((size_t)(nnz ? nnz : 1) * 2 * sizeof(double));
+    int64_t off = 0;
+    for (int t = 0; t < nt; ++t) {
+        if (accs[t].nnz) {
+            memcpy(cols + off, accs[t].cols, (size_t)accs[t].nnz * sizeof(int32_t));
+            memcpy(vals + 2 * off, accs[t].vals, (size_t)accs[t].nnz * 2 * sizeof(double));
+        }
+        off += accs[t].nnz;
+        free(accs[t].cols);
+        free(accs[t].vals);
+    }
+    free(accs);
+    *m_out = m; *nnz_out = nnz;
+    *rowptr_out = rowptr; *cols_out = cols; *vals_out = vals;
     return 0;
 }
 
@@ -224,6 +284,7 @@ void orc_free(void *p) { free(p); }
 void orc_matvec_csr(int64_t m, const int64_t *rowptr, const int32_t *cols,
                     const double *vals, const double *x, double *y)
 {
+    #pragma omp parallel for schedule(static) if (m >= 16384)
     for (int64_t i = 0; i < m; ++i) {
         int64_t b = rowptr[i], e = rowptr[i + 1];
         if (b >= e) continue;
