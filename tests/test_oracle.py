@@ -169,6 +169,32 @@ def test_row_ranges_concatenate():
     assert np.array_equal(np.concatenate([B1.vals, B2.vals]), A.vals)
 
 
+def test_thread_count_changes_nothing():
+    # the builds and products split rows over OpenMP threads; one thread and
+    # several give the same arrays bit for bit (H_n of 38,610 rows: 18 chunks)
+    import ctypes
+    import ctypes.util
+    gomp = ctypes.CDLL(ctypes.util.find_library("gomp") or "libgomp.so.1")
+    m = H.small_siam(nbath=5, seed=2)
+    fn = H.fock_words(m.norb, m.nocc + 1)
+    x = np.random.default_rng(3).normal(size=len(fn) * m.ncore) + 0.5j
+    orig = gomp.omp_get_max_threads()
+    out = []
+    try:
+        for nt in (1, 4):
+            gomp.omp_set_num_threads(nt)
+            assert o.lib().orc_num_threads() == nt
+            A = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n)
+            out.append((A, o.matvec(A, x)))
+    finally:
+        gomp.omp_set_num_threads(orig)
+    (A1, y1), (A4, y4) = out
+    assert A1.m == 38610
+    assert np.array_equal(A1.rowptr, A4.rowptr) and np.array_equal(A1.cols, A4.cols)
+    assert np.array_equal(A1.vals.view(np.float64), A4.vals.view(np.float64))
+    assert np.array_equal(y1.view(np.float64), y4.view(np.float64))
+
+
 # ----------------------------------------------------------------- solvers
 
 def _small_problem(seed=2, n=8, N=4):
