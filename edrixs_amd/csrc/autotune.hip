@@ -50,7 +50,8 @@ int tune_spmv(edx_csr *a, int precision) {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(tune_key(a, precision, 1));
-        if (it != g_tuned.end()) {
+        // schedule 145 needs the packed form, which a matrix of the same shape may lack
+        if (it != g_tuned.end() && (it->second != 145 || a->d_off_pk)) {
             *slot = it->second;
             return EDX_OK;
         }
