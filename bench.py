@@ -35,6 +35,9 @@ import os
 import sys
 import time
 
+T_START = time.monotonic()   # the wall-clock budget counts from process start
+DEADLINE = T_START + 480.0
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -55,6 +58,12 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--budget", type=float, default=480.0,
+                   help="wall-clock seconds for the whole run, counted from process start: "
+                        "warm-up is capped at one step (it holds the schedule autotuning), "
+                        "as many of the requested steps are timed as leave room for the "
+                        "comparison forms, the plain FP64 run, serving and the CPU baseline, "
+                        "and each of those runs only if its estimated time still fits")
     p.add_argument("--nbath", type=int, default=14)
     p.add_argument("--nkryl", type=int, default=300)
     p.add_argument("--nw", type=int, default=2000)
@@ -263,27 +272,56 @@ def plain_fp64(work, ref, pairs):
                      else "every pair timed")}
 
 
-def timed(work, precision, steps, warmup):
+def left():
+    """Seconds left of the run's wall-clock budget."""
+    return DEADLINE - time.monotonic()
+
+
+def timed(work, precision, steps, warmup, reserve=lambda t: 0.0):
+    """``warmup`` untimed steps, then up to ``steps`` timed ones.  The first
+    timed step always runs; a further one only if the budget left, less
+    ``reserve(t)`` for the work that follows, still holds one step at the
+    slowest step time t seen so far (warm-up included).  Returns the timed
+    seconds, the number of timed steps, the last step's output and the kernel
+    profile of the timed region."""
     import torch
     c = work.core
+    slowest = 0.0
     for i in range(warmup):
         t = time.perf_counter()
         work.step(precision)
-        log("%s warmup %d: %.2f s" % (precision, i, time.perf_counter() - t))
+        slowest = max(slowest, time.perf_counter() - t)
+        log("%s warmup %d: %.2f s" % (precision, i, slowest))
     torch.cuda.synchronize()
     c.profile(True)
     t0 = time.perf_counter()
     out = None
-    for i in range(steps):
+    done = 0
+    while done < steps:
+        if done > 0 and left() - reserve(slowest) < slowest:
+            log("%s: budget holds no further step (%.0f s left, step %.1f s, %.0f s reserved); "
+                "%d of %d steps timed" % (precision, left(), slowest, reserve(slowest), done, steps))
+            break
+        ts = time.perf_counter()
         out = work.step(precision)
-        log("%s step %d done at %.2f s  phases %s" % (precision, i, time.perf_counter() - t0,
+        slowest = max(slowest, time.perf_counter() - ts)
+        done += 1
+        log("%s step %d done at %.2f s  phases %s" % (precision, done - 1, time.perf_counter() - t0,
             {k: round(v, 3) for k, v in out["info"]["phases_s"].items()}))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     prof = {k: c.profile_read(precision, k) for k in c.KERNELS}
     prof["all"] = c.profile_read(precision)
     c.profile(False)
-    return dt, out, prof
+    return dt, done, out, prof
+
+
+# Wall-clock cost of the work after the timed steps, in units of one step of
+# the reported form (c128i) where it scales with it, else in seconds: one
+# step of each comparison form, the plain FP64 run with its default pairs,
+# the serving measurement and the CPU baseline sample.
+COST_STEPS = {"c128": 1.6, "c64d": 1.3, "c64": 1.3, "c128i": 1.0, "plain": 2.2}
+COST_SECONDS = {"serve": 45.0, "cpu": 30.0}
 
 
 def rel(a, b):
@@ -686,7 +724,9 @@ def probe(args):
 
 
 def main():
+    global DEADLINE
     args = parse()
+    DEADLINE = T_START + args.budget
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, and the "
               "multi-GPU layout is a design note, not a measured path (got --gpus %d)"
@@ -711,13 +751,33 @@ def main():
         return
     work = Workload(args)
     forms = args.forms.split(",")
-    results, times, profs = {}, {}, {}
-    for i, f in enumerate(forms):
-        # the first form's warm-up also warms the device and library for the others
-        dt, out, prof = timed(work, f, args.steps, args.warmup if i == 0 else 0)
-        results[f], times[f], profs[f] = out, dt, prof
-    nspec = 1 + 2 * args.n_in
     main_form = forms[0]
+    want_plain = "c128" in forms and args.plain_pairs > 0
+
+    def reserve(t):
+        r = sum(COST_STEPS.get(f, 1.6) for f in forms[1:]) * t
+        r += COST_STEPS["plain"] * t * (args.plain_pairs / 4.0) if want_plain else 0.0
+        r += COST_SECONDS["serve"] * (args.serve / 12.0) if args.serve > 0 else 0.0
+        r += 0.0 if args.no_cpu else COST_SECONDS["cpu"]
+        return r
+
+    results, times, nsteps, profs, skipped = {}, {}, {}, {}, {}
+    # the reported form first: its warm-up (at most one step, which holds the
+    # schedule autotuning) also warms the device and library for the others
+    warmup = min(args.warmup, 1)
+    dt, k, out, prof = timed(work, main_form, args.steps, warmup, reserve)
+    results[main_form], times[main_form], nsteps[main_form], profs[main_form] = out, dt, k, prof
+    t_step = dt / k
+    for f in forms[1:]:
+        need = COST_STEPS.get(f, 1.6) * t_step
+        if left() < need:
+            skipped[f] = "budget: %.0f s left, about %.0f s needed" % (left(), need)
+            log("skipping form %s (%s)" % (f, skipped[f]))
+            continue
+        dt, k, out, prof = timed(work, f, 1, 0)
+        results[f], times[f], nsteps[f], profs[f] = out, dt, k, prof
+    forms = [f for f in forms if f in results]
+    nspec = 1 + 2 * args.n_in
     t_main = times[main_form]
     # the dominant kernel: the SpMV/SpMM variant with the most kernel time
     kname = max(work.core.KERNELS, key=lambda k: profs[main_form][k][1])
@@ -733,8 +793,8 @@ def main():
             if pn:
                 kern[k] = {"launches": pn, "total_s": pms * 1e-3, "avg_us": pms / pn * 1e3,
                            "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
-        d = {"spectra_per_s": nspec * args.steps / times[f],
-             "ms_per_step": times[f] / args.steps * 1e3,
+        d = {"spectra_per_s": nspec * nsteps[f] / times[f],
+             "ms_per_step": times[f] / nsteps[f] * 1e3, "steps": nsteps[f],
              "phases_s": r["info"]["phases_s"],
              "spmv_kernels": kern,
              "lanczos_iters": r["info"]["lanczos_iters"],
@@ -753,12 +813,15 @@ def main():
         forms_out[f] = d
     line = {
         "metric": METRIC,
-        "value": nspec * args.steps / t_main,
+        "value": nspec * nsteps[main_form] / t_main,
         "unit": "spectra/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_main / args.steps * 1e3,
+        "steps": nsteps[main_form],
+        "steps_requested": args.steps,
+        "warmup": warmup,
+        "warmup_requested": args.warmup,
+        "budget_s": args.budget,
+        "ms_per_step": t_main / nsteps[main_form] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -791,19 +854,32 @@ def main():
         "forms": forms_out,
     }
     if "c128" in results:
-        line["speedup_vs_fp64"] = times["c128"] / t_main
-        if args.plain_pairs > 0:
+        line["speedup_vs_fp64"] = (times["c128"] / nsteps["c128"]) / (t_main / nsteps[main_form])
+    if not args.no_cpu:
+        need = COST_SECONDS["cpu"]
+        rows = args.cpu_rows if left() >= need else max(20000, int(args.cpu_rows * left() / need))
+        cb = cpu_baseline(work, results[main_form]["info"], rows)
+        line["cpu_baseline"] = cb
+        line["gpu_vs_cpu"] = line["value"] / cb["value"]
+    if args.serve > 0:
+        need = COST_SECONDS["serve"] * args.serve / 12.0
+        if left() >= need:
+            line["serving"] = serve(args)
+        else:
+            skipped["serving"] = "budget: %.0f s left, about %.0f s needed" % (left(), need)
+    if want_plain:
+        need = COST_STEPS["plain"] * t_step * args.plain_pairs / 4.0
+        if left() >= need:
             pl = plain_fp64(work, results["c128"], args.plain_pairs)
             line["plain_fp64"] = pl
             line["speedup_vs_plain_fp64"] = line["value"] / pl["spectra_per_s"]
+        else:
+            skipped["plain_fp64"] = "budget: %.0f s left, about %.0f s needed" % (left(), need)
     if args.exact and "c128" in results:
         line["accuracy_vs_exact_projection"] = exact_reference(work, results["c128"]["gs"], results)
-    if args.serve > 0:
-        line["serving"] = serve(args)
-    if not args.no_cpu:
-        cb = cpu_baseline(work, results[main_form]["info"], args.cpu_rows)
-        line["cpu_baseline"] = cb
-        line["gpu_vs_cpu"] = line["value"] / cb["value"]
+    if skipped:
+        line["skipped"] = skipped
+    line["wall_s"] = time.monotonic() - T_START
     print(json.dumps(line))
 
 
