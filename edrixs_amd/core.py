@@ -182,6 +182,15 @@ class CSRMatrix:
         _lib.call("edx_csr_spmv_lanes", self._h, _prec(precision), int(set), ctypes.byref(out))
         return out.value
 
+    def factored(self):
+        """The slot-factored form made by prepare("c128i") on a core-hole matrix
+        in the interleaved layout: (core slots, entries stored once per valence
+        row, entries stored per row), or None when absent."""
+        nc, hv, r = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("edx_csr_factored_info", self._h, ctypes.byref(nc), ctypes.byref(hv),
+                  ctypes.byref(r))
+        return (nc.value, hv.value, r.value) if nc.value else None
+
     def add_c64(self):
         _lib.call("edx_csr_add_c64", self._h)
         self.has_c64 = True

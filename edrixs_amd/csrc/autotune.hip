@@ -11,20 +11,93 @@
 // launches on scratch vectors, with the epilogue of a Krylov step (addend and
 // dot partials), since without it the ranking differs from the one inside the
 // solvers -- and the fastest is kept.  Choices are cached by matrix shape
-// (rows, columns, nonzeros), precision and K, so a matrix rebuilt with the
-// same structure reuses them.  Matrices below TUNE_MIN_ROWS keep the rules;
-// EDX_NO_TUNE=1 in the environment turns tuning off.
+// (rows, columns, nonzeros), precision, K and the storage forms present (the
+// packed indexed and slot-factored forms add candidates), so a matrix rebuilt
+// with the same structure reuses them.  Matrices below TUNE_MIN_ROWS keep the
+// rules; EDX_NO_TUNE=1 in the environment turns tuning off.
+//
+// The schedule fixes the order of every sum (lanes per row, dot partials per
+// workgroup), so two processes that picked different schedules for the same
+// matrix would round differently.  Choices therefore persist in a text file,
+// one line per choice keyed also by the device (architecture and CU count):
+// $EDX_TUNE_CACHE, else $XDG_CACHE_HOME/edrixs_amd/tune-v1.txt, else
+// $HOME/.cache/edrixs_amd/tune-v1.txt.  A process reads it on its first
+// lookup and appends what it tunes; the first line of a key wins, so every
+// later process runs the schedule the first one chose.  EDX_TUNE_CACHE=0 (or
+// empty) keeps choices in memory only.
 
 namespace {
 
 constexpr int TUNE_REPS = 5;
 
-using TuneKey = std::tuple<int64_t, int64_t, int64_t, int, int>;   // m, n, nnz, precision, K
+// m, n, nnz, precision, K, forms present
+using TuneKey = std::tuple<int64_t, int64_t, int64_t, int, int, int>;
 std::map<TuneKey, int> g_tuned;   // single vector: lanes; multi-RHS: rl * 256 + p
 std::mutex g_tune_mu;
+bool g_tune_loaded = false;
+std::string g_tune_path;          // empty: no file
+std::string g_tune_device;        // "<arch>:<CUs>"
+
+int forms_of(const edx_csr *a) {
+    return (a->d_off_pk ? 1 : 0) | (a->d_fr_rowptr ? 2 : 0);
+}
 
 TuneKey tune_key(const edx_csr *a, int precision, int K) {
-    return TuneKey{a->m, a->n, a->nnz, precision, K};
+    return TuneKey{a->m, a->n, a->nnz, precision, K, forms_of(a)};
+}
+
+std::string tune_cache_path() {
+    const char *p = getenv("EDX_TUNE_CACHE");
+    if (p) return (p[0] == 0 || (p[0] == '0' && p[1] == 0)) ? std::string() : std::string(p);
+    const char *x = getenv("XDG_CACHE_HOME");
+    if (x && x[0]) return std::string(x) + "/edrixs_amd/tune-v1.txt";
+    const char *h = getenv("HOME");
+    if (h && h[0]) return std::string(h) + "/.cache/edrixs_amd/tune-v1.txt";
+    return std::string();
+}
+
+// Reads the cache file once (caller holds g_tune_mu).
+void tune_load_locked() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, edx::current_device()) == hipSuccess)
+        g_tune_device = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount);
+    else
+        g_tune_device = "unknown:0";
+    for (char &c : g_tune_device)
+        if (c == ' ') c = '_';
+    g_tune_path = tune_cache_path();
+    if (g_tune_path.empty()) return;
+    FILE *f = fopen(g_tune_path.c_str(), "r");
+    if (!f) return;
+    char dev[256];
+    long long m, n, nnz;
+    int prec, K, forms, choice;
+    char line[512];
+    while (fgets(line, sizeof(line), f)) {
+        if (sscanf(line, "%255s %lld %lld %lld %d %d %d %d", dev, &m, &n, &nnz, &prec, &K, &forms,
+                   &choice) != 8)
+            continue;
+        if (g_tune_device != dev) continue;
+        g_tuned.emplace(TuneKey{m, n, nnz, prec, K, forms}, choice);   // first line wins
+    }
+    fclose(f);
+}
+
+// Records a new choice in memory and appends it to the cache file (caller
+// holds g_tune_mu).  A file that cannot be written leaves the choice in memory.
+void tune_store_locked(const TuneKey &k, int choice) {
+    if (!g_tuned.emplace(k, choice).second) return;
+    if (g_tune_path.empty()) return;
+    for (size_t i = 1; i < g_tune_path.size(); ++i)   // create the directories
+        if (g_tune_path[i] == '/') mkdir(g_tune_path.substr(0, i).c_str(), 0755);
+    FILE *f = fopen(g_tune_path.c_str(), "a");
+    if (!f) return;
+    fprintf(f, "%s %lld %lld %lld %d %d %d %d\n", g_tune_device.c_str(),
+            (long long)std::get<0>(k), (long long)std::get<1>(k), (long long)std::get<2>(k),
+            std::get<3>(k), std::get<4>(k), std::get<5>(k), choice);
+    fclose(f);
 }
 
 bool tuning_off() {
@@ -49,9 +122,9 @@ int tune_spmv(edx_csr *a, int precision) {
     if (a->m < TUNE_MIN_ROWS || tuning_off()) return EDX_OK;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
         auto it = g_tuned.find(tune_key(a, precision, 1));
-        // schedule 145 needs the packed form, which a matrix of the same shape may lack
-        if (it != g_tuned.end() && (it->second != 145 || a->d_off_pk)) {
+        if (it != g_tuned.end() && schedule_available(a, precision, it->second)) {
             *slot = it->second;
             return EDX_OK;
         }
@@ -82,8 +155,8 @@ int tune_spmv(edx_csr *a, int precision) {
     }
     prof_saved = g_prof;   // tuning launches stay out of the live profile
     g_prof = false;
-    for (int G : {4, 8, 16, 32, 64, 143, 144, 145, 184}) {
-        if (G == 145 && !(precision == EDX_C128I && a->d_off_pk)) continue;
+    for (int G : {4, 8, 16, 32, 64, 143, 144, 145, 184, FAC_CODE, FAC_CODE + 1, FAC_CODE + 2}) {
+        if (!schedule_available(a, precision, G)) continue;
         // the epilogue of a Krylov step: y = A x + z, with the dot <d|y>
         SpmvEpi e{};
         e.y = (double2 *)y;
@@ -107,11 +180,13 @@ int tune_spmv(edx_csr *a, int precision) {
             best_ms = ms;
         }
     }
-    *slot = best;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[tune_key(a, precision, 1)] = best;
+        tune_store_locked(tune_key(a, precision, 1), best);
+        // another thread may have stored the key first: take the stored choice
+        best = g_tuned[tune_key(a, precision, 1)];
     }
+    *slot = best;
 done:
     g_prof = prof_saved;
     for (hipEvent_t e : ev)
@@ -128,6 +203,7 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
     const TuneKey key = tune_key(a, precision, K);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
             *sched = BSched{it->second / 256, it->second % 256};
@@ -188,11 +264,13 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
             have = true;
         }
     }
-    *sched = best;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[key] = best.rl * 256 + best.p;
+        tune_store_locked(key, best.rl * 256 + best.p);
+        const int v = g_tuned[key];
+        best = BSched{v / 256, v % 256};
     }
+    *sched = best;
 done:
     g_prof = prof_saved;
     for (hipEvent_t e : ev)
@@ -213,10 +291,10 @@ int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes) {
     EDX_CHECK(precision >= EDX_C128 && precision <= EDX_C128I, EDX_ERR_ARG, "bad precision %d",
               precision);
     EDX_CHECK(set == 0 || set == 4 || set == 8 || set == 16 || set == 32 || set == 64 ||
-                  edx::is_pipe_code(set),
+                  edx::is_pipe_code(set) || edx::is_fac_code(set),
               EDX_ERR_ARG,
-              "SpMV schedule must be 4, 8, 16, 32 or 64 lanes per row, or the pipelined 143, 144, "
-              "145 (packed indexed form) or 184 (got %d)", set);
+              "SpMV schedule must be 4, 8, 16, 32 or 64 lanes per row, the pipelined 143, 144, "
+              "145 (packed indexed form) or 184, or 206-208 (slot-factored form) (got %d)", set);
     int *slot = lanes_slot(a, precision);
     if (set) *slot = set;
     if (lanes) *lanes = *slot;

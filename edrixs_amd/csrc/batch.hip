@@ -1119,10 +1119,15 @@ fail:
     return rc;
 }
 
+int csr_add_factored(edx_csr *a);
+
 int ensure_precision(edx_csr *a, int precision) {
     if (precision == EDX_C64) EDX_TRY(edx_csr_add_c64(a));
     else if (precision == EDX_C64D) EDX_TRY(csr_add_split(a));
-    else if (precision == EDX_C128I) EDX_TRY(csr_add_indexed(a));
+    else if (precision == EDX_C128I) {
+        EDX_TRY(csr_add_indexed(a));
+        EDX_TRY(csr_add_factored(a));
+    }
     else if (precision != EDX_C128) return set_error(EDX_ERR_ARG, "unknown precision %d", precision);
     if (a->m == a->n) EDX_TRY(tune_spmv(a, precision));
     return EDX_OK;

@@ -96,6 +96,18 @@ struct edx_csr {
     // index layout of the core-hole space on the rows and/or columns that are
     // in it (edx_corehole_layout); 0 for matrices without one
     int corehole_layout;
+    // core slots of a square core-hole matrix in the interleaved layout (row
+    // v * nc_slots + s), 0 otherwise
+    int nc_slots;
+    // slot-factored form (factor.hip): the off-diagonal entries shared by all
+    // slots of a valence row once per valence row (H_v: valence column | value
+    // number << 24), the rest per row (R: column | value number << 24)
+    int64_t *d_fv_rowptr;   // m / nc_slots + 1
+    uint32_t *d_fv_pk;
+    int64_t fv_nnz;
+    int64_t *d_fr_rowptr;   // m + 1
+    uint32_t *d_fr_pk;
+    int64_t fr_nnz;
 };
 
 // Lookup descriptor for a column space basis.

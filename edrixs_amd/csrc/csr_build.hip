@@ -448,6 +448,7 @@ int build_csr(BuildParams p, int64_t ncols, int precision, edx_csr **out) {
     }
     edx_csr *a = new edx_csr{};
     a->corehole_layout = p.inter;
+    if (RK == ROW_COREHOLE && CK == COL_COREHOLE && p.inter) a->nc_slots = p.nc;
     a->m = p.M;
     a->n = ncols;
     a->nnz = nnz;
@@ -638,6 +639,10 @@ void edx_csr_free(edx_csr *a) {
     edx::dfree(a->d_off_idx);
     edx::dfree(a->d_vtab);
     edx::dfree(a->d_off_pk);
+    edx::dfree(a->d_fv_rowptr);
+    edx::dfree(a->d_fv_pk);
+    edx::dfree(a->d_fr_rowptr);
+    edx::dfree(a->d_fr_pk);
     delete a;
 }
 

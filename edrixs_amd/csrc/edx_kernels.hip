@@ -1,7 +1,11 @@
 // Single HIP translation unit for all device code of libedx, so the
 // constant-memory binomial table and the shared kernels need no relocatable
 // device code.
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <string>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -20,6 +24,7 @@
 #include "krylov.hip"
 #include "mf.hip"
 #include "batch.hip"
+#include "factor.hip"
 #include "autotune.hip"
 #include "multishift.hip"
 #include "group.hip"

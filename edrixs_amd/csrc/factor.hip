@@ -1,0 +1,394 @@
+// Slot-factored form of a core-hole Hamiltonian (part of edx_kernels.hip).
+//
+// In the interleaved core-hole layout row r = v * nc + s holds valence
+// configuration v with the core hole in slot s.  A term on valence orbitals
+// only leaves the core word alone, and its fermion sign counts occupied
+// orbitals below its own position, which are all valence orbitals (the core
+// bits sit above them, src/ham.f90:419-446); so its contribution is the same
+// for every slot: that part of H is exactly H_v (x) I_nc.  The build
+// (src/ham.f90:352-711) sums every term of an entry, so the form is found
+// from the finished CSR, not from the terms: for each valence row v, the
+// off-diagonal entries that keep the slot (column c with c % nc == s) are
+// compared across the nc rows v * nc + s, value bit for value bit.  Where
+// they agree they are stored once, as valence row v of H_v (valence column,
+// value number); every other off-diagonal entry stays in a per-row remainder
+// R.  A valence row whose slot rows disagree keeps all its entries in R.
+//
+//     (H x)[v nc + s] = diag[r] x[r] + sum_k Hv[v, k] x[k nc + s] + sum_j R[r, j] x[j]
+//
+// The values are the entries of the exact complex128 CSR, referenced by their
+// numbers in the indexed form's table, so products are those of the FP64
+// operator (the summation order differs from the row-wise product, as it
+// does between any two schedules).  The payoff is in the gathers: the nc
+// lanes of a valence row read x[k nc + 0 .. nc - 1], nc * 16 contiguous
+// bytes, for each H_v entry, where the row-wise product makes nc separate
+// 16-byte gathers from nc different nonzeros.
+
+namespace {
+
+constexpr uint32_t FAC_COL_MASK = (1u << PK_COL_BITS) - 1;
+
+// per valence row: whether the slot-keeping entries agree across the nc slot
+// rows, and the entry counts of H_v (per valence row) and R (per row)
+__global__ void k_fac_count(int64_t mv, int nc, const int64_t *orow, const int32_t *ocols,
+                            const uint16_t *oidx, uint8_t *kron, int64_t *vcnt, int64_t *rcnt) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < mv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r0 = v * nc;
+        // slot-0 entries of row r0 that keep the slot: the reference list
+        int64_t n0 = 0;
+        for (int64_t j = orow[r0]; j < orow[r0 + 1]; ++j) n0 += (ocols[j] % nc == 0);
+        bool same = true;
+        for (int s = 1; s < nc && same; ++s) {
+            const int64_t r = r0 + s;
+            int64_t j0 = orow[r0];
+            const int64_t e0 = orow[r0 + 1];
+            int64_t ns = 0;
+            for (int64_t j = orow[r]; j < orow[r + 1] && same; ++j) {
+                const int32_t c = ocols[j];
+                if (c % nc != s) continue;
+                ++ns;
+                while (j0 < e0 && ocols[j0] % nc != 0) ++j0;   // next slot-0 reference entry
+                if (j0 == e0 || ocols[j0] / nc != c / nc || oidx[j0] != oidx[j]) same = false;
+                ++j0;
+            }
+            if (ns != n0) same = false;
+        }
+        kron[v] = same ? 1 : 0;
+        vcnt[v] = same ? n0 : 0;
+        for (int s = 0; s < nc; ++s) {
+            const int64_t r = r0 + s;
+            int64_t k = 0;
+            for (int64_t j = orow[r]; j < orow[r + 1]; ++j)
+                k += !(same && ocols[j] % nc == s);
+            rcnt[r] = k;
+        }
+    }
+}
+
+__global__ void k_fac_fill(int64_t mv, int nc, const int64_t *orow, const int32_t *ocols,
+                           const uint16_t *oidx, const uint8_t *kron, const int64_t *vrow,
+                           uint32_t *vpk, const int64_t *rrow, uint32_t *rpk) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < mv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r0 = v * nc;
+        const bool same = kron[v] != 0;
+        if (same) {
+            int64_t o = vrow[v];
+            for (int64_t j = orow[r0]; j < orow[r0 + 1]; ++j)
+                if (ocols[j] % nc == 0)
+                    vpk[o++] = (uint32_t)(ocols[j] / nc) | ((uint32_t)oidx[j] << PK_COL_BITS);
+        }
+        for (int s = 0; s < nc; ++s) {
+            const int64_t r = r0 + s;
+            int64_t o = rrow[r];
+            for (int64_t j = orow[r]; j < orow[r + 1]; ++j)
+                if (!(same && ocols[j] % nc == s))
+                    rpk[o++] = (uint32_t)ocols[j] | ((uint32_t)oidx[j] << PK_COL_BITS);
+        }
+    }
+}
+
+struct FacArgs {
+    int64_t mv;              // valence rows (m / NC)
+    const int64_t *vrow;     // mv + 1
+    const uint32_t *vpk;     // valence column | value number << 24
+    const int64_t *rrow;     // m + 1
+    const uint32_t *rpk;     // column | value number << 24
+    const double2 *tab;      // value table (ntab entries, staged in LDS)
+    int ntab;
+    const double2 *diag;     // exact diagonal, m entries
+};
+
+// y = c0 * (H x) + c1 * z (+ <d|y> block partials) in the slot-factored form.
+// Lane l of a wave owns valence row v = wave_base + l / NC and slot s = l % NC
+// (64 / NC valence rows per wave, 64 % NC lanes idle), so the NC lanes of a
+// valence row read each H_v entry's x[k NC + s] as one contiguous NC * 16-byte
+// segment, and the rows a wave writes are consecutive.  Each lane loads the
+// first UR words of its remainder row into registers before it walks H_v, so
+// those loads overlap the valence gathers; H_v is walked four entries at a
+// time (four independent gathers in flight).
+template <int NC, int UR, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_spmv_fac(FacArgs f, const double2 *__restrict__ x,
+                                                       SpmvEpi e) {
+    if (e.run && *e.run == 0) return;
+    extern __shared__ double2 s_tab[];
+    for (int i = threadIdx.x; i < f.ntab; i += 256) s_tab[i] = f.tab[i];
+    __syncthreads();
+    constexpr int VPW = 64 / NC;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int vl = lane / NC, s = lane - vl * NC;
+    const int64_t bid = e.xmap ? (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
+                               : (int64_t)blockIdx.x;
+    const int64_t v = (bid * 4 + wave) * VPW + vl;
+    const bool act = vl < VPW && v < f.mv;
+    double2 dot = make_double2(0.0, 0.0);
+    if (act) {
+        const int64_t r = v * NC + s;
+        const int64_t jb = f.rrow[r], je = f.rrow[r + 1];
+        uint32_t rw[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) rw[u] = jb + u < je ? f.rpk[jb + u] : 0u;
+        const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
+        const double2 dg = f.diag[r];
+        const double2 xr = x[r];
+        double2 pz = make_double2(0, 0), pd = pz;
+        if (e.z) pz = e.z[r];
+        if (e.dvec) pd = e.dvec[r];
+        double sr = 0.0, si = 0.0;
+        auto fma_c = [&](double2 a, double2 xv) {
+            sr += a.x * xv.x - a.y * xv.y;
+            si += a.x * xv.y + a.y * xv.x;
+        };
+        int64_t k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            const uint32_t w0 = f.vpk[k], w1 = f.vpk[k + 1], w2 = f.vpk[k + 2], w3 = f.vpk[k + 3];
+            const double2 x0 = x[(int64_t)(w0 & FAC_COL_MASK) * NC + s];
+            const double2 x1 = x[(int64_t)(w1 & FAC_COL_MASK) * NC + s];
+            const double2 x2 = x[(int64_t)(w2 & FAC_COL_MASK) * NC + s];
+            const double2 x3 = x[(int64_t)(w3 & FAC_COL_MASK) * NC + s];
+            fma_c(s_tab[w0 >> PK_COL_BITS], x0);
+            fma_c(s_tab[w1 >> PK_COL_BITS], x1);
+            fma_c(s_tab[w2 >> PK_COL_BITS], x2);
+            fma_c(s_tab[w3 >> PK_COL_BITS], x3);
+        }
+        for (; k < ke; ++k) {
+            const uint32_t w = f.vpk[k];
+            fma_c(s_tab[w >> PK_COL_BITS], x[(int64_t)(w & FAC_COL_MASK) * NC + s]);
+        }
+#pragma unroll
+        for (int u = 0; u < UR; ++u)
+            if (jb + u < je) fma_c(s_tab[rw[u] >> PK_COL_BITS], x[rw[u] & FAC_COL_MASK]);
+        for (int64_t j = jb + UR; j < je; ++j) {
+            const uint32_t w = f.rpk[j];
+            fma_c(s_tab[w >> PK_COL_BITS], x[w & FAC_COL_MASK]);
+        }
+        fma_c(dg, xr);
+        const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
+        const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
+        double2 out = cmul(c0, make_double2(sr, si));
+        if (e.z) out = cadd(out, cmul(c1, pz));
+        e.y[r] = out;
+        if (e.dvec) dot = cconjmul(pd, out);
+    }
+    if (e.dvec) {
+        const double2 bs = block_sum_256(dot);
+        if (threadIdx.x == 0) e.partials[blockIdx.x] = bs;
+    }
+}
+
+// The same product with more loads in flight per lane: every lane first loads
+// up to UV words of its valence row's H_v entries and up to UR words of its
+// remainder row (all independent), then issues the gathers of the first eight
+// H_v entries together with its remainder row's, then the next H_v entries,
+// so a row costs about three memory latencies instead of one per group of four
+// entries.  Entries beyond UV (H_v) or UR (R) are read in place.
+template <int NC, int UV, int UR, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_spmv_fac_b(FacArgs f, const double2 *__restrict__ x,
+                                                         SpmvEpi e) {
+    if (e.run && *e.run == 0) return;
+    extern __shared__ double2 s_tab[];
+    for (int i = threadIdx.x; i < f.ntab; i += 256) s_tab[i] = f.tab[i];
+    __syncthreads();
+    constexpr int VPW = 64 / NC;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int vl = lane / NC, s = lane - vl * NC;
+    const int64_t bid = e.xmap ? (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
+                               : (int64_t)blockIdx.x;
+    const int64_t v = (bid * 4 + wave) * VPW + vl;
+    const bool act = vl < VPW && v < f.mv;
+    double2 dot = make_double2(0.0, 0.0);
+    if (act) {
+        const int64_t r = v * NC + s;
+        const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
+        const int64_t jb = f.rrow[r], je = f.rrow[r + 1];
+        const int nv = (int)(ke - kb), nr = (int)(je - jb);
+        uint32_t vw[UV], rw[UR];
+#pragma unroll
+        for (int u = 0; u < UV; ++u) vw[u] = u < nv ? f.vpk[kb + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < UR; ++u) rw[u] = u < nr ? f.rpk[jb + u] : 0u;
+        double sr = 0.0, si = 0.0;
+        auto fma_c = [&](double2 a, double2 xv) {
+            sr += a.x * xv.x - a.y * xv.y;
+            si += a.x * xv.y + a.y * xv.x;
+        };
+        constexpr int B1 = UV < 8 ? UV : 8;
+        double2 xa[B1], xb[UR];
+#pragma unroll
+        for (int u = 0; u < B1; ++u)
+            xa[u] = u < nv ? x[(int64_t)(vw[u] & FAC_COL_MASK) * NC + s] : make_double2(0, 0);
+#pragma unroll
+        for (int u = 0; u < UR; ++u) xb[u] = u < nr ? x[rw[u] & FAC_COL_MASK] : make_double2(0, 0);
+#pragma unroll
+        for (int u = 0; u < B1; ++u)
+            if (u < nv) fma_c(s_tab[vw[u] >> PK_COL_BITS], xa[u]);
+#pragma unroll
+        for (int u = 0; u < UR; ++u)
+            if (u < nr) fma_c(s_tab[rw[u] >> PK_COL_BITS], xb[u]);
+        if (UV > B1 && nv > B1) {
+            double2 xc[UV > B1 ? UV - B1 : 1];
+#pragma unroll
+            for (int u = B1; u < UV; ++u)
+                xc[u - B1] = u < nv ? x[(int64_t)(vw[u] & FAC_COL_MASK) * NC + s] : make_double2(0, 0);
+#pragma unroll
+            for (int u = B1; u < UV; ++u)
+                if (u < nv) fma_c(s_tab[vw[u] >> PK_COL_BITS], xc[u - B1]);
+        }
+        for (int64_t k = kb + UV; k < ke; ++k) {
+            const uint32_t w = f.vpk[k];
+            fma_c(s_tab[w >> PK_COL_BITS], x[(int64_t)(w & FAC_COL_MASK) * NC + s]);
+        }
+        for (int64_t j = jb + UR; j < je; ++j) {
+            const uint32_t w = f.rpk[j];
+            fma_c(s_tab[w >> PK_COL_BITS], x[w & FAC_COL_MASK]);
+        }
+        fma_c(f.diag[r], x[r]);
+        const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
+        const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
+        double2 out = cmul(c0, make_double2(sr, si));
+        if (e.z) out = cadd(out, cmul(c1, e.z[r]));
+        e.y[r] = out;
+        if (e.dvec) dot = cconjmul(e.dvec[r], out);
+    }
+    if (e.dvec) {
+        const double2 bs = block_sum_256(dot);
+        if (threadIdx.x == 0) e.partials[blockIdx.x] = bs;
+    }
+}
+
+}  // namespace
+
+namespace edx {
+
+// Valence rows per workgroup of the factored product (4 waves).
+inline int64_t fac_rows_per_block(int nc) { return 4 * (64 / nc); }
+
+int fac_blocks(const edx_csr *a) {
+    const int64_t mv = a->m / a->nc_slots;
+    int64_t blocks = (mv + fac_rows_per_block(a->nc_slots) - 1) / fac_rows_per_block(a->nc_slots);
+    if (blocks < 1) blocks = 1;
+    if (blocks >= 64) blocks = (blocks + 7) / 8 * 8;
+    return (int)blocks;
+}
+
+// Launch of the factored product (schedules FAC_CODE and the batched-load
+// variants FAC_CODE + 1, + 2); returns the block count.
+int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e_in, int variant,
+               int *nblocks_out) {
+    if (!a->d_fr_rowptr)
+        return set_error(EDX_ERR_ARG, "SpMV schedule %d needs the slot-factored form", FAC_CODE);
+    const int nb = fac_blocks(a);
+    if (a->m == 0) return EDX_OK;
+    if (nblocks_out) *nblocks_out = nb;
+    SpmvEpi e = e_in;
+    e.xmap = nb % 8 == 0 && xcd_map_enabled();
+    FacArgs f{a->m / a->nc_slots, a->d_fv_rowptr, a->d_fv_pk, a->d_fr_rowptr, a->d_fr_pk,
+              a->d_vtab, (int)a->nvtab, a->d_diag};
+    if (a->nc_slots != 6)
+        return set_error(EDX_ERR_ARG, "slot-factored product for %d core slots not built",
+                         a->nc_slots);
+    const size_t lds = (size_t)a->nvtab * sizeof(double2);
+    if (variant == 0)
+        hipLaunchKernelGGL((k_spmv_fac<6, 8, 4>), dim3(nb), dim3(256), lds, stream(), f, x, e);
+    else if (variant == 1)
+        hipLaunchKernelGGL((k_spmv_fac_b<6, 16, 8, 2>), dim3(nb), dim3(256), lds, stream(), f, x, e);
+    else
+        hipLaunchKernelGGL((k_spmv_fac_b<6, 12, 8, 4>), dim3(nb), dim3(256), lds, stream(), f, x, e);
+    EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
+// Algorithmic bytes of the factored product as stored: the words and row
+// pointers of H_v and R, the value table, the diagonal, x once, y written and
+// the epilogue's addend and dot vector.
+double fac_bytes(const edx_csr *a, bool z, bool dvec) {
+    const double mv = (double)(a->m / a->nc_slots);
+    return (double)(a->fv_nnz + a->fr_nnz) * 4.0 + (mv + 1.0) * 8.0 + (double)(a->m + 1) * 8.0 +
+           (double)a->nvtab * 16.0 + (double)a->m * 16.0 + (double)a->n * 16.0 +
+           (double)a->m * 16.0 * (1.0 + (z ? 1.0 : 0.0) + (dvec ? 1.0 : 0.0));
+}
+
+// Builds the slot-factored form from the indexed form.  Only square matrices
+// in the interleaved core-hole layout with a packable indexed form (fewer
+// than 2^24 columns, at most 256 values) and a built core-slot count qualify;
+// for others nothing is made and the row-wise forms serve.
+int csr_add_factored(edx_csr *a) {
+    if (a->d_fr_rowptr) return EDX_OK;
+    const int nc = a->nc_slots;
+    if (nc != 6 || a->m != a->n || a->m % nc != 0 || !a->d_off_idx) return EDX_OK;
+    if (a->n >= (int64_t(1) << PK_COL_BITS) || a->nvtab > (int64_t(1) << (32 - PK_COL_BITS)))
+        return EDX_OK;
+    const char *off = getenv("EDX_NO_FACTOR");
+    if (off && off[0] == '1') return EDX_OK;
+    const int64_t m = a->m, mv = m / nc;
+    auto *kron = (uint8_t *)dmalloc((size_t)mv, "factored flags");
+    auto *vrow = (int64_t *)dmalloc((size_t)(mv + 1) * sizeof(int64_t), "factored H_v rowptr");
+    auto *rrow = (int64_t *)dmalloc((size_t)(m + 1) * sizeof(int64_t), "factored R rowptr");
+    uint32_t *vpk = nullptr, *rpk = nullptr;
+    int64_t vn = 0, rn = 0;
+    int rc = EDX_OK;
+    if (!kron || !vrow || !rrow) {
+        rc = EDX_ERR_NOMEM;
+        goto fail;
+    }
+    if (hipMemsetAsync(vrow + mv, 0, sizeof(int64_t), stream()) != hipSuccess ||
+        hipMemsetAsync(rrow + m, 0, sizeof(int64_t), stream()) != hipSuccess) {
+        rc = set_error(EDX_ERR_HIP, "factored form: memset failed");
+        goto fail;
+    }
+    hipLaunchKernelGGL(k_fac_count, dim3(elem_blocks(mv)), dim3(256), 0, stream(), mv, nc,
+                       a->d_off_rowptr, a->d_off_cols, a->d_off_idx, kron, vrow, rrow);
+    if (hipGetLastError() != hipSuccess) {
+        rc = set_error(EDX_ERR_HIP, "factored form: count kernel failed");
+        goto fail;
+    }
+    if ((rc = exclusive_scan(vrow, mv + 1)) || (rc = exclusive_scan(rrow, m + 1))) goto fail;
+    if (hipMemcpy(&vn, vrow + mv, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&rn, rrow + m, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = set_error(EDX_ERR_HIP, "factored form: download failed");
+        goto fail;
+    }
+    vpk = (uint32_t *)dmalloc((size_t)std::max<int64_t>(vn, 1) * sizeof(uint32_t), "factored H_v words");
+    rpk = (uint32_t *)dmalloc((size_t)std::max<int64_t>(rn, 1) * sizeof(uint32_t), "factored R words");
+    if (!vpk || !rpk) {
+        rc = EDX_ERR_NOMEM;
+        goto fail;
+    }
+    hipLaunchKernelGGL(k_fac_fill, dim3(elem_blocks(mv)), dim3(256), 0, stream(), mv, nc,
+                       a->d_off_rowptr, a->d_off_cols, a->d_off_idx, kron, vrow, vpk, rrow, rpk);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream()) != hipSuccess) {
+        rc = set_error(EDX_ERR_HIP, "factored form: fill kernel failed");
+        goto fail;
+    }
+    dfree(kron);
+    a->d_fv_rowptr = vrow;
+    a->d_fv_pk = vpk;
+    a->fv_nnz = vn;
+    a->d_fr_rowptr = rrow;
+    a->d_fr_pk = rpk;
+    a->fr_nnz = rn;
+    return EDX_OK;
+fail:
+    dfree(kron);
+    dfree(vrow);
+    dfree(rrow);
+    dfree(vpk);
+    dfree(rpk);
+    return rc;
+}
+
+}  // namespace edx
+
+extern "C" {
+
+int edx_csr_factored_info(const edx_csr *a, int *nc_slots, int64_t *hv_nnz, int64_t *r_nnz) {
+    EDX_CHECK(a, EDX_ERR_ARG, "null matrix");
+    const bool on = a->d_fr_rowptr != nullptr;
+    if (nc_slots) *nc_slots = on ? a->nc_slots : 0;
+    if (hv_nnz) *hv_nnz = on ? a->fv_nnz : 0;
+    if (r_nnz) *r_nnz = on ? a->fr_nnz : 0;
+    return EDX_OK;
+}
+
+}  // extern "C"

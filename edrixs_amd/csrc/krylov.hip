@@ -53,33 +53,73 @@ int apply(const edx_csr *h, int precision, const double2 *x, float2 *x32, const 
     return edx::spmv_launch(h, precision, x, e, 0, nb);
 }
 
+// Device memory of one Krylov run beyond its input and outputs: two vectors,
+// the dot partials, the device state, the (alpha, beta) history and the
+// per-step run flags.  krylov_scratch_bytes gives the size of one block that
+// holds them all, carved by krylov_scratch_carve at 256-byte boundaries.
+struct KrylovScratch {
+    double2 *v;        // 2 n
+    double2 *part;     // max(partials_len, vec_blocks(n))
+    KrylovDev *S;
+    double *ab;        // 2 maxn + 2
+    int *run;          // maxn + 2
+};
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t krylov_scratch_bytes(int64_t n, int64_t partials_len, int maxn) {
+    const int64_t np = std::max<int64_t>(partials_len, vec_blocks(n));
+    return align256(2 * (size_t)n * sizeof(double2)) + align256((size_t)np * sizeof(double2)) +
+           align256(sizeof(KrylovDev)) + align256((2 * (size_t)maxn + 2) * sizeof(double)) +
+           align256(((size_t)maxn + 2) * sizeof(int));
+}
+
+KrylovScratch krylov_scratch_carve(char *base, int64_t n, int64_t partials_len, int maxn) {
+    KrylovScratch k;
+    const int64_t np = std::max<int64_t>(partials_len, vec_blocks(n));
+    k.v = (double2 *)base;
+    base += align256(2 * (size_t)n * sizeof(double2));
+    k.part = (double2 *)base;
+    base += align256((size_t)np * sizeof(double2));
+    k.S = (KrylovDev *)base;
+    base += align256(sizeof(KrylovDev));
+    k.ab = (double *)base;
+    base += align256((2 * (size_t)maxn + 2) * sizeof(double));
+    k.run = (int *)base;
+    return k;
+}
+
+// Zero nbytes (a multiple of 8, 8-byte aligned) with vector stores; used
+// instead of hipMemsetAsync where the caller may capture the launches.
+__global__ void k_zero_words(int64_t nwords, uint64_t *p) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwords;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 // build_krylov_mp (src/lanczos.f90:210-303) over any product
 // prod(x, epilogue, &nblocks) that writes y = c0 A x + c1 z with dot partials;
-// partials_len bounds the partials one product writes.
+// the run's device memory comes in k (see KrylovScratch).
 template <class Prod>
-int krylov_loop(int64_t n, Prod prod, int64_t partials_len, const double2 *v0, int maxn,
-                int *neff, double *alpha, double *beta, double *norm) {
-    DevBuf vb, part, st, ab, runf;
-    EDX_TRY(vb.alloc(2 * n * sizeof(double2), "krylov vectors"));
+int krylov_loop_in(int64_t n, Prod prod, const KrylovScratch &k, const double2 *v0, int maxn,
+                   int *neff, double *alpha, double *beta, double *norm) {
     int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc((size_t)std::max<int64_t>(partials_len, nbv) * sizeof(double2),
-                       "partials"));
-    EDX_TRY(st.alloc(sizeof(KrylovDev), "krylov state"));
-    EDX_TRY(ab.alloc((2 * (size_t)maxn + 2) * sizeof(double), "alpha beta"));
-    EDX_TRY(runf.alloc(((size_t)maxn + 2) * sizeof(int), "run flags"));
-    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + n};
-    double *d_alpha = ab.as<double>(), *d_beta = ab.as<double>() + maxn + 1;
-    int *run = runf.as<int>();
-    KrylovDev *S = st.as<KrylovDev>();
+    double2 *v[2] = {k.v, k.v + n};
+    double *d_alpha = k.ab, *d_beta = k.ab + maxn + 1;
+    int *run = k.run;
+    KrylovDev *S = k.S;
     hipStream_t s = edx::stream();
-    EDX_HIP(hipMemsetAsync(vb.p, 0, 2 * n * sizeof(double2), s));
-    EDX_HIP(hipMemsetAsync(ab.p, 0, (2 * (size_t)maxn + 2) * sizeof(double), s));
-    EDX_HIP(hipMemsetAsync(run, 0, ((size_t)maxn + 2) * sizeof(int), s));
+    // clear the vectors, the history and the run flags (fill kernels, no memset nodes)
+    const int64_t nz[3] = {4 * n, 2 * (int64_t)maxn + 2, ((int64_t)maxn + 2 + 1) / 2};
+    uint64_t *pz[3] = {(uint64_t *)k.v, (uint64_t *)k.ab, (uint64_t *)k.run};
+    for (int q = 0; q < 3; ++q)
+        hipLaunchKernelGGL(k_zero_words, dim3(vec_blocks(nz[q])), dim3(VB), 0, s, nz[q], pz[q]);
     EDX_HIP(hipMemcpyAsync(v[0], v0, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    double2 *part_p = k.part;
     hipLaunchKernelGGL(k_krylov_init, dim3(1), dim3(1), 0, s, S, run);
     // norm = <v0|v0>; v1 = v0 / sqrt(norm)   (src/lanczos.f90:256-262)
-    hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, v[0], v[0], part.as<double2>(), nullptr);
-    hipLaunchKernelGGL(k_krylov_norm, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S);
+    hipLaunchKernelGGL(k_dot, dim3(nbv), dim3(VB), 0, s, n, v[0], v[0], part_p, nullptr);
+    hipLaunchKernelGGL(k_krylov_norm, dim3(1), dim3(VB), 0, s, part_p, nbv, S);
     hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[0], &S->sqrt_norm, nullptr);
     EDX_LAUNCH_CHECK();
     int curr = 0, prev = 1;
@@ -92,15 +132,15 @@ int krylov_loop(int64_t n, Prod prod, int64_t partials_len, const double2 *v0, i
         e.c0 = &S->coef[0];
         e.c1 = &S->coef[1];
         e.dvec = v[curr];
-        e.partials = part.as<double2>();
+        e.partials = part_p;
         e.run = run + (i - 1);
         int nb = 0;
         EDX_TRY(prod((const double2 *)v[curr], e, &nb, run + (i - 1)));
-        hipLaunchKernelGGL(k_krylov_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
+        hipLaunchKernelGGL(k_krylov_alpha, dim3(1), dim3(VB), 0, s, part_p, nb, S,
                            d_alpha, i, run);
         hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
-                           d_alpha + (i - 1), part.as<double2>(), run + (i - 1));
-        hipLaunchKernelGGL(k_krylov_beta, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S,
+                           d_alpha + (i - 1), part_p, run + (i - 1));
+        hipLaunchKernelGGL(k_krylov_beta, dim3(1), dim3(VB), 0, s, part_p, nbv, S,
                            d_beta, i, run);
         hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[prev], d_beta + i, run + i);
         EDX_LAUNCH_CHECK();
@@ -114,13 +154,22 @@ int krylov_loop(int64_t n, Prod prod, int64_t partials_len, const double2 *v0, i
     }
     KrylovDev hs;
     EDX_HIP(hipMemcpy(&hs, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
-    std::vector<double> hb(maxn + 1);
     EDX_HIP(hipMemcpy(alpha, d_alpha, maxn * sizeof(double), hipMemcpyDeviceToHost));
-    EDX_HIP(hipMemcpy(hb.data(), d_beta, (maxn + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    // beta_1 .. beta_maxn
+    EDX_HIP(hipMemcpy(beta, d_beta + 1, maxn * sizeof(double), hipMemcpyDeviceToHost));
     *neff = hs.neff;
     *norm = hs.norm;
-    for (int i = 0; i < maxn; ++i) beta[i] = hb[i + 1];   // beta_1 .. beta_maxn
     return EDX_OK;
+}
+
+// krylov_loop_in with its device memory allocated for the run.
+template <class Prod>
+int krylov_loop(int64_t n, Prod prod, int64_t partials_len, const double2 *v0, int maxn,
+                int *neff, double *alpha, double *beta, double *norm) {
+    DevBuf buf;
+    EDX_TRY(buf.alloc(krylov_scratch_bytes(n, partials_len, maxn), "krylov scratch"));
+    const KrylovScratch k = krylov_scratch_carve(buf.as<char>(), n, partials_len, maxn);
+    return krylov_loop_in(n, prod, k, v0, maxn, neff, alpha, beta, norm);
 }
 
 int krylov_run(const edx_csr *h, int precision, const double2 *v0, int maxn, int *neff,
@@ -770,3 +819,4 @@ int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, do
 }
 
 }  // extern "C"
+

@@ -523,10 +523,28 @@ constexpr int PIPE_MAX_BPC = 5;
 bool is_pipe_code(int code) { return code == 143 || code == 144 || code == 145 || code == 184; }
 int pipe_blocks(int bpc) { return (num_cus() * bpc + 7) / 8 * 8; }
 
+// Schedule FAC_CODE: the slot-factored product of a core-hole matrix in the
+// interleaved layout (factor.hip), indexed form only.
+constexpr int FAC_CODE = 206;
+bool is_fac_code(int code) { return code >= FAC_CODE && code <= FAC_CODE + 2; }
+int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e, int variant,
+               int *nblocks_out);
+int fac_blocks(const edx_csr *a);
+double fac_bytes(const edx_csr *a, bool z, bool dvec);
+
+// Whether schedule code G can run on this matrix in this precision.
+bool schedule_available(const edx_csr *a, int precision, int G) {
+    if (G == 145) return precision == EDX_C128I && a->d_off_pk;
+    if (is_fac_code(G)) return precision == EDX_C128I && a->d_fr_rowptr;
+    return true;
+}
+
 // Length of a partials buffer that serves every single-vector SpMV launch of
 // the matrix: one entry per block of the widest launch plus the folded sums.
 int64_t spmv_partials_len(const edx_csr *a) {
-    return (int64_t)std::max(spmv_blocks(a, 64), pipe_blocks(PIPE_MAX_BPC)) + FOLD_N + 64;
+    int64_t nb = std::max(spmv_blocks(a, 64), pipe_blocks(PIPE_MAX_BPC));
+    if (a->nc_slots > 0 && a->m == a->n) nb = std::max<int64_t>(nb, fac_blocks(a));
+    return nb + FOLD_N + 64;
 }
 
 // Launch y = c0*A x + c1*z (+ dot partials).  x is complex128 for C128 and
@@ -554,6 +572,25 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
         else if (precision == EDX_C64D) G = a->spmv_lanes64d ? a->spmv_lanes64d : lanes_for_split(a);
         else if (precision == EDX_C128I) G = a->spmv_lanes128i ? a->spmv_lanes128i : lanes_for_split(a);
         else G = a->spmv_lanes ? a->spmv_lanes : lanes_for(a);
+    }
+    if (is_fac_code(G)) {
+        if (!schedule_available(a, precision, G))
+            return set_error(EDX_ERR_ARG, "SpMV schedule %d needs the slot-factored form", G);
+        int nb = 0;
+        hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
+                                   fac_bytes(a, e_in.z != nullptr, e_in.dvec != nullptr));
+        EDX_TRY(fac_launch(a, (const double2 *)x, e_in, G - FAC_CODE, &nb));
+        prof_stop(pe);
+        if (nblocks_out) *nblocks_out = nb;
+        if (e_in.dvec && nb > FOLD_N) {
+            hipLaunchKernelGGL(k_fold_partials, dim3(FOLD_N), dim3(256), 0, stream(), e_in.partials,
+                               nb, e_in.run);
+            hipLaunchKernelGGL(k_move_folded, dim3(1), dim3(FOLD_N), 0, stream(), e_in.partials, nb,
+                               e_in.run);
+            EDX_LAUNCH_CHECK();
+            if (nblocks_out) *nblocks_out = FOLD_N;
+        }
+        return EDX_OK;
     }
     const bool pipe = is_pipe_code(G);
     int nb = pipe ? pipe_blocks(G % 10) : spmv_blocks(a, G);

@@ -146,8 +146,20 @@ int edx_spmm(const edx_csr *a, int precision, int nrhs, const void *d_X, void *d
 /* Lanes per row of the single-vector SpMV of one precision: set = 4..64
  * fixes it, set = 0 only reads it; *lanes = 0 means the mean-row-length rule
  * (matrices under 131072 rows, or before the first edx_csr_prepare, which
- * times the choices on larger matrices).  No reference counterpart. */
+ * times the choices on larger matrices).  Codes above 64 name other
+ * schedules: 143, 144, 184 (pipelined), 145 (pipelined, packed indexed form
+ * only), 206 (slot-factored form, EDX_C128I only).  No reference counterpart. */
 int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes);
+
+/* The slot-factored form of a square core-hole matrix in the interleaved
+ * layout, made by edx_csr_prepare(EDX_C128I) where the matrix qualifies (6
+ * core slots, fewer than 2^24 rows, at most 256 distinct off-diagonal
+ * values): the off-diagonal entries that are the same for every core slot of
+ * a valence row (terms on valence orbitals only, src/ham.f90:419-446) stored
+ * once per valence row, the rest per row.  *nc_slots = 0 when the form is
+ * absent; *hv_nnz and *r_nnz count the two parts' entries.  No reference
+ * counterpart (a storage form of the matrix of src/ham.f90:352-711). */
+int edx_csr_factored_info(const edx_csr *a, int *nc_slots, int64_t *hv_nnz, int64_t *r_nnz);
 
 /* Live profiling of SpMV/SpMM launches (no reference counterpart): while
  * enabled, every launch is bracketed by HIP events on its stream and its
