@@ -26,6 +26,7 @@
  *                         implicit QL with Wilkinson shifts (Golub & Van Loan, 8.3)
  *   orc_pminres           src/linsys.f90:3-128
  *   orc_denmat            src/ed_driver.f90:219-248
+ *   orc_opavg             src/opavg_driver.f90:93-108 (w = O v, dot_product(v, w))
  *   orc_cf_spectrum       edrixs/plot_spectrum.py:52-59 (and src/lanczos.f90:306-336)
  *
  * CSR layout used everywhere: rowptr int64[m+1] (0-based), cols int32[nnz]
@@ -238,20 +239,23 @@ int orc_build(int rowkind, int64_t nleft, const int64_t *fock_left,
     int64_t m = row_end - row_begin;
     if (m < 0) m = 0;
 
-    /* contiguous row chunks, one per thread, joined in row order */
+    /* nt contiguous row chunks, each with its own accumulator, joined in row
+       order; every chunk is built whatever number of threads the runtime gives */
     int nt = omp_get_max_threads();
     if (nt > 1 && m / 2048 < nt) nt = (int)(m / 2048) > 1 ? (int)(m / 2048) : 1;
     orc_csr_acc *accs = (orc_csr_acc *)calloc((size_t)nt, sizeof(orc_csr_acc));
-    int64_t *rowptr = (int64_t *)malloc((size_t)(m + 1) * sizeof(int64_t));
-    #pragma omp parallel num_threads(nt)
+    int64_t *rowptr = (int64_t *)calloc((size_t)(m + 1), sizeof(int64_t));
+    #pragma omp parallel
     {
-        const int t = omp_get_thread_num();
-        const int64_t b = row_begin + m * t / nt, e = row_begin + m * (t + 1) / nt;
         orc_row row = {0};
-        for (int64_t r = b; r < e; ++r) {
-            build_row(&A, r, &row);
-            acc_push_row(&accs[t], &row);
-            rowptr[r - row_begin + 1] = row.n;
+        #pragma omp for schedule(static)
+        for (int t = 0; t < nt; ++t) {
+            const int64_t b = row_begin + m * t / nt, e = row_begin + m * (t + 1) / nt;
+            for (int64_t r = b; r < e; ++r) {
+                build_row(&A, r, &row);
+                acc_push_row(&accs[t], &row);
+                rowptr[r - row_begin + 1] = row.n;
+            }
         }
         free(row.e);
     }
@@ -607,4 +611,22 @@ void orc_cf_spectrum(int neff, const double *alpha, const double *beta, double n
         tv = 1.0 / (z - alpha[0] - tv);
         spec[w] = -1.0 / pi * cimag(tv) * norm;
     }
+}
+
+/* src/opavg_driver.f90:93-108: w = 0; w = w + O v (pspmv_csr); the average is
+   dot_product(v, w) = sum_k conj(v_k) w_k, summed in index order. */
+void orc_opavg(int64_t m, const int64_t *rowptr, const int32_t *cols, const double *vals,
+               const double *v, double *out)
+{
+    double *w = (double *)calloc((size_t)(2 * m > 0 ? 2 * m : 1), sizeof(double));
+    orc_matvec_csr(m, rowptr, cols, vals, v, w);
+    double sr = 0.0, si = 0.0;
+    for (int64_t k = 0; k < m; ++k) {
+        const double vr = v[2 * k], vi = v[2 * k + 1], wr = w[2 * k], wi = w[2 * k + 1];
+        sr += vr * wr + vi * wi;
+        si += vr * wi - vi * wr;
+    }
+    free(w);
+    out[0] = sr;
+    out[1] = si;
 }

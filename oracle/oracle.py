@@ -55,6 +55,7 @@ def lib():
         L.orc_tridiag_eigs.argtypes = [ctypes.c_int, P, P, P, P]
         L.orc_pminres.argtypes = [i64, P, P, P, P, P, ctypes.c_double, ctypes.c_int, P, P]
         L.orc_denmat.argtypes = [i64, P, ctypes.c_int, P, P]
+        L.orc_opavg.argtypes = [i64, P, P, P, P, P]
         L.orc_cf_spectrum.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, P, P, P]
         L.orc_make_newfock.argtypes = [ctypes.c_char, ctypes.c_int, i64, P, P]
@@ -234,6 +235,22 @@ def denmat(fock, num_val_orbs, vec):
     return dm
 
 
+def opavg(A, v):
+    """<v|A|v> as the opavg driver computes it (src/opavg_driver.f90:93-108)."""
+    L = lib()
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    out = np.zeros(2)
+    L.orc_opavg(A.m, _p(A.rowptr), _p(A.cols), _p(A.vals.view(np.float64)), _p(v), _p(out))
+    return complex(out[0], out[1])
+
+
+def opavg_lines(evals, avgs):
+    """The lines of opavg.dat: format (1i5, 4f20.10) with three values per
+    record (src/io.f90:638-659)."""
+    return ["%5d%20.10f%20.10f%20.10f" % (i + 1, e, a.real, a.imag)
+            for i, (e, a) in enumerate(zip(evals, avgs))]
+
+
 def cf_spectrum(alpha, beta, norm, e_gs, omega, gamma):
     """alpha, beta as in a pole file (alpha_1..n, beta_1..n)."""
     L = lib()
@@ -261,10 +278,25 @@ def binary_search(vec, m):
 
 # ----------------------------------------------------------------- dense checks
 
+def _popcount64(w):
+    """Population count of each element of a uint64 array (SWAR bit sums)."""
+    w = w - ((w >> np.uint64(1)) & np.uint64(0x5555555555555555))
+    w = (w & np.uint64(0x3333333333333333)) + ((w >> np.uint64(2)) & np.uint64(0x3333333333333333))
+    w = (w + (w >> np.uint64(4))) & np.uint64(0x0F0F0F0F0F0F0F0F)
+    return (w * np.uint64(0x0101010101010101)) >> np.uint64(56)
+
+
 def fock_sorted(norb, nocc):
-    """All norb-bit words with popcount nocc in ascending order (brute force)."""
-    words = [sum(1 << p for p in c) for c in __import__("itertools").combinations(range(norb), nocc)]
-    return np.array(sorted(words), dtype=np.int64)
+    """All norb-bit words with popcount nocc in ascending order, by brute force:
+    every word below 2**norb is tested, in ascending chunks, and the ones with
+    nocc set bits are kept (the definition of the basis, edrixs/fock_basis.py:
+    398-451, with no combinatorial shortcut)."""
+    chunk = 1 << 22
+    out = []
+    for lo in range(0, 1 << norb, chunk):
+        w = np.arange(lo, min(lo + chunk, 1 << norb), dtype=np.uint64)
+        out.append(w[_popcount64(w) == nocc].astype(np.int64))
+    return np.concatenate(out) if out else np.zeros(0, np.int64)
 
 
 def jw_annihilators(norb):

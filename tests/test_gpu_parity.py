@@ -1318,3 +1318,174 @@ def test_gs_sector_search_matches_dense_sectors(core):
         assert abs(e - e_dense[n]) < 1e-9
         assert -1e-9 <= occ <= 10 + 1e-9
     assert table[0][0] == n_gs and [t[1] for t in table] == sorted(t[1] for t in table)
+
+
+def test_slot_factored_form_matches_oracle(core):
+    # H_n in the interleaved core-hole layout: the off-diagonal entries that keep
+    # the core slot come from valence-only terms and are the same for the six
+    # slots of a valence row, so the factored form stores them once per valence
+    # row (H_v) and the rest per row (R).  The split is checked against counts
+    # taken from the oracle's CSR (reference layout), and the factored product
+    # (schedule 206) against the oracle's matvec: on the plain model, and on one
+    # with core-valence terms that keep the slot, so some valence rows differ
+    # between slots and stay whole in R.
+    from edrixs_amd import siam
+    base = H.small_siam(nbath=4, seed=2)
+    nv, nc = base.norb, base.ncore
+    u = base.umat_n.copy()
+    for pc, d1, d2 in ((nv + 0, 0, 3), (nv + 4, 1, 6)):   # c+_p c+_d1 c_d2 c_p and partner
+        u[pc, d1, d2, pc] += 0.3 + 0.2j
+        u[pc, d2, d1, pc] += 0.3 - 0.2j
+    variants = [(base.coul_n, True), (siam.umat_terms(u), False)]
+    fn = H.fock_words(nv, base.nocc + 1)
+    dv = len(fn)
+    bn = core.Basis.from_words(fn, nv)
+    p = core.corehole_permutation(dv, nc)            # x_inter = x_ref[p]
+    pinv = np.empty_like(p)
+    pinv[p] = np.arange(len(p))
+    rng = np.random.default_rng(23)
+    for coul, all_same in variants:
+        A = core.build_ham_n(bn, nv, nc, base.hop_n, coul, layout="interleaved")
+        R = o.build_ham_n(fn, nv, nc, base.hop_n, coul)
+        A.prepare("c128i")
+        f = A.factored()
+        assert f is not None and f[0] == nc
+        # expected split from the oracle's entries: per valence row, the
+        # slot-keeping off-diagonal (valence column, value) lists of the six slots
+        lists = [[[] for _ in range(nc)] for _ in range(dv)]
+        n_off = 0
+        for r in range(R.m):
+            s, v = divmod(r, dv)
+            for j in range(R.rowptr[r], R.rowptr[r + 1]):
+                c = int(R.cols[j])
+                if c == r:
+                    continue
+                n_off += 1
+                s2, v2 = divmod(c, dv)
+                if s2 == s:
+                    lists[v][s].append((v2, R.vals[j].real, R.vals[j].imag))
+        same = [all(lists[v][s] == lists[v][0] for s in range(1, nc)) for v in range(dv)]
+        hv = sum(len(lists[v][0]) for v in range(dv) if same[v])
+        assert all(same) == all_same and any(same)
+        assert f[1] == hv and f[2] == n_off - nc * hv, (f, hv, n_off)
+        x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+        y0 = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+        ref = o.matvec(R, x[pinv], y0[pinv])[p]
+        scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
+        A.spmv_lanes("c128i", set=206)
+        assert A.spmv_lanes("c128i") == 206
+        y = A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128i").cpu().numpy()
+        assert np.abs(y - ref).max() < 1e-14 * scale
+        # a Krylov recurrence (fused epilogue, dot partials) on the factored product
+        a1, b1, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+        A.spmv_lanes("c128i", set=144)
+        a2, b2, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+        np.testing.assert_allclose(a1, a2, rtol=1e-12)
+        np.testing.assert_allclose(b1, b2, rtol=1e-12, atol=1e-12 * np.abs(a2).max())
+    # matrices outside the form: none is made, and the schedule refuses them
+    Aref = core.build_ham_n(bn, nv, nc, base.hop_n, base.coul_n)      # reference layout
+    Aref.prepare("c128i")
+    assert Aref.factored() is None
+    Aref.spmv_lanes("c128i", set=206)
+    with pytest.raises(Exception, match="slot-factored form"):
+        Aref.matvec(_to_dev(core, np.ones(Aref.n, np.complex128)), precision="c128i")
+
+
+def _read_eigvec(path, ndim):
+    """eigvec.N: Fortran unformatted records (real(8) eigenvalue; complex(8) vector)."""
+    import struct
+    raw = open(path, "rb").read()
+    e = struct.unpack("<d", raw[4:12])[0]
+    v = np.frombuffer(raw[16 + 4:16 + 4 + 16 * ndim], dtype=np.complex128).copy()
+    return e, v
+
+
+def test_opavg_of_an_operator_other_than_h_matches_oracle(core, tmp_path):
+    """opavg_fsolver (src/opavg_driver.f90) with O != H: a non-Hermitian set of
+    one- and two-body terms in hopping_i.in / coulomb_i.in, averaged over the
+    two eigenvectors an ED run wrote; opavg.dat against the oracle, line by line
+    in the (1i5, 4f20.10) format of src/io.f90:638-659."""
+    import re
+    from edrixs_amd import fedrixs, iostream, siam, solvers
+    m = H.small_siam(nbath=2, seed=11)
+    wd = str(tmp_path)
+    solvers.ed_siam(m, wd, neval=2, nvector=2, eigval_tol=1e-12, min_ndim=10)
+    rng = np.random.default_rng(21)
+    n = m.norb
+    emat = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * (rng.random((n, n)) < 0.3)
+    umat = np.zeros((n,) * 4, np.complex128)
+    for _ in range(60):
+        i, j, k, l = rng.integers(0, n, size=4)
+        if i != j and k != l:
+            umat[i, j, k, l] += rng.normal() + 1j * rng.normal()
+    et, ut = siam.emat_terms(emat), siam.umat_terms(umat)
+    fi = H.fock_words(n, m.nocc)
+    O = o.build_ham_i(fi, et, ut)
+    cwd = os.getcwd()
+    os.chdir(wd)
+    try:
+        iostream.write_terms(et, "hopping_i.in")
+        iostream.write_terms(ut, "coulomb_i.in")
+        iostream.write_config(num_val_orbs=n, num_gs=2)
+        fedrixs.opavg_fsolver(0, 0, 1)
+        lines = open("opavg.dat").read().splitlines()
+        vecs = [_read_eigvec("eigvec.%d" % k, len(fi)) for k in (1, 2)]
+    finally:
+        os.chdir(cwd)
+    assert len(lines) == 2
+    ref = [o.opavg(O, v) for _, v in vecs]
+    assert all(abs(a.imag) > 1e-3 for a in ref)       # a genuinely complex average
+    want = o.opavg_lines([e for e, _ in vecs], ref)
+    for got, exp in zip(lines, want):
+        assert re.fullmatch(r"[ \d]{4}\d( {0,19}-?\d+\.\d{10}){3}", got) and len(got) == 65, got
+        assert got[:5] == exp[:5]
+        g, x = np.array(got[5:].split(), float), np.array(exp[5:].split(), float)
+        np.testing.assert_allclose(g, x, atol=2e-10)
+
+
+def test_two_ground_states_through_the_xas_and_rixs_drivers(core, tmp_path):
+    """num_gs = 2 over a degenerate ground level (an odd electron count with
+    time-reversal symmetric terms: a Kramers doublet).  Each eigenvector of the
+    doublet is arbitrary within it, but the Boltzmann-weighted sum of the two
+    spectra (edrixs/plot_spectrum.py:43-60) is not: XAS and RIXS through the
+    file drivers against dense Lorentzian sums over the oracle's operators."""
+    from edrixs_amd import solvers
+    m = H.small_siam(nbath=2, seed=11, nocc=5)         # norb 12, N 5: dims 792 / 6*924
+    wd = str(tmp_path)
+    evals, _ = solvers.ed_siam(m, wd, neval=2, nvector=2, eigval_tol=1e-12, min_ndim=10)
+    fi, fn = H.fock_words(m.norb, m.nocc), H.fock_words(m.norb, m.nocc + 1)
+    Hi = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    e, V = np.linalg.eigh(Hi)
+    assert e[1] - e[0] < 1e-9 and e[2] - e[1] > 1e-3
+    np.testing.assert_allclose(evals, e[:2], atol=2e-10)
+    Hn = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n).todense()
+    en, Vn = np.linalg.eigh(Hn)
+    T = 300.0
+    om = np.linspace(-10, 20, 400)
+    xas, _ = solvers.xas_siam(m, om, wd, gamma_c=0.5, num_gs=2, nkryl=3000, temperature=T)
+    ref = np.zeros(len(om))
+    for k in range(3):
+        D = o.build_transop_i(fn, fi, m.norb, m.ncore, m.trans_i[k]).todense()
+        for g in range(2):                              # equal weights: a degenerate level
+            amp = np.abs(Vn.conj().T @ (D @ V[:, g])) ** 2
+            ref += 0.5 * (-1 / np.pi) * np.imag(
+                (amp[None, :] / (om[:, None] + 0.5j + e[0] - en[None, :])).sum(1))
+    ref /= 3
+    assert np.linalg.norm(xas - ref) / np.linalg.norm(ref) < 1e-6
+    # RIXS: one incident energy, one polarization pair, both ground states
+    w_in, el = 2.0, np.linspace(-1, 10, 300)
+    pols = [([1, 0, 0], [0, 1, 0])]
+    rixs, _ = solvers.rixs_siam(m, [w_in], el, pols, wd, gamma_c=0.5, gamma_f=0.1, num_gs=2,
+                                nkryl=2000, linsys_tol=1e-12, linsys_max=4000, temperature=T)
+    ti, _ = m.pol_trans([1, 0, 0])
+    _, tf = m.pol_trans([0, 1, 0])
+    Di = o.build_transop_i(fn, fi, m.norb, m.ncore, ti).todense()
+    Df = o.build_transop_f(fi, fn, m.norb, m.ncore, tf).todense()
+    rref = np.zeros(len(el))
+    for g in range(2):
+        x = np.linalg.solve((w_in + e[0] + 0.5j) * np.eye(len(en)) - Hn, Di @ V[:, g])
+        amp = np.abs(V.conj().T @ (Df @ x)) ** 2
+        rref += 0.5 * (-1 / np.pi) * np.imag(
+            (amp[None, :] / (el[:, None] + 0.1j + e[0] - e[None, :])).sum(1))
+    got = rixs[0, :, 0]
+    assert np.linalg.norm(got - rref) / np.linalg.norm(rref) < 1e-6

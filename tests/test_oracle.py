@@ -299,3 +299,32 @@ def test_cf_single_pole_closed_form():
     om = np.linspace(-3, 3, 11)
     s = o.cf_spectrum([0.4], [0.0], 2.0, 0.1, om, 0.3)
     np.testing.assert_allclose(s, -1 / np.pi * np.imag(2.0 / (om + 0.3j + 0.1 - 0.4)), rtol=1e-14)
+
+
+def test_opavg_equals_dense_expectation():
+    # <v|O|v> for a non-Hermitian O (random one- and two-body terms without
+    # the Hermitian partners) and a complex v: conjugating the wrong vector or
+    # applying O^T instead of O changes the value
+    rng = np.random.default_rng(12)
+    n, N = 7, 3
+    from edrixs_amd import siam
+    emat = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * (rng.random((n, n)) < 0.5)
+    umat = np.zeros((n,) * 4, np.complex128)
+    for _ in range(40):
+        i, j, k, l = rng.integers(0, n, size=4)
+        if i != j and k != l:
+            umat[i, j, k, l] += rng.normal() + 1j * rng.normal()
+    et, ut = siam.emat_terms(emat), siam.umat_terms(umat)
+    fock = H.fock_words(n, N)
+    A = o.build_ham_i(fock, et, ut)
+    Od = o.dense_operator(n, _terms_to_emat(et, n), _terms_to_umat(ut, n))[np.ix_(fock, fock)]
+    assert not np.allclose(Od, Od.conj().T)
+    v = rng.normal(size=A.m) + 1j * rng.normal(size=A.m)
+    ref = np.vdot(v, Od @ v)
+    got = o.opavg(A, v)
+    assert abs(got - ref) < 1e-12 * abs(ref) and abs(got.imag) > 1e-3
+    assert abs(got - np.vdot(v, Od.T @ v)) > 1e-6 and abs(got - np.vdot(v.conj(), Od @ v)) > 1e-6
+    # the file lines: (1i5, 4f20.10) with three values per record
+    lines = o.opavg_lines([-1.5, 2.25], [got, 1.0 + 0.0j])
+    assert lines[0] == "%5d%20.10f%20.10f%20.10f" % (1, -1.5, got.real, got.imag)
+    assert len(lines[1]) == 65 and lines[1].startswith("    2")
