@@ -1,10 +1,13 @@
 """Serving loop: mixed XAS / RIXS requests on one GPU with continuous batching.
 
 Each request is an impurity model of its own size and parameters.  Requests
-arrive over time; a request is admitted as soon as the HBM it will hold (its
-CSR matrices and Krylov vectors, estimated from its dimensions) fits in the
-budget next to the requests already running, and leaves as soon as its
-spectra have converged, freeing its memory for the next one.  Every Krylov
+arrive over time; a request is admitted as soon as the HBM it will hold fits
+in the budget next to what the device holds now, and leaves as soon as its
+spectra have converged, freeing its memory for the next one.  What the device
+holds is measured, not estimated: the library's own allocations
+(``edx_mem_in_use``) plus torch's.  What a new request will hold is estimated
+from its dimensions, scaled by what earlier requests of the same size class
+were measured to take (the bytes the device gained over their setup).  Every Krylov
 recurrence of every running request (the three XAS dipole components, the
 RIXS final states of each (incident energy, polarization)) is a member of
 the :class:`core.KrylovGroup` of its request's size class (norb): a step of
@@ -109,7 +112,9 @@ class Engine:
         self.max_wait_s = max_wait_s
         self.groups = {}        # norb -> KrylovGroup of that size class
         self.step_s = {}        # norb -> measured seconds per group step
-        self.used = 0.0
+        self.used = 0.0         # bytes the running requests were measured to take
+        self.scale = {}         # norb -> measured / estimated bytes of past setups
+        self.device_bytes = self._device_bytes
 
     def group_for(self, norb):
         if norb not in self.groups:
@@ -130,6 +135,31 @@ class Engine:
         vecs = 16.0 * (2 * 3 * n_n if req.kind == "xas"
                        else 2 * self.n_in * self.npol * n_i + 6 * n_n + 2 * self.n_in * n_i)
         return mats + vecs
+
+    def need_bytes(self, req):
+        """Estimated bytes of a request, scaled by the measured/estimated ratio
+        of earlier requests of its size class (1 until one was measured)."""
+        return self.estimate_bytes(req) * self.scale.get(req.norb, 1.0)
+
+    @staticmethod
+    def _device_bytes():
+        """Device memory in use now: the library's allocations plus torch's."""
+        import torch
+        return float(core.mem_in_use()) + float(torch.cuda.memory_allocated())
+
+    def pick(self, queue, running, now, in_use):
+        """The queued request to admit now, or None.  The oldest request that
+        has waited longer than max_wait_s goes first, else the smallest; it is
+        admitted if nothing runs or if in_use plus its need fits the budget
+        (a request larger than the whole budget thus runs alone)."""
+        if not queue:
+            return None
+        old = [r for r in queue if now - r.arrival > self.max_wait_s]
+        req = min(old, key=lambda r: r.arrival) if old else \
+            min(queue, key=lambda r: (r.norb, r.arrival))
+        if running == 0 or in_use + self.need_bytes(req) <= self.budget:
+            return req
+        return None
 
     def _model(self, req):
         f2 = req.F2
@@ -238,26 +268,22 @@ class Engine:
             t = now()
             while pending and (not realtime or pending[0].arrival <= t):
                 queue.append(pending.pop(0))
-            # admission against the HBM budget, one setup per round so running
-            # requests keep stepping between setups; the smallest queued request
-            # goes first unless one has waited longer than max_wait_s
-            if queue:
-                t = now()
-                old = [r for r in queue if t - r.arrival > self.max_wait_s]
-                req = min(old, key=lambda r: r.arrival) if old else \
-                    min(queue, key=lambda r: (r.norb, r.arrival))
-                need = self.estimate_bytes(req)
-                if not active or self.used + need <= self.budget:
-                    queue.remove(req)
-                else:
-                    req = None
-            else:
-                req = None
+            # admission against the HBM budget by the bytes the device holds
+            # now, one setup per round so running requests keep stepping
+            # between setups
+            req = self.pick(queue, len(active), now(), self.device_bytes()) if queue else None
             if req is not None:
+                queue.remove(req)
                 req.admitted = now()
+                before = self.device_bytes()
                 st = self.setup(req)
-                st["bytes"] = need
-                self.used += need
+                torch.cuda.synchronize()
+                took = max(0.0, self.device_bytes() - before)
+                est = self.estimate_bytes(req)
+                if est > 0 and took > 0:
+                    self.scale[req.norb] = max(self.scale.get(req.norb, 0.0), took / est)
+                st["bytes"] = took
+                self.used += took
                 active.append(st)
                 if log:
                     log("admit r%d %s norb=%d at %.2f s (%d running)" %

@@ -74,3 +74,29 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stderr and not r.stdout.strip()
+
+
+def test_serving_admission_by_measured_bytes():
+    # the admission decision alone (no device): a request is admitted when the
+    # bytes the device holds now plus its scaled estimate fit the budget, or
+    # when nothing runs; the oldest request past max_wait_s goes first
+    from edrixs_amd import serving
+    eng = serving.Engine(budget_gb=10.0, max_wait_s=5.0)
+    reqs = serving.make_requests(3, rate=1.0, sizes=(16,), seed=1)
+    for r, t, n in zip(reqs, (0.0, 1.0, 2.0), (20, 16, 18)):
+        r.arrival, r.norb = t, n
+    need16 = eng.need_bytes(reqs[1])
+    assert 0 < need16 < 10e9
+    # smallest first while nobody waited long
+    assert eng.pick(reqs, running=1, now=3.0, in_use=0.0) is reqs[1]
+    # over budget with others running: wait; alone: admitted anyway
+    full = 10e9 - 0.5 * need16
+    assert eng.pick(reqs, running=1, now=3.0, in_use=full) is None
+    assert eng.pick(reqs, running=0, now=3.0, in_use=full) is reqs[1]
+    # a measured size class scales the estimate
+    eng.scale[16] = 3.0
+    assert eng.need_bytes(reqs[1]) == 3.0 * eng.estimate_bytes(reqs[1])
+    assert eng.pick(reqs, running=1, now=3.0, in_use=10e9 - 2.0 * need16) is None
+    # the oldest past max_wait_s goes first, even if larger
+    assert eng.pick(reqs, running=0, now=5.5, in_use=0.0) is reqs[0]
+    assert eng.pick([], running=0, now=0.0, in_use=0.0) is None
