@@ -365,18 +365,27 @@ def block_lanczos_eigs(H, nev, maxblocks=60, tol=1e-10, seed=1, nvec=None, preci
 
 
 def ground_state(H, nev=1, precision="c128", maxiter=500, tol=1e-10, seed=1, deflate=False,
-                 refine_maxiter=200):
-    """Lowest eigenpairs.  With precision 'c64' or 'c64d' the Lanczos products
-    round values to complex64 and the result is finished by an FP64 Lanczos
-    started from those eigenvectors; 'c128' and 'c128i' are exact and need no
-    refinement.  One state without deflation takes the reference's two-pass
-    Lanczos (no stored basis); several states, or deflated restarts, the
-    fully reorthogonalized solver.  Returns (evals, evecs, iterations)."""
+                 refine_maxiter=200, solver="lanczos", ncv=None):
+    """Lowest eigenpairs.  With precision 'c64' or 'c64d' the products round
+    values to complex64 and the result is finished by an FP64 Lanczos started
+    from those eigenvectors; 'c128' and 'c128i' are exact and need no
+    refinement.  solver 'lanczos': one state without deflation takes the
+    reference's two-pass Lanczos (no stored basis), several states or deflated
+    restarts the fully reorthogonalized solver; 'trlan': thick-restart Lanczos
+    over a basis of ncv (default 2 nev + 16) vectors, maxiter restarts, which
+    finds several states with the fewest reorthogonalization flops.  Returns
+    (evals, evecs, products with H or Lanczos steps)."""
+    def lowest(p, t):
+        if solver == "trlan":
+            ev, v, nmv, _ = trlan_eigs(H, nev, ncv=ncv if ncv else 2 * nev + 16, maxiter=maxiter,
+                                       tol=t, seed=seed, deflate=deflate, nvec=nev, precision=p)
+            return ev, v, nmv
+        return lanczos_eigs(H, nev, maxiter, t, seed, deflate, nev, p)
     if _prec(precision) in (EDX_C128, EDX_C128I):   # exact products: no refinement
-        if nev == 1 and not deflate:
+        if nev == 1 and not deflate and solver == "lanczos":
             return lanczos_ground(H, 1, maxiter, tol, seed, 1, precision)
-        return lanczos_eigs(H, nev, maxiter, tol, seed, deflate, nev, precision)
-    ev, vecs, it = lanczos_eigs(H, nev, maxiter, max(tol, 1e-6), seed, deflate, nev, precision)
+        return lowest(precision, tol)
+    ev, vecs, it = lowest(precision, max(tol, 1e-6))
     ev2, vecs2, it2 = lanczos_refine(H, vecs, nev, refine_maxiter, tol, nev, "c128")
     return ev2, vecs2, it + it2
 

@@ -155,7 +155,7 @@ int tune_spmv(edx_csr *a, int precision) {
     }
     prof_saved = g_prof;   // tuning launches stay out of the live profile
     g_prof = false;
-    for (int G : {4, 8, 16, 32, 64, 143, 144, 145, 184, FAC_CODE, FAC_CODE + 1, FAC_CODE + 2}) {
+    for (int G : {4, 8, 16, 32, 64, 143, 144, 145, 184, FAC_CODE}) {
         if (!schedule_available(a, precision, G)) continue;
         // the epilogue of a Krylov step: y = A x + z, with the dot <d|y>
         SpmvEpi e{};
@@ -291,10 +291,10 @@ int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes) {
     EDX_CHECK(precision >= EDX_C128 && precision <= EDX_C128I, EDX_ERR_ARG, "bad precision %d",
               precision);
     EDX_CHECK(set == 0 || set == 4 || set == 8 || set == 16 || set == 32 || set == 64 ||
-                  edx::is_pipe_code(set) || edx::is_fac_code(set),
+                  edx::is_pipe_code(set) || set == edx::FAC_CODE,
               EDX_ERR_ARG,
               "SpMV schedule must be 4, 8, 16, 32 or 64 lanes per row, the pipelined 143, 144, "
-              "145 (packed indexed form) or 184, or 206-208 (slot-factored form) (got %d)", set);
+              "145 (packed indexed form) or 184, or 206 (slot-factored form) (got %d)", set);
     int *slot = lanes_slot(a, precision);
     if (set) *slot = set;
     if (lanes) *lanes = *slot;

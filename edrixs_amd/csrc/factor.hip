@@ -100,14 +100,61 @@ struct FacArgs {
     const double2 *diag;     // exact diagonal, m entries
 };
 
+// Off-diagonal sum of row r = v NC + s in the slot-factored form: the H_v
+// entries of valence row v gathered at slot s, then the row's remainder R.
+// The first UR words of the remainder row are loaded into registers before
+// H_v is walked, so those loads overlap the valence gathers; H_v is walked
+// four entries at a time (four independent gathers in flight).  s_tab: the
+// value table staged in LDS.
+template <int NC, int UR>
+__device__ __forceinline__ double2 fac_offdiag_sum(const FacArgs &f, const double2 *__restrict__ x,
+                                                   const double2 *s_tab, int64_t v, int s,
+                                                   int64_t r) {
+    const int64_t jb = f.rrow[r], je = f.rrow[r + 1];
+    uint32_t rw[UR];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) rw[u] = jb + u < je ? f.rpk[jb + u] : 0u;
+    const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
+    double sr = 0.0, si = 0.0;
+    auto fma_c = [&](double2 a, double2 xv) {
+        sr += a.x * xv.x - a.y * xv.y;
+        si += a.x * xv.y + a.y * xv.x;
+    };
+    int64_t k = kb;
+    for (; k + 4 <= ke; k += 4) {
+        const uint32_t w0 = f.vpk[k], w1 = f.vpk[k + 1], w2 = f.vpk[k + 2], w3 = f.vpk[k + 3];
+        const double2 x0 = x[(int64_t)(w0 & FAC_COL_MASK) * NC + s];
+        const double2 x1 = x[(int64_t)(w1 & FAC_COL_MASK) * NC + s];
+        const double2 x2 = x[(int64_t)(w2 & FAC_COL_MASK) * NC + s];
+        const double2 x3 = x[(int64_t)(w3 & FAC_COL_MASK) * NC + s];
+        fma_c(s_tab[w0 >> PK_COL_BITS], x0);
+        fma_c(s_tab[w1 >> PK_COL_BITS], x1);
+        fma_c(s_tab[w2 >> PK_COL_BITS], x2);
+        fma_c(s_tab[w3 >> PK_COL_BITS], x3);
+    }
+    for (; k < ke; ++k) {
+        const uint32_t w = f.vpk[k];
+        fma_c(s_tab[w >> PK_COL_BITS], x[(int64_t)(w & FAC_COL_MASK) * NC + s]);
+    }
+#pragma unroll
+    for (int u = 0; u < UR; ++u)
+        if (jb + u < je) fma_c(s_tab[rw[u] >> PK_COL_BITS], x[rw[u] & FAC_COL_MASK]);
+    for (int64_t j = jb + UR; j < je; ++j) {
+        const uint32_t w = f.rpk[j];
+        fma_c(s_tab[w >> PK_COL_BITS], x[w & FAC_COL_MASK]);
+    }
+    return make_double2(sr, si);
+}
+
 // y = c0 * (H x) + c1 * z (+ <d|y> block partials) in the slot-factored form.
 // Lane l of a wave owns valence row v = wave_base + l / NC and slot s = l % NC
 // (64 / NC valence rows per wave, 64 % NC lanes idle), so the NC lanes of a
 // valence row read each H_v entry's x[k NC + s] as one contiguous NC * 16-byte
-// segment, and the rows a wave writes are consecutive.  Each lane loads the
-// first UR words of its remainder row into registers before it walks H_v, so
-// those loads overlap the valence gathers; H_v is walked four entries at a
-// time (four independent gathers in flight).
+// segment, and the rows a wave writes are consecutive.  The epilogue's
+// operands (diagonal, x[r], addend, dot vector) are loaded before the sums so
+// they arrive with the first gathers.  One grid pass, no stride: workgroups
+// are dispatched in row order, so the rows in flight stay in one window whose
+// gathers share cache lines.
 template <int NC, int UR, int WPE>
 __global__ __launch_bounds__(256, WPE) void k_spmv_fac(FacArgs f, const double2 *__restrict__ x,
                                                        SpmvEpi e) {
@@ -121,135 +168,20 @@ __global__ __launch_bounds__(256, WPE) void k_spmv_fac(FacArgs f, const double2 
     const int64_t bid = e.xmap ? (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
                                : (int64_t)blockIdx.x;
     const int64_t v = (bid * 4 + wave) * VPW + vl;
-    const bool act = vl < VPW && v < f.mv;
     double2 dot = make_double2(0.0, 0.0);
-    if (act) {
+    if (vl < VPW && v < f.mv) {
         const int64_t r = v * NC + s;
-        const int64_t jb = f.rrow[r], je = f.rrow[r + 1];
-        uint32_t rw[UR];
-#pragma unroll
-        for (int u = 0; u < UR; ++u) rw[u] = jb + u < je ? f.rpk[jb + u] : 0u;
-        const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
-        const double2 dg = f.diag[r];
-        const double2 xr = x[r];
+        const double2 dg = f.diag[r], xr = x[r];
         double2 pz = make_double2(0, 0), pd = pz;
         if (e.z) pz = e.z[r];
         if (e.dvec) pd = e.dvec[r];
-        double sr = 0.0, si = 0.0;
-        auto fma_c = [&](double2 a, double2 xv) {
-            sr += a.x * xv.x - a.y * xv.y;
-            si += a.x * xv.y + a.y * xv.x;
-        };
-        int64_t k = kb;
-        for (; k + 4 <= ke; k += 4) {
-            const uint32_t w0 = f.vpk[k], w1 = f.vpk[k + 1], w2 = f.vpk[k + 2], w3 = f.vpk[k + 3];
-            const double2 x0 = x[(int64_t)(w0 & FAC_COL_MASK) * NC + s];
-            const double2 x1 = x[(int64_t)(w1 & FAC_COL_MASK) * NC + s];
-            const double2 x2 = x[(int64_t)(w2 & FAC_COL_MASK) * NC + s];
-            const double2 x3 = x[(int64_t)(w3 & FAC_COL_MASK) * NC + s];
-            fma_c(s_tab[w0 >> PK_COL_BITS], x0);
-            fma_c(s_tab[w1 >> PK_COL_BITS], x1);
-            fma_c(s_tab[w2 >> PK_COL_BITS], x2);
-            fma_c(s_tab[w3 >> PK_COL_BITS], x3);
-        }
-        for (; k < ke; ++k) {
-            const uint32_t w = f.vpk[k];
-            fma_c(s_tab[w >> PK_COL_BITS], x[(int64_t)(w & FAC_COL_MASK) * NC + s]);
-        }
-#pragma unroll
-        for (int u = 0; u < UR; ++u)
-            if (jb + u < je) fma_c(s_tab[rw[u] >> PK_COL_BITS], x[rw[u] & FAC_COL_MASK]);
-        for (int64_t j = jb + UR; j < je; ++j) {
-            const uint32_t w = f.rpk[j];
-            fma_c(s_tab[w >> PK_COL_BITS], x[w & FAC_COL_MASK]);
-        }
-        fma_c(dg, xr);
+        const double2 ax = cadd(fac_offdiag_sum<NC, UR>(f, x, s_tab, v, s, r), cmul(dg, xr));
         const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
         const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
-        double2 out = cmul(c0, make_double2(sr, si));
+        double2 out = cmul(c0, ax);
         if (e.z) out = cadd(out, cmul(c1, pz));
         e.y[r] = out;
         if (e.dvec) dot = cconjmul(pd, out);
-    }
-    if (e.dvec) {
-        const double2 bs = block_sum_256(dot);
-        if (threadIdx.x == 0) e.partials[blockIdx.x] = bs;
-    }
-}
-
-// The same product with more loads in flight per lane: every lane first loads
-// up to UV words of its valence row's H_v entries and up to UR words of its
-// remainder row (all independent), then issues the gathers of the first eight
-// H_v entries together with its remainder row's, then the next H_v entries,
-// so a row costs about three memory latencies instead of one per group of four
-// entries.  Entries beyond UV (H_v) or UR (R) are read in place.
-template <int NC, int UV, int UR, int WPE>
-__global__ __launch_bounds__(256, WPE) void k_spmv_fac_b(FacArgs f, const double2 *__restrict__ x,
-                                                         SpmvEpi e) {
-    if (e.run && *e.run == 0) return;
-    extern __shared__ double2 s_tab[];
-    for (int i = threadIdx.x; i < f.ntab; i += 256) s_tab[i] = f.tab[i];
-    __syncthreads();
-    constexpr int VPW = 64 / NC;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int vl = lane / NC, s = lane - vl * NC;
-    const int64_t bid = e.xmap ? (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
-                               : (int64_t)blockIdx.x;
-    const int64_t v = (bid * 4 + wave) * VPW + vl;
-    const bool act = vl < VPW && v < f.mv;
-    double2 dot = make_double2(0.0, 0.0);
-    if (act) {
-        const int64_t r = v * NC + s;
-        const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
-        const int64_t jb = f.rrow[r], je = f.rrow[r + 1];
-        const int nv = (int)(ke - kb), nr = (int)(je - jb);
-        uint32_t vw[UV], rw[UR];
-#pragma unroll
-        for (int u = 0; u < UV; ++u) vw[u] = u < nv ? f.vpk[kb + u] : 0u;
-#pragma unroll
-        for (int u = 0; u < UR; ++u) rw[u] = u < nr ? f.rpk[jb + u] : 0u;
-        double sr = 0.0, si = 0.0;
-        auto fma_c = [&](double2 a, double2 xv) {
-            sr += a.x * xv.x - a.y * xv.y;
-            si += a.x * xv.y + a.y * xv.x;
-        };
-        constexpr int B1 = UV < 8 ? UV : 8;
-        double2 xa[B1], xb[UR];
-#pragma unroll
-        for (int u = 0; u < B1; ++u)
-            xa[u] = u < nv ? x[(int64_t)(vw[u] & FAC_COL_MASK) * NC + s] : make_double2(0, 0);
-#pragma unroll
-        for (int u = 0; u < UR; ++u) xb[u] = u < nr ? x[rw[u] & FAC_COL_MASK] : make_double2(0, 0);
-#pragma unroll
-        for (int u = 0; u < B1; ++u)
-            if (u < nv) fma_c(s_tab[vw[u] >> PK_COL_BITS], xa[u]);
-#pragma unroll
-        for (int u = 0; u < UR; ++u)
-            if (u < nr) fma_c(s_tab[rw[u] >> PK_COL_BITS], xb[u]);
-        if (UV > B1 && nv > B1) {
-            double2 xc[UV > B1 ? UV - B1 : 1];
-#pragma unroll
-            for (int u = B1; u < UV; ++u)
-                xc[u - B1] = u < nv ? x[(int64_t)(vw[u] & FAC_COL_MASK) * NC + s] : make_double2(0, 0);
-#pragma unroll
-            for (int u = B1; u < UV; ++u)
-                if (u < nv) fma_c(s_tab[vw[u] >> PK_COL_BITS], xc[u - B1]);
-        }
-        for (int64_t k = kb + UV; k < ke; ++k) {
-            const uint32_t w = f.vpk[k];
-            fma_c(s_tab[w >> PK_COL_BITS], x[(int64_t)(w & FAC_COL_MASK) * NC + s]);
-        }
-        for (int64_t j = jb + UR; j < je; ++j) {
-            const uint32_t w = f.rpk[j];
-            fma_c(s_tab[w >> PK_COL_BITS], x[w & FAC_COL_MASK]);
-        }
-        fma_c(f.diag[r], x[r]);
-        const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
-        const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
-        double2 out = cmul(c0, make_double2(sr, si));
-        if (e.z) out = cadd(out, cmul(c1, e.z[r]));
-        e.y[r] = out;
-        if (e.dvec) dot = cconjmul(e.dvec[r], out);
     }
     if (e.dvec) {
         const double2 bs = block_sum_256(dot);
@@ -272,10 +204,8 @@ int fac_blocks(const edx_csr *a) {
     return (int)blocks;
 }
 
-// Launch of the factored product (schedules FAC_CODE and the batched-load
-// variants FAC_CODE + 1, + 2); returns the block count.
-int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e_in, int variant,
-               int *nblocks_out) {
+// Launch of the factored product (schedule FAC_CODE); returns the block count.
+int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e_in, int *nblocks_out) {
     if (!a->d_fr_rowptr)
         return set_error(EDX_ERR_ARG, "SpMV schedule %d needs the slot-factored form", FAC_CODE);
     const int nb = fac_blocks(a);
@@ -289,12 +219,7 @@ int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e_in, int vari
         return set_error(EDX_ERR_ARG, "slot-factored product for %d core slots not built",
                          a->nc_slots);
     const size_t lds = (size_t)a->nvtab * sizeof(double2);
-    if (variant == 0)
-        hipLaunchKernelGGL((k_spmv_fac<6, 8, 4>), dim3(nb), dim3(256), lds, stream(), f, x, e);
-    else if (variant == 1)
-        hipLaunchKernelGGL((k_spmv_fac_b<6, 16, 8, 2>), dim3(nb), dim3(256), lds, stream(), f, x, e);
-    else
-        hipLaunchKernelGGL((k_spmv_fac_b<6, 12, 8, 4>), dim3(nb), dim3(256), lds, stream(), f, x, e);
+    hipLaunchKernelGGL((k_spmv_fac<6, 8, 4>), dim3(nb), dim3(256), lds, stream(), f, x, e);
     EDX_LAUNCH_CHECK();
     return EDX_OK;
 }

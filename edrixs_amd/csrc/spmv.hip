@@ -526,16 +526,14 @@ int pipe_blocks(int bpc) { return (num_cus() * bpc + 7) / 8 * 8; }
 // Schedule FAC_CODE: the slot-factored product of a core-hole matrix in the
 // interleaved layout (factor.hip), indexed form only.
 constexpr int FAC_CODE = 206;
-bool is_fac_code(int code) { return code >= FAC_CODE && code <= FAC_CODE + 2; }
-int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e, int variant,
-               int *nblocks_out);
+int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e, int *nblocks_out);
 int fac_blocks(const edx_csr *a);
 double fac_bytes(const edx_csr *a, bool z, bool dvec);
 
 // Whether schedule code G can run on this matrix in this precision.
 bool schedule_available(const edx_csr *a, int precision, int G) {
     if (G == 145) return precision == EDX_C128I && a->d_off_pk;
-    if (is_fac_code(G)) return precision == EDX_C128I && a->d_fr_rowptr;
+    if (G == FAC_CODE) return precision == EDX_C128I && a->d_fr_rowptr;
     return true;
 }
 
@@ -573,13 +571,13 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
         else if (precision == EDX_C128I) G = a->spmv_lanes128i ? a->spmv_lanes128i : lanes_for_split(a);
         else G = a->spmv_lanes ? a->spmv_lanes : lanes_for(a);
     }
-    if (is_fac_code(G)) {
+    if (G == FAC_CODE) {
         if (!schedule_available(a, precision, G))
             return set_error(EDX_ERR_ARG, "SpMV schedule %d needs the slot-factored form", G);
         int nb = 0;
         hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
                                    fac_bytes(a, e_in.z != nullptr, e_in.dvec != nullptr));
-        EDX_TRY(fac_launch(a, (const double2 *)x, e_in, G - FAC_CODE, &nb));
+        EDX_TRY(fac_launch(a, (const double2 *)x, e_in, &nb));
         prof_stop(pe);
         if (nblocks_out) *nblocks_out = nb;
         if (e_in.dvec && nb > FOLD_N) {

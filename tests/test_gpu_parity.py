@@ -1444,13 +1444,15 @@ def test_opavg_of_an_operator_other_than_h_matches_oracle(core, tmp_path):
 
 
 def test_two_ground_states_through_the_xas_and_rixs_drivers(core, tmp_path):
-    """num_gs = 2 over a degenerate ground level (an odd electron count with
-    time-reversal symmetric terms: a Kramers doublet).  Each eigenvector of the
-    doublet is arbitrary within it, but the Boltzmann-weighted sum of the two
-    spectra (edrixs/plot_spectrum.py:43-60) is not: XAS and RIXS through the
-    file drivers against dense Lorentzian sums over the oracle's operators."""
+    """num_gs = 2 over a degenerate ground level: an atomic d7 shell with
+    spin-orbit coupling and a cubic field (no bath, time-reversal symmetric, an
+    odd electron count) has a Kramers doublet lowest, 0.053 eV below a quartet.
+    Each eigenvector of the doublet is arbitrary within it, but the
+    Boltzmann-weighted sum of the two spectra (edrixs/plot_spectrum.py:43-60)
+    is not: XAS and RIXS through the file drivers against dense Lorentzian sums
+    over the oracle's operators."""
     from edrixs_amd import solvers
-    m = H.small_siam(nbath=2, seed=11, nocc=5)         # norb 12, N 5: dims 792 / 6*924
+    m = H.small_siam(nbath=0, seed=11, nocc=7)         # norb 10, N 7: dims 120 / 6*45
     wd = str(tmp_path)
     evals, _ = solvers.ed_siam(m, wd, neval=2, nvector=2, eigval_tol=1e-12, min_ndim=10)
     fi, fn = H.fock_words(m.norb, m.nocc), H.fock_words(m.norb, m.nocc + 1)
@@ -1489,3 +1491,52 @@ def test_two_ground_states_through_the_xas_and_rixs_drivers(core, tmp_path):
             (amp[None, :] / (el[:, None] + 0.1j + e[0] - e[None, :])).sum(1))
     got = rixs[0, :, 0]
     assert np.linalg.norm(got - rref) / np.linalg.norm(rref) < 1e-6
+
+
+def test_group_factored_member_matches_oracle_and_is_mix_independent(core, monkeypatch):
+    # an interleaved-layout H_n in an indexed (c128i) group takes the
+    # slot-factored product; its (alpha, beta) against the oracle's recurrence
+    # from the same start vector (reference layout: the vector permuted), and
+    # the same bits alone, in a changing mix and without graph replay
+    m, fn, bn, _, R = _siam_hn(core, nbath=4, seed=2)
+    A = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout="interleaved")
+    A.prepare("c128i")
+    assert A.factored() is not None
+    p = core.corehole_permutation(len(fn), m.ncore)      # x_inter = x_ref[p]
+    m2 = H.small_siam(nbath=3, seed=5)
+    f2 = H.fock_words(m2.norb, m2.nocc)
+    A2 = core.build_ham_i(core.Basis.from_words(f2, m2.norb), m2.hop_i, m2.coul_i)
+    rng = np.random.default_rng(47)
+    vref = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+    va = _to_dev(core, vref[p])
+    vb = _to_dev(core, rng.normal(size=A2.n) + 1j * rng.normal(size=A2.n))
+    g1 = core.KrylovGroup("c128i", max_steps=200)
+    a = g1.add(A, va)
+    g1.advance(40)
+    al1, be1, n1, run1 = g1.coefficients(a)
+    g1.close()
+    assert len(al1) == 40 and run1
+    ra, rb, rn = o.build_krylov(R, vref, 40)
+    np.testing.assert_allclose(al1[:20], ra[:20], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(be1[:20], rb[:20], rtol=1e-9, atol=1e-9)
+    assert abs(n1 - rn) <= 1e-12 * rn
+
+    def mix():
+        g = core.KrylovGroup("c128i", max_steps=200)
+        b = g.add(A2, vb)
+        g.advance(5)
+        a2 = g.add(A, va)
+        c = g.add(A2, vb)
+        g.advance(17)
+        g.remove(b)
+        g.advance(23)
+        out = g.coefficients(a2), g.coefficients(c)
+        g.close()
+        return out
+
+    (al2, be2, n2, _), cg = mix()
+    assert np.array_equal(al2, al1) and np.array_equal(be2, be1) and n2 == n1
+    monkeypatch.setenv("EDX_NO_GRAPH", "1")
+    (al3, be3, n3, _), ce = mix()
+    assert np.array_equal(al3, al1) and np.array_equal(be3, be1) and n3 == n1
+    assert np.array_equal(ce[0], cg[0]) and np.array_equal(ce[1], cg[1])
