@@ -106,6 +106,8 @@ _SIGS = {
                                     ctypes.POINTER(_I), _P]),
     "edx_build_krylov": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
                               ctypes.POINTER(_D)]),
+    "edx_build_krylov_noalloc": (_I, [_P, _I, _P, _I, ctypes.POINTER(_I), _P, _P,
+                                      ctypes.POINTER(_D)]),
     "edx_minres": (_I, [_P, _I, _P, _P, _D, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "edx_cf_spectrum": (_I, [_I, _P, _P, _D, _D, _D, _I, _P, _P, _P]),
     "edx_denmat": (_I, [_P, _I, _P, _P]),

@@ -820,3 +820,71 @@ int edx_lanczos_refine(const edx_csr *h, int precision, int nev, int maxiter, do
 
 }  // extern "C"
 
+// ------------------------------------------------ the no-allocation variant
+//
+// edx_build_krylov_noalloc runs the recurrence of edx_build_krylov with its
+// device memory (vectors, partials, state, history, run flags) carved from
+// `scratch`, a file-scope device array of NOALLOC_SCRATCH_BYTES that every GPU
+// the library runs on holds until the process ends.  In host code `scratch`
+// names a host-side stand-in, not device memory: its device address is
+// resolved once per device with hipGetSymbolAddress (with that device
+// current) and only that address is handed to kernels and copies.  Calls on
+// one device share the array, so calls that overlap on one device race.
+
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_ptr;   // device -> device address of scratch
+
+int noalloc_scratch(char **out) {
+    const int dev = edx::current_device();
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        int cur = -1;
+        EDX_HIP(hipGetDevice(&cur));
+        EDX_CHECK(cur == dev, EDX_ERR_ARG, "device %d is not current (%d is)", dev, cur);
+        char *ptr = nullptr;
+        EDX_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+        EDX_CHECK(ptr, EDX_ERR_HIP, "no device address for the scratch array");
+        it = g_scratch_ptr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return EDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edx_build_krylov_noalloc(const edx_csr *h, int precision, const void *d_v0, int maxn,
+                             int *neff, double *alpha, double *beta, double *norm) {
+    EDX_CHECK(h && d_v0 && neff && alpha && beta && norm && maxn >= 1, EDX_ERR_ARG, "bad argument");
+    EDX_CHECK(h->m == h->n, EDX_ERR_ARG, "matrix is %lld x %lld, not square", (long long)h->m,
+              (long long)h->n);
+    // preparing a form (and tuning its schedule) allocates: it must come first
+    EDX_CHECK(precision == EDX_C128 || (precision == EDX_C64D && h->d_diag) ||
+                  (precision == EDX_C128I && h->d_off_idx),
+              EDX_ERR_ARG,
+              "precision %d is not prepared on this matrix (call edx_csr_prepare first; the "
+              "complex64-vector form EDX_C64 is not offered here)", precision);
+    const int64_t plen = edx::spmv_partials_len(h);
+    const size_t need = krylov_scratch_bytes(h->n, plen, maxn);
+    EDX_CHECK(need <= NOALLOC_SCRATCH_BYTES, EDX_ERR_CAPACITY,
+              "a Krylov run over %lld rows with maxn %d needs %zu bytes of scratch; the static "
+              "buffer holds %zu", (long long)h->n, maxn, need, NOALLOC_SCRATCH_BYTES);
+    char *base = nullptr;
+    EDX_TRY(noalloc_scratch(&base));
+    const KrylovScratch k = krylov_scratch_carve(base, h->n, plen, maxn);
+    auto prod = [&](const double2 *x, const SpmvEpi &e, int *nb, const int *) {
+        return edx::spmv_launch(h, precision, x, e, 0, nb);
+    };
+    EDX_TRY(krylov_loop_in(h->n, prod, k, (const double2 *)d_v0, maxn, neff, alpha, beta, norm));
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    return EDX_OK;
+}
+
+}  // extern "C"

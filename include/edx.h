@@ -269,6 +269,17 @@ int edx_block_lanczos_eigs(const edx_csr *h, int precision, int nev, int maxbloc
  * (beta_1..beta_neff as written to the pole files), norm = <v|v>. */
 int edx_build_krylov(const edx_csr *h, int precision, const void *d_v0, int maxn, int *neff,
                      double *alpha, double *beta, double *norm);
+/* edx_build_krylov without device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * the same results bit for bit; its vectors, partials and history live in a
+ * static 64 MiB device array per GPU (held until the process ends), which
+ * bounds a call to about 2 n + 2 maxn <= 4e6 complex128 values (requests
+ * beyond it fail with EDX_ERR_CAPACITY).  The precision's form must already be
+ * prepared (edx_csr_prepare); EDX_C64 is not offered.  Calls that overlap on
+ * one GPU share the array and race. */
+int edx_build_krylov_noalloc(const edx_csr *h, int precision, const void *d_v0, int maxn,
+                             int *neff, double *alpha, double *beta, double *norm);
 /* The same tridiagonalization with every new vector reorthogonalized against
  * all previous ones (two classical Gram-Schmidt passes): the exact Krylov
  * projection, against which the rounding of the plain recurrence is measured
