@@ -202,7 +202,7 @@ class Workload:
             chunk=a.cf_chunk, min_steps=a.nkryl, max_steps=a.nkryl_max)
         torch.cuda.synchronize()
         T["rixs_krylov"] = time.perf_counter() - t0
-        info = dict(nnz_i=Hi.nnz, nnz_n=Hn.nnz, lanczos_iters=it,
+        info = dict(nnz_i=Hi.nnz, nnz_n=Hn.nnz, m_n=Hn.m, fac_n=Hn.factored(), lanczos_iters=it,
                     rixs_iters=iters, rixs_res_rel_max=max(res),
                     rixs_krylov_steps=[int(v) for v in ksteps],
                     xas_krylov_steps=[int(v) for v in xsteps], xas_cf_change_max=float(max(xchange)),
@@ -880,6 +880,21 @@ def main():
         },
         "forms": forms_out,
     }
+    inf = results[main_form]["info"]
+    if kname == "k_spmv_fac" and inf.get("fac_n") and ms > 0:
+        # the same launches priced as the complex128 CSR the reference's
+        # matvec_csr streams (src/spmv.f90:73-103): 16-byte value and 4-byte
+        # column per nonzero (diagonal included), 8-byte row pointers, the same
+        # vector terms
+        mh, nc, hv, rr = inf["m_n"], *inf["fac_n"]
+        stored = (hv + rr) * 4.0 + (mh // nc + 1) * 8.0 + (mh + 1) * 8.0 + mh * 16.0
+        csr = inf["nnz_n"] * 20.0 + (mh + 1) * 8.0
+        cby = by + n * (csr - stored)
+        line["roofline"]["csr_equivalent"] = {
+            "achieved": cby / (ms * 1e-3) / 1e9, "unit": "GB/s", "bytes_per_launch": cby / max(n, 1),
+            "bytes_counted": ("the complex128 CSR of H_n (20 bytes per nonzero, row pointers) in "
+                              "place of the factored form's words, pointers and diagonal; the "
+                              "same vector terms")}
     if "c128" in results:
         line["speedup_vs_fp64"] = (times["c128"] / nsteps["c128"]) / (t_main / nsteps[main_form])
     if not args.no_cpu:

@@ -612,10 +612,17 @@ def converged_spectra(H, V0, e_gs, grid, gamma, precision="c128", tol=1e-8, chun
     down.  Returns (spectra, steps, last_change); a spectrum that never met tol
     by max_steps keeps its last value and reports its last change.  Fewer than
     8 vectors whose interleaved block exceeds SPLIT_BLOCK_BYTES advance as
-    separate single-vector recurrences."""
+    separate single-vector recurrences, except 2 to 4 vectors over a matrix
+    with the slot-factored form in the indexed precision, whose block product
+    gathers contiguous segments of all vectors."""
     V0 = V0.contiguous()
     K = V0.shape[0]
-    if K < 8 and K * H.n * 16 > SPLIT_BLOCK_BYTES:
+    # a block of up to 4 vectors over a matrix with the slot-factored form stays
+    # one block in the indexed precision (its products gather contiguous
+    # segments of all K vectors); otherwise blocks of fewer than 8 vectors too
+    # large for the last-level cache run as single-vector recurrences
+    fac_block = 2 <= K <= 4 and _prec(precision) == EDX_C128I and H.factored() is not None
+    if K < 8 and K * H.n * 16 > SPLIT_BLOCK_BYTES and not fac_block:
         groups = [[k] for k in range(K)]
     else:
         groups = [list(range(K))]
@@ -736,7 +743,7 @@ def profile(enable=True):
     _lib.call("edx_profile", int(bool(enable)))
 
 
-KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4}
+KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4, "k_spmv_fac": 7}
 BLOCK_KERNELS = {"k_bproj": 5, "k_bupd": 6}   # matrix-core kernels (recorded as c128)
 
 

@@ -218,6 +218,10 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
             if (bsched_blocks(a, BSched{2, bpc}) <= bspmm_blocks(a->m))
                 cand.push_back(BSched{2, bpc});
     }
+    if (fac_spmm_ok(a, precision, K))   // slot-factored, workgroups per CU
+        for (int bpc : {4, 8})
+            if (bsched_blocks(a, BSched{3, bpc}) <= bspmm_blocks(a->m))
+                cand.push_back(BSched{3, bpc});
     const size_t nk = (size_t)a->n * K;
     double2 *X = (double2 *)dmalloc(nk * sizeof(double2), "tune X");
     float2 *X32 = precision == EDX_C64 ? (float2 *)dmalloc(nk * sizeof(float2), "tune X32") : nullptr;

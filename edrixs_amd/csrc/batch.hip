@@ -800,12 +800,20 @@ __global__ void k_vt_renumber(int64_t nnz, const uint16_t *remap, uint16_t *idx)
 
 namespace edx {
 
-// schedule of a multi-RHS product: row-split (rl = 0, p lanes per row) or
-// RHS-parallel (rl = 1, tiles of 16 p right-hand sides)
+// schedule of a multi-RHS product: row-split (rl = 0, p lanes per row),
+// RHS-parallel (rl = 1, tiles of 16 p right-hand sides), pipelined
+// RHS-parallel (rl = 2, p workgroups per CU) or, for K <= 4 on a matrix with
+// the slot-factored form in the indexed precision, the factored product
+// (rl = 3, p workgroups per CU)
 struct BSched {
     int rl;
     int p;
 };
+int fac_spmm_launch(const edx_csr *a, int K, const double2 *X, const BEpi &e, int bpc);
+double fac_bytes_k(const edx_csr *a, int K, bool z, bool dvec);
+inline bool fac_spmm_ok(const edx_csr *a, int precision, int K) {
+    return precision == EDX_C128I && a->d_fr_rowptr && a->nc_slots == 6 && K >= 2 && K <= 4;
+}
 int tune_spmv(edx_csr *a, int precision);
 int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched);
 int bsched_blocks(const edx_csr *a, BSched sched);
@@ -822,19 +830,30 @@ void forced_bsched(int K, BSched *sched) {
     int rl = 0, p = 0;
     if (K >= 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && rl >= 0 && rl <= 2 && p > 0)
         *sched = BSched{rl, p};
+    if (K < 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && (rl == 0 || rl == 3) && p > 0)
+        *sched = BSched{rl, p};
 }
 
 // Workgroups (and dot partials per right-hand side) of a schedule's launch.
 // The pipelined schedule needs a multiple of 8 and at most bspmm_blocks(m),
 // the partials every caller provides.
 int bsched_blocks(const edx_csr *a, BSched sched) {
-    if (sched.rl == 2) return (edx::num_cus() * sched.p + 7) / 8 * 8;
+    if (sched.rl == 2 || sched.rl == 3) return (edx::num_cus() * sched.p + 7) / 8 * 8;
     return bspmm_blocks(a->m);
 }
 
 int bspmm_launch(const edx_csr *a, int precision, int K, const double2 *X64, const float2 *X32,
                  const BEpi &e, BSched sched) {
     const int nb = bspmm_blocks(a->m);
+    if (sched.rl == 3) {   // slot-factored, sched.p workgroups per CU
+        if (!fac_spmm_ok(a, precision, K))
+            return set_error(EDX_ERR_ARG, "the factored multi-RHS product needs the slot-factored "
+                             "form, the indexed precision and 2 <= K <= 4");
+        if (bsched_blocks(a, sched) > nb)
+            return set_error(EDX_ERR_ARG, "factored SpMM needs %d workgroups, %d available",
+                             bsched_blocks(a, sched), nb);
+        return fac_spmm_launch(a, K, X64, e, sched.p);
+    }
     if (sched.rl == 2) {   // pipelined RHS-parallel, sched.p workgroups per CU
         const int nbp = bsched_blocks(a, sched);
         if (nbp > nb)
@@ -957,9 +976,15 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
     BSched sched = bspmm_rule(a, K);
     if (a->m >= TUNE_MIN_ROWS) EDX_TRY(tune_bspmm(a, precision, K, &sched));
     forced_bsched(K, &sched);
+    if (sched.rl == 3 && !fac_spmm_ok(a, precision, K))
+        return set_error(EDX_ERR_ARG, "the factored multi-RHS product needs the slot-factored "
+                         "form, the indexed precision and 2 <= K <= 4");
     if (nblocks_out) *nblocks_out = bsched_blocks(a, sched);
-    hipEvent_t pe = prof_start(precision, sched.rl ? EDX_KERNEL_SPMM_RL : EDX_KERNEL_SPMM_IL,
-                               spmv_bytes_k(a, precision, K, e.z != nullptr, e.dvec != nullptr));
+    hipEvent_t pe = prof_start(precision, (sched.rl == 1 || sched.rl == 2) ? EDX_KERNEL_SPMM_RL
+                                                                           : EDX_KERNEL_SPMM_IL,
+                               sched.rl == 3 ? fac_bytes_k(a, K, e.z != nullptr, e.dvec != nullptr)
+                                             : spmv_bytes_k(a, precision, K, e.z != nullptr,
+                                                            e.dvec != nullptr));
     EDX_TRY(bspmm_launch(a, precision, K, X64, X32, e, sched));
     prof_stop(pe);
     return EDX_OK;

@@ -189,6 +189,87 @@ __global__ __launch_bounds__(256, WPE) void k_spmv_fac(FacArgs f, const double2 
     }
 }
 
+// Multi-RHS factored product for small K (interleaved right-hand sides,
+// X[r K + k], and the batched epilogue of batch.hip): lane (v, s) as in
+// k_spmv_fac computes its row for all K right-hand sides, so an H_v entry
+// gathers NC K 16-byte values contiguous over the valence row's lanes and an
+// R entry K contiguous values.  A capped grid (a multiple of 8 workgroups)
+// walks XCD-contiguous eighths of the valence rows, 4 (64 / NC) valence rows
+// per workgroup per pass; dot partials per workgroup and right-hand side.
+template <int NC, int K>
+__global__ __launch_bounds__(256) void k_spmm_fac(FacArgs f, const double2 *__restrict__ X, BEpi e) {
+    extern __shared__ double2 s_tab[];
+    for (int i = threadIdx.x; i < f.ntab; i += 256) s_tab[i] = f.tab[i];
+    __shared__ double2 red[4][K];
+    __syncthreads();
+    constexpr int VPW = 64 / NC;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int vl = lane / NC, s = lane - vl * NC;
+    const int64_t per_x = gridDim.x / 8, xcd = blockIdx.x % 8, wg = blockIdx.x / 8;
+    const int64_t vrows_x = (f.mv + 7) / 8;
+    const int64_t vbeg = xcd * vrows_x, vend = min(f.mv, vbeg + vrows_x);
+    double2 dot[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) dot[k] = make_double2(0, 0);
+    if (vl < VPW) {
+        for (int64_t v = vbeg + (wg * 4 + wave) * VPW + vl; v < vend; v += per_x * 4 * VPW) {
+            const int64_t r = v * NC + s;
+            double sr[K], si[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) sr[k] = si[k] = 0.0;
+            auto add = [&](double2 a, int64_t base) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double2 xv = X[base + k];
+                    sr[k] += a.x * xv.x - a.y * xv.y;
+                    si[k] += a.x * xv.y + a.y * xv.x;
+                }
+            };
+            const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
+            int64_t q = kb;
+            for (; q + 2 <= ke; q += 2) {
+                const uint32_t w0 = f.vpk[q], w1 = f.vpk[q + 1];
+                add(s_tab[w0 >> PK_COL_BITS], ((int64_t)(w0 & FAC_COL_MASK) * NC + s) * K);
+                add(s_tab[w1 >> PK_COL_BITS], ((int64_t)(w1 & FAC_COL_MASK) * NC + s) * K);
+            }
+            if (q < ke) {
+                const uint32_t w = f.vpk[q];
+                add(s_tab[w >> PK_COL_BITS], ((int64_t)(w & FAC_COL_MASK) * NC + s) * K);
+            }
+            const int64_t jb = f.rrow[r], je = f.rrow[r + 1];
+            for (int64_t j = jb; j < je; ++j) {
+                const uint32_t w = f.rpk[j];
+                add(s_tab[w >> PK_COL_BITS], (int64_t)(w & FAC_COL_MASK) * K);
+            }
+            add(f.diag[r], r * K);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int64_t o = r * K + k;
+                const double2 out = epilogue(e, k, o, make_double2(sr[k], si[k]), X);
+                e.y[o] = out;
+                if (e.dvec) dot[k] = cadd(dot[k], cconjmul(e.dvec[o], out));
+            }
+        }
+    }
+    if (e.dvec) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double2 t = dot[k];
+            for (int off = 32; off > 0; off >>= 1) {
+                t.x += __shfl_xor(t.x, off, 64);
+                t.y += __shfl_xor(t.y, off, 64);
+            }
+            if (lane == 0) red[wave][k] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x < K) {
+            const int k = threadIdx.x;
+            e.partials[(int64_t)blockIdx.x * K + k] =
+                cadd(cadd(red[0][k], red[1][k]), cadd(red[2][k], red[3][k]));
+        }
+    }
+}
+
 }  // namespace
 
 namespace edx {
@@ -224,14 +305,41 @@ int fac_launch(const edx_csr *a, const double2 *x, const SpmvEpi &e_in, int *nbl
     return EDX_OK;
 }
 
+// Launch of the multi-RHS factored product with bpc workgroups per CU
+// (K = 2, 3 or 4; the caller has checked the form exists).
+int fac_spmm_launch(const edx_csr *a, int K, const double2 *X, const BEpi &e, int bpc) {
+    const int nb = (num_cus() * bpc + 7) / 8 * 8;
+    FacArgs f{a->m / a->nc_slots, a->d_fv_rowptr, a->d_fv_pk, a->d_fr_rowptr, a->d_fr_pk,
+              a->d_vtab, (int)a->nvtab, a->d_diag};
+    const size_t lds = (size_t)a->nvtab * sizeof(double2);
+    if (a->m == 0) return EDX_OK;
+    switch (K) {
+        case 2: hipLaunchKernelGGL((k_spmm_fac<6, 2>), dim3(nb), dim3(256), lds, stream(), f, X, e); break;
+        case 3: hipLaunchKernelGGL((k_spmm_fac<6, 3>), dim3(nb), dim3(256), lds, stream(), f, X, e); break;
+        case 4: hipLaunchKernelGGL((k_spmm_fac<6, 4>), dim3(nb), dim3(256), lds, stream(), f, X, e); break;
+        default: return set_error(EDX_ERR_ARG, "factored multi-RHS product for K = %d not built", K);
+    }
+    EDX_LAUNCH_CHECK();
+    return EDX_OK;
+}
+
 // Algorithmic bytes of the factored product as stored: the words and row
 // pointers of H_v and R, the value table, the diagonal, x once, y written and
 // the epilogue's addend and dot vector.
 double fac_bytes(const edx_csr *a, bool z, bool dvec) {
-    const double mv = (double)(a->m / a->nc_slots);
+    const double mv = a->nc_slots > 0 ? (double)(a->m / a->nc_slots) : 0.0;
     return (double)(a->fv_nnz + a->fr_nnz) * 4.0 + (mv + 1.0) * 8.0 + (double)(a->m + 1) * 8.0 +
            (double)a->nvtab * 16.0 + (double)a->m * 16.0 + (double)a->n * 16.0 +
            (double)a->m * 16.0 * (1.0 + (z ? 1.0 : 0.0) + (dvec ? 1.0 : 0.0));
+}
+
+// The same for K interleaved right-hand sides: the stored form once, the
+// vector terms K times.
+double fac_bytes_k(const edx_csr *a, int K, bool z, bool dvec) {
+    const double mv = a->nc_slots > 0 ? (double)(a->m / a->nc_slots) : 0.0;
+    return (double)(a->fv_nnz + a->fr_nnz) * 4.0 + (mv + 1.0) * 8.0 + (double)(a->m + 1) * 8.0 +
+           (double)a->nvtab * 16.0 + (double)a->m * 16.0 +
+           K * ((double)a->n * 16.0 + (double)a->m * 16.0 * (1.0 + (z ? 1.0 : 0.0) + (dvec ? 1.0 : 0.0)));
 }
 
 // Builds the slot-factored form from the indexed form.  Only square matrices

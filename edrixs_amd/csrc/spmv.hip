@@ -457,29 +457,85 @@ int lanes_for_split(const edx_csr *a) {
 namespace edx {
 
 // ---------------------------------------------------------------- profiling
+// Launches are bracketed by a ring of PROF_CAP event pairs.  Before a slot is
+// reused, its launch (PROF_CAP launches back, long finished: the host never
+// runs that far ahead of the device) is folded into per-(precision, kernel)
+// totals, so every launch of a profiled region counts however long it runs.
 struct ProfRec {
     int kind;      // precision
     int kernel;    // EDX_KERNEL_*
     double bytes;
     double flops;
+    bool stopped;  // the stop event was recorded (the launch was made)
 };
+struct ProfSum {
+    int64_t n = 0;
+    double ms = 0.0, bytes = 0.0, flops = 0.0;
+};
+constexpr int PROF_KINDS = 8, PROF_KERNELS = 8;
 static bool g_prof = false;
 static std::vector<hipEvent_t> g_ev;      // pairs: 2k start, 2k+1 stop
-static std::vector<ProfRec> g_rec;
+static std::vector<ProfRec> g_rec;        // PROF_CAP slots
+static uint64_t g_prof_n = 0;             // launches recorded since edx_profile(1)
+static ProfSum g_sum[PROF_KINDS][PROF_KERNELS];   // launches folded out of the ring
 constexpr size_t PROF_CAP = 16384;
 
+static int prof_fold(size_t slot, ProfSum &acc) {
+    const ProfRec &r = g_rec[slot];
+    if (!r.stopped) return EDX_OK;
+    float t = 0.f;
+    EDX_HIP(hipEventSynchronize(g_ev[2 * slot + 1]));
+    EDX_HIP(hipEventElapsedTime(&t, g_ev[2 * slot], g_ev[2 * slot + 1]));
+    acc.n += 1;
+    acc.ms += t;
+    acc.bytes += r.bytes;
+    acc.flops += r.flops;
+    return EDX_OK;
+}
+
 static hipEvent_t prof_start(int kind, int kernel, double bytes, double flops = 0.0) {
-    if (!g_prof || g_rec.size() >= PROF_CAP) return nullptr;
-    if (g_ev.size() < 2 * PROF_CAP) return nullptr;
-    g_rec.push_back(ProfRec{kind, kernel, bytes, flops});
-    hipEvent_t e = g_ev[2 * (g_rec.size() - 1)];
+    if (!g_prof || g_ev.size() < 2 * PROF_CAP) return nullptr;
+    if (kind < 0 || kind >= PROF_KINDS || kernel < 0 || kernel >= PROF_KERNELS) return nullptr;
+    const size_t slot = (size_t)(g_prof_n % PROF_CAP);
+    if (g_prof_n >= PROF_CAP) {
+        const ProfRec &old = g_rec[slot];
+        if (prof_fold(slot, g_sum[old.kind][old.kernel]) != EDX_OK) return nullptr;
+    }
+    g_rec[slot] = ProfRec{kind, kernel, bytes, flops, false};
+    ++g_prof_n;
+    hipEvent_t e = g_ev[2 * slot];
     (void)hipEventRecord(e, stream());
     return e;
 }
 
 static void prof_stop(hipEvent_t started) {
     if (!started) return;
-    (void)hipEventRecord(g_ev[2 * (g_rec.size() - 1) + 1], stream());
+    const size_t slot = (size_t)((g_prof_n - 1) % PROF_CAP);
+    g_rec[slot].stopped = hipEventRecord(g_ev[2 * slot + 1], stream()) == hipSuccess;
+}
+
+// Totals of one precision and one kernel (-1: all kernels): the folded
+// launches plus those still in the ring.
+static int prof_total(int precision, int kernel, ProfSum *out) {
+    ProfSum acc;
+    if (precision >= 0 && precision < PROF_KINDS)
+        for (int k = 0; k < PROF_KERNELS; ++k)
+            if (kernel < 0 || kernel == k) {
+                const ProfSum &s = g_sum[precision][k];
+                acc.n += s.n;
+                acc.ms += s.ms;
+                acc.bytes += s.bytes;
+                acc.flops += s.flops;
+            }
+    const uint64_t first = g_prof_n > PROF_CAP ? g_prof_n - PROF_CAP : 0;
+    for (uint64_t i = first; i < g_prof_n; ++i) {
+        const size_t slot = (size_t)(i % PROF_CAP);
+        if (g_rec[slot].kind != precision) continue;
+        if (kernel >= 0 && g_rec[slot].kernel != kernel) continue;
+        EDX_TRY(prof_fold(slot, acc));
+    }
+    *out = acc;
+    return EDX_OK;
 }
 
 // Algorithmic HBM bytes of one product with K right-hand sides: every stored
@@ -575,7 +631,7 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
         if (!schedule_available(a, precision, G))
             return set_error(EDX_ERR_ARG, "SpMV schedule %d needs the slot-factored form", G);
         int nb = 0;
-        hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
+        hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV_FAC,
                                    fac_bytes(a, e_in.z != nullptr, e_in.dvec != nullptr));
         EDX_TRY(fac_launch(a, (const double2 *)x, e_in, &nb));
         prof_stop(pe);
@@ -752,8 +808,11 @@ int edx_profile(int enable) {
                 EDX_HIP(hipEventCreate(&e));
                 edx::g_ev.push_back(e);
             }
+            edx::g_rec.resize(edx::PROF_CAP);
         }
-        edx::g_rec.clear();
+        edx::g_prof_n = 0;
+        for (auto &row : edx::g_sum)
+            for (auto &s : row) s = edx::ProfSum{};
     }
     edx::g_prof = enable != 0;
     return EDX_OK;
@@ -762,30 +821,20 @@ int edx_profile(int enable) {
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
                      double *total_bytes) {
     EDX_HIP(hipStreamSynchronize(edx::stream()));
-    int64_t n = 0;
-    double ms = 0.0, by = 0.0;
-    for (size_t i = 0; i < edx::g_rec.size(); ++i) {
-        if (edx::g_rec[i].kind != precision) continue;
-        if (kernel >= 0 && edx::g_rec[i].kernel != kernel) continue;
-        float t = 0.f;
-        EDX_HIP(hipEventElapsedTime(&t, edx::g_ev[2 * i], edx::g_ev[2 * i + 1]));
-        ++n;
-        ms += t;
-        by += edx::g_rec[i].bytes;
-    }
-    if (launches) *launches = n;
-    if (total_ms) *total_ms = ms;
-    if (total_bytes) *total_bytes = by;
+    edx::ProfSum s;
+    EDX_TRY(edx::prof_total(precision, kernel, &s));
+    if (launches) *launches = s.n;
+    if (total_ms) *total_ms = s.ms;
+    if (total_bytes) *total_bytes = s.bytes;
     return EDX_OK;
 }
 
 int edx_profile_read_flops(int precision, int kernel, double *total_flops) {
     EDX_CHECK(total_flops, EDX_ERR_ARG, "null argument");
-    double f = 0.0;
-    for (size_t i = 0; i < edx::g_rec.size(); ++i)
-        if (edx::g_rec[i].kind == precision && (kernel < 0 || edx::g_rec[i].kernel == kernel))
-            f += edx::g_rec[i].flops;
-    *total_flops = f;
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    edx::ProfSum s;
+    EDX_TRY(edx::prof_total(precision, kernel, &s));
+    *total_flops = s.flops;
     return EDX_OK;
 }
 

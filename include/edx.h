@@ -165,7 +165,9 @@ int edx_csr_factored_info(const edx_csr *a, int *nc_slots, int64_t *hv_nnz, int6
  * enabled, every launch is bracketed by HIP events on its stream and its
  * algorithmic HBM bytes
  *   nnz*(s_v + s_i) + (m+1)*s_p + K*(n*s_x + m*s_y*(reads + writes of y-like vectors))
- * are recorded.  edx_profile(1) clears and starts, edx_profile(0) stops;
+ * are recorded; a ring of event pairs is folded into running totals as it
+ * wraps, so every launch of a profiled region counts, however many there are.
+ * edx_profile(1) clears and starts, edx_profile(0) stops;
  * edx_profile_read synchronises and returns the launch count, summed kernel
  * milliseconds and summed bytes of the launches in one precision
  * (edx_precision) and one kernel (edx_kernel, or -1 for all). */
@@ -176,7 +178,8 @@ enum edx_kernel {
     EDX_KERNEL_SPMM_CM = 3,   /* k_spmm: column-major blocks (edx_spmm) */
     EDX_KERNEL_MF = 4,        /* k_mf: matrix-free apply (recorded as precision EDX_C128) */
     EDX_KERNEL_BPROJ = 5,     /* k_bproj: block projection V^H W on the matrix cores (EDX_C128) */
-    EDX_KERNEL_BUPD = 6       /* k_bupd: block update W - V C on the matrix cores (EDX_C128) */
+    EDX_KERNEL_BUPD = 6,      /* k_bupd: block update W - V C on the matrix cores (EDX_C128) */
+    EDX_KERNEL_SPMV_FAC = 7   /* k_spmv_fac: one vector, slot-factored form (EDX_C128I) */
 };
 int edx_profile(int enable);
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,

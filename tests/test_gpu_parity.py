@@ -1367,21 +1367,28 @@ def test_slot_factored_form_matches_oracle(core):
         same = [all(lists[v][s] == lists[v][0] for s in range(1, nc)) for v in range(dv)]
         hv = sum(len(lists[v][0]) for v in range(dv) if same[v])
         assert all(same) == all_same and any(same)
+        # rows of more than 8 remainder entries exist, so the loop past the 8
+        # words a lane prefetches runs
+        n_off_row = np.diff(R.rowptr) - 1                  # every row has its diagonal
+        r_len = [n_off_row[s_ * dv + v] - (len(lists[v][s_]) if same[v] else 0)
+                 for s_ in range(nc) for v in range(dv)]
+        assert max(r_len) > 8 and sum(r_len) == f[2]
         assert f[1] == hv and f[2] == n_off - nc * hv, (f, hv, n_off)
         x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
         y0 = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
         ref = o.matvec(R, x[pinv], y0[pinv])[p]
         scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
-        A.spmv_lanes("c128i", set=206)
-        assert A.spmv_lanes("c128i") == 206
-        y = A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128i").cpu().numpy()
-        assert np.abs(y - ref).max() < 1e-14 * scale
-        # a Krylov recurrence (fused epilogue, dot partials) on the factored product
-        a1, b1, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
-        A.spmv_lanes("c128i", set=144)
-        a2, b2, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
-        np.testing.assert_allclose(a1, a2, rtol=1e-12)
-        np.testing.assert_allclose(b1, b2, rtol=1e-12, atol=1e-12 * np.abs(a2).max())
+        for code in (206,):
+            A.spmv_lanes("c128i", set=code)
+            assert A.spmv_lanes("c128i") == code
+            y = A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128i").cpu().numpy()
+            assert np.abs(y - ref).max() < 1e-14 * scale, code
+            # a Krylov recurrence (fused epilogue, dot partials) on the factored product
+            a1, b1, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+            A.spmv_lanes("c128i", set=144)
+            a2, b2, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
+            np.testing.assert_allclose(a1, a2, rtol=1e-12)
+            np.testing.assert_allclose(b1, b2, rtol=1e-12, atol=1e-12 * np.abs(a2).max())
     # matrices outside the form: none is made, and the schedule refuses them
     Aref = core.build_ham_n(bn, nv, nc, base.hop_n, base.coul_n)      # reference layout
     Aref.prepare("c128i")
@@ -1540,3 +1547,62 @@ def test_group_factored_member_matches_oracle_and_is_mix_independent(core, monke
     (al3, be3, n3, _), ce = mix()
     assert np.array_equal(al3, al1) and np.array_equal(be3, be1) and n3 == n1
     assert np.array_equal(ce[0], cg[0]) and np.array_equal(ce[1], cg[1])
+
+
+@pytest.mark.parametrize("K", [2, 3, 4])
+def test_factored_multi_rhs_product_matches_single_vectors(core, monkeypatch, K):
+    # the factored product over K interleaved vectors (forced schedule 3:4 of
+    # the multi-RHS products) in a batched Krylov recurrence: each vector's
+    # (alpha, beta) against the single-vector recurrence and the oracle's
+    m, fn, bn, _, R = _siam_hn(core, nbath=4, seed=2)
+    A = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout="interleaved")
+    A.prepare("c128i")
+    assert A.factored() is not None
+    p = core.corehole_permutation(len(fn), m.ncore)
+    rng = np.random.default_rng(53 + K)
+    V = rng.normal(size=(K, A.n)) + 1j * rng.normal(size=(K, A.n))
+    monkeypatch.setenv("EDX_FORCE_BSCHED", "3:4")
+    al, be, nr = core.build_krylov_batch(A, _to_dev(core, V), 50, precision="c128i")
+    monkeypatch.delenv("EDX_FORCE_BSCHED")
+    for k in range(K):
+        a1, b1, n1 = core.build_krylov(A, _to_dev(core, V[k]), 50, precision="c128i")
+        np.testing.assert_allclose(al[k], a1, rtol=1e-12, atol=1e-12 * np.abs(a1).max())
+        np.testing.assert_allclose(be[k], b1, rtol=1e-12, atol=1e-12 * np.abs(a1).max())
+        assert abs(nr[k] - n1) <= 1e-13 * n1
+        vref = np.empty_like(V[k])
+        vref[p] = V[k]                                   # x_inter = x_ref[p]
+        ra, rb, rn = o.build_krylov(R, vref, 50)
+        np.testing.assert_allclose(al[k][:20], ra[:20], rtol=1e-9, atol=1e-9)
+    # the schedule refuses a matrix without the form
+    B = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    B.prepare("c128i")
+    monkeypatch.setenv("EDX_FORCE_BSCHED", "3:4")
+    with pytest.raises(Exception, match="slot-factored"):
+        core.build_krylov_batch(B, _to_dev(core, V), 5, precision="c128i")
+
+
+def test_profile_counts_every_launch_beyond_the_event_ring(core):
+    # more products than the profiler's ring of 16,384 event pairs holds: every
+    # launch is counted with its bytes, in each precision and kernel apart
+    m, fn, bn, A, R = _siam_hn(core, nbath=2, seed=1)
+    A.prepare("c128i")
+    x = _to_dev(core, np.random.default_rng(3).normal(size=A.n) + 0.25j)
+    y = core.torch().zeros(A.m, dtype=core.torch().complex128, device="cuda")
+    nrep = 20000
+    core.profile(True)
+    for _ in range(nrep):
+        A.spmv(x, y, precision="c128")
+    for _ in range(7):
+        A.spmv(x, y, precision="c128i")
+    n128, ms128, by128 = core.profile_read("c128", "k_spmv")
+    n128i, _, _ = core.profile_read("c128i", None)
+    core.profile(False)
+    assert n128 == nrep and n128i == 7
+    assert ms128 > 0 and by128 > 0
+    # bytes of one product: values and columns, row pointers, x, y read and written
+    one = A.nnz * 20.0 + (A.m + 1) * 8.0 + A.n * 16.0 + A.m * 32.0
+    assert abs(by128 / nrep - one) <= 1e-9 * one
+    # a cleared profile starts from zero
+    core.profile(True)
+    assert core.profile_read("c128", "k_spmv")[0] == 0
+    core.profile(False)

@@ -54,7 +54,8 @@ def test_serving_memory_estimate_grows_with_size():
 def test_bench_reads_measured_traffic():
     sys.path.insert(0, ROOT)
     import bench
-    for kernel, form in (("k_spmv", "c128i"), ("k_spmv", "c64d"), ("k_spmm_rl", "c128i")):
+    for kernel, form in (("k_spmv_fac", "c128i"), ("k_spmv", "c128i"), ("k_spmv", "c64d"),
+                         ("k_spmm_rl", "c128i")):
         v = bench.measured_traffic(kernel, form, 24)
         assert v is not None and 1e9 < v < 1e11
     assert bench.measured_traffic("k_spmv", "c128i", 20) is None
