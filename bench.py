@@ -886,8 +886,10 @@ def main():
         # matvec_csr streams (src/spmv.f90:73-103): 16-byte value and 4-byte
         # column per nonzero (diagonal included), 8-byte row pointers, the same
         # vector terms
-        mh, nc, hv, rr = inf["m_n"], *inf["fac_n"]
-        stored = (hv + rr) * 4.0 + (mh // nc + 1) * 8.0 + (mh + 1) * 8.0 + mh * 16.0
+        mh, (nc, hv, rr, nb, nt) = inf["m_n"], inf["fac_n"]
+        mv = mh // nc
+        stored = ((hv + rr + nb) * 4.0 + (mv + 1) * 8.0 + ((mv + 1) * 8.0 if nb else 0.0) +
+                  ((mh + 1) * 8.0 if rr else 0.0) + nt * 8.0 + mh * 16.0)
         csr = inf["nnz_n"] * 20.0 + (mh + 1) * 8.0
         cby = by + n * (csr - stored)
         line["roofline"]["csr_equivalent"] = {

@@ -88,6 +88,7 @@ _SIGS = {
     "edx_csr_spmv_lanes": (_I, [_P, _I, _I, ctypes.POINTER(_I)]),
     "edx_csr_factored_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I64),
                                    ctypes.POINTER(_I64)]),
+    "edx_csr_factored_blocks": (_I, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I)]),
     "edx_spmv": (_I, [_P, _I, _P, _P]),
     "edx_spmm": (_I, [_P, _I, _I, _P, _P]),
     "edx_profile": (_I, [_I]),

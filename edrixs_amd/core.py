@@ -185,11 +185,16 @@ class CSRMatrix:
     def factored(self):
         """The slot-factored form made by prepare("c128i") on a core-hole matrix
         in the interleaved layout: (core slots, entries stored once per valence
-        row, entries stored per row), or None when absent."""
+        row, entries stored per row, typed block words, block types), or None
+        when absent."""
         nc, hv, r = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
         _lib.call("edx_csr_factored_info", self._h, ctypes.byref(nc), ctypes.byref(hv),
                   ctypes.byref(r))
-        return (nc.value, hv.value, r.value) if nc.value else None
+        if not nc.value:
+            return None
+        nb, nt = ctypes.c_int64(), ctypes.c_int()
+        _lib.call("edx_csr_factored_blocks", self._h, ctypes.byref(nb), ctypes.byref(nt))
+        return nc.value, hv.value, r.value, nb.value, nt.value
 
     def add_c64(self):
         _lib.call("edx_csr_add_c64", self._h)
@@ -743,7 +748,8 @@ def profile(enable=True):
     _lib.call("edx_profile", int(bool(enable)))
 
 
-KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4, "k_spmv_fac": 7}
+KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4, "k_spmv_fac": 7,
+           "k_spmm_fac": 8}
 BLOCK_KERNELS = {"k_bproj": 5, "k_bupd": 6}   # matrix-core kernels (recorded as c128)
 
 

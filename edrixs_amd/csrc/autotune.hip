@@ -217,6 +217,10 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
         for (int bpc : {4, 8})   // pipelined RHS-parallel, workgroups per CU
             if (bsched_blocks(a, BSched{2, bpc}) <= bspmm_blocks(a->m))
                 cand.push_back(BSched{2, bpc});
+        if (packed_spmm_ok(a, precision))   // the same over the packed words, table in LDS
+            for (int bpc : {4, 8})
+                if (bsched_blocks(a, BSched{4, bpc}) <= bspmm_blocks(a->m))
+                    cand.push_back(BSched{4, bpc});
     }
     if (fac_spmm_ok(a, precision, K))   // slot-factored, workgroups per CU
         for (int bpc : {4, 8})

@@ -55,6 +55,9 @@ __device__ __forceinline__ double2 epilogue(const BEpi &e, int k, int64_t o, dou
 __device__ __forceinline__ uint16_t shfl16(uint16_t v, int src) {
     return (uint16_t)__shfl((int)v, src, 16);
 }
+__device__ __forceinline__ uint32_t shfl16(uint32_t v, int src) {
+    return (uint32_t)__shfl((int)v, src, 16);
+}
 __device__ __forceinline__ float2 shfl16(float2 v, int src) {
     return make_float2(__shfl(v.x, src, 16), __shfl(v.y, src, 16));
 }
@@ -69,16 +72,27 @@ __device__ __forceinline__ double2 shfl16(double2 v, int src) {
 // its group's row (one coalesced load per row instead of a broadcast per
 // nonzero), hands them to the group by shuffles while each lane gathers its own
 // right-hand side, and loads those of the group's next row first.  Nonzeros
-// beyond 32 in a row are read in place.
-template <class VP, class XT, class ACC, bool SPLIT>
+// beyond 32 in a row are read in place.  LTAB (indexed forms): the value table
+// (ntab entries) is staged in LDS and read from there.
+template <class VP, class XT, class ACC, bool SPLIT, bool LTAB = false>
 __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_t *__restrict__ rowptr,
                                                   const int32_t *__restrict__ cols, VP vals,
                                                   const double2 *__restrict__ diag,
                                                   const XT *__restrict__ X,
-                                                  const double2 *__restrict__ X64, BEpi e) {
+                                                  const double2 *__restrict__ X64, BEpi e,
+                                                  int ntab = 0) {
     using PV = PipeVal<VP>;
     using H = typename PV::type;
     __shared__ double2 red[16 * 16];
+    extern __shared__ double2 s_tab[];
+    if constexpr (LTAB) {
+        for (int i = threadIdx.x; i < ntab; i += 256) s_tab[i] = vals.tab[i];
+        __syncthreads();
+    }
+    auto value = [&](H h) {
+        if constexpr (LTAB) return s_tab[h];
+        else return PV::value(vals, h);
+    };
     const int l = threadIdx.x & 15;
     const int grp = threadIdx.x >> 4;
     const int k = blockIdx.y * 16 + l;
@@ -94,9 +108,10 @@ __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_
         b = rowptr[r];
         end = rowptr[r + 1];
     }
-    int ca = b + l < end ? cols[b + l] : 0, cb = b + 16 + l < end ? cols[b + 16 + l] : 0;
-    H ha = b + l < end ? PV::load(vals, b + l) : PV::zero();
-    H hb = b + 16 + l < end ? PV::load(vals, b + 16 + l) : PV::zero();
+    int ca, cb;
+    H ha, hb;
+    PV::fetch(cols, vals, b + l, b + l < end, ca, ha);
+    PV::fetch(cols, vals, b + 16 + l, b + 16 + l < end, cb, hb);
     while (__any(r < rend)) {
         const int64_t rn = r + stride;
         int64_t bn = 0, endn = 0;
@@ -104,10 +119,10 @@ __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_
             bn = rowptr[rn];
             endn = rowptr[rn + 1];
         }
-        const int can = bn + l < endn ? cols[bn + l] : 0;
-        const int cbn = bn + 16 + l < endn ? cols[bn + 16 + l] : 0;
-        const H han = bn + l < endn ? PV::load(vals, bn + l) : PV::zero();
-        const H hbn = bn + 16 + l < endn ? PV::load(vals, bn + 16 + l) : PV::zero();
+        int can, cbn;
+        H han, hbn;
+        PV::fetch(cols, vals, bn + l, bn + l < endn, can, han);
+        PV::fetch(cols, vals, bn + 16 + l, bn + 16 + l < endn, cbn, hbn);
         if (r < rend) {
             ACC sr = 0, si = 0;
             const int cnt = (int)(end - b);
@@ -115,7 +130,7 @@ __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_
             for (int t = 0; t < nf; ++t) {
                 const int src = t & 15;
                 const int c = __shfl(t < 16 ? ca : cb, src, 16);
-                const auto a = PV::value(vals, shfl16(t < 16 ? ha : hb, src));
+                const auto a = value(shfl16(t < 16 ? ha : hb, src));
                 if (kv) {
                     const XT xv = X[(int64_t)c * K + k];
                     sr += (ACC)a.x * (ACC)xv.x - (ACC)a.y * (ACC)xv.y;
@@ -123,10 +138,12 @@ __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_
                 }
             }
             for (int64_t j = b + 32; j < end; ++j) {
-                const auto a = nt_load2(vals + j);
-                const int64_t c = nt_load(cols + j);
+                int c;
+                H h;
+                PV::fetch(cols, vals, j, true, c, h);
+                const auto a = value(h);
                 if (kv) {
-                    const XT xv = X[c * K + k];
+                    const XT xv = X[(int64_t)c * K + k];
                     sr += (ACC)a.x * (ACC)xv.x - (ACC)a.y * (ACC)xv.y;
                     si += (ACC)a.x * (ACC)xv.y + (ACC)a.y * (ACC)xv.x;
                 }
@@ -802,9 +819,11 @@ namespace edx {
 
 // schedule of a multi-RHS product: row-split (rl = 0, p lanes per row),
 // RHS-parallel (rl = 1, tiles of 16 p right-hand sides), pipelined
-// RHS-parallel (rl = 2, p workgroups per CU) or, for K <= 4 on a matrix with
-// the slot-factored form in the indexed precision, the factored product
-// (rl = 3, p workgroups per CU)
+// RHS-parallel (rl = 2, p workgroups per CU), for K <= 4 on a matrix with
+// the slot-factored form in the indexed precision the factored product
+// (rl = 3, p workgroups per CU), or the pipelined RHS-parallel product over
+// the packed indexed words with the value table in LDS (rl = 4, p
+// workgroups per CU)
 struct BSched {
     int rl;
     int p;
@@ -813,6 +832,9 @@ int fac_spmm_launch(const edx_csr *a, int K, const double2 *X, const BEpi &e, in
 double fac_bytes_k(const edx_csr *a, int K, bool z, bool dvec);
 inline bool fac_spmm_ok(const edx_csr *a, int precision, int K) {
     return precision == EDX_C128I && a->d_fr_rowptr && a->nc_slots == 6 && K >= 2 && K <= 4;
+}
+inline bool packed_spmm_ok(const edx_csr *a, int precision) {
+    return precision == EDX_C128I && a->d_off_pk && a->nvtab <= LDS_TAB_MAX;
 }
 int tune_spmv(edx_csr *a, int precision);
 int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched);
@@ -828,7 +850,7 @@ int bsched_blocks(const edx_csr *a, BSched sched);
 void forced_bsched(int K, BSched *sched) {
     const char *f = getenv("EDX_FORCE_BSCHED");
     int rl = 0, p = 0;
-    if (K >= 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && rl >= 0 && rl <= 2 && p > 0)
+    if (K >= 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && (rl <= 2 || rl == 4) && rl >= 0 && p > 0)
         *sched = BSched{rl, p};
     if (K < 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && (rl == 0 || rl == 3) && p > 0)
         *sched = BSched{rl, p};
@@ -838,7 +860,7 @@ void forced_bsched(int K, BSched *sched) {
 // The pipelined schedule needs a multiple of 8 and at most bspmm_blocks(m),
 // the partials every caller provides.
 int bsched_blocks(const edx_csr *a, BSched sched) {
-    if (sched.rl == 2 || sched.rl == 3) return (edx::num_cus() * sched.p + 7) / 8 * 8;
+    if (sched.rl >= 2) return (edx::num_cus() * sched.p + 7) / 8 * 8;
     return bspmm_blocks(a->m);
 }
 
@@ -854,7 +876,19 @@ int bspmm_launch(const edx_csr *a, int precision, int K, const double2 *X64, con
                              bsched_blocks(a, sched), nb);
         return fac_spmm_launch(a, K, X64, e, sched.p);
     }
-    if (sched.rl == 2) {   // pipelined RHS-parallel, sched.p workgroups per CU
+    if (sched.rl == 4) {   // packed words, value table in LDS, sched.p workgroups per CU
+        if (!packed_spmm_ok(a, precision))
+            return set_error(EDX_ERR_ARG, "multi-RHS schedule 4 needs the packed indexed form");
+        const int nbp = bsched_blocks(a, sched);
+        if (nbp > nb)
+            return set_error(EDX_ERR_ARG, "pipelined SpMM needs %d workgroups, %d available", nbp,
+                             nb);
+        dim3 grid((unsigned)nbp, (unsigned)((K + 15) / 16));
+        hipLaunchKernelGGL((k_spmm_rlp<PkPtr, double2, double, true, true>), grid, dim3(256),
+                           (size_t)a->nvtab * sizeof(double2), stream(), a->m, K, a->d_off_rowptr,
+                           nullptr, PkPtr{a->d_off_pk, a->d_vtab}, a->d_diag, X64, X64, e,
+                           (int)a->nvtab);
+    } else if (sched.rl == 2) {   // pipelined RHS-parallel, sched.p workgroups per CU
         const int nbp = bsched_blocks(a, sched);
         if (nbp > nb)
             return set_error(EDX_ERR_ARG, "pipelined SpMM needs %d workgroups, %d available", nbp,
@@ -979,12 +1013,17 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
     if (sched.rl == 3 && !fac_spmm_ok(a, precision, K))
         return set_error(EDX_ERR_ARG, "the factored multi-RHS product needs the slot-factored "
                          "form, the indexed precision and 2 <= K <= 4");
+    if (sched.rl == 4 && !packed_spmm_ok(a, precision))
+        return set_error(EDX_ERR_ARG, "multi-RHS schedule 4 needs the packed indexed form");
     if (nblocks_out) *nblocks_out = bsched_blocks(a, sched);
-    hipEvent_t pe = prof_start(precision, (sched.rl == 1 || sched.rl == 2) ? EDX_KERNEL_SPMM_RL
-                                                                           : EDX_KERNEL_SPMM_IL,
-                               sched.rl == 3 ? fac_bytes_k(a, K, e.z != nullptr, e.dvec != nullptr)
-                                             : spmv_bytes_k(a, precision, K, e.z != nullptr,
-                                                            e.dvec != nullptr));
+    // the packed words read 4 bytes of index per nonzero instead of 6
+    const double bytes = sched.rl == 3 ? fac_bytes_k(a, K, e.z != nullptr, e.dvec != nullptr)
+                         : spmv_bytes_k(a, precision, K, e.z != nullptr, e.dvec != nullptr) -
+                               (sched.rl == 4 ? 2.0 * (double)a->off_nnz : 0.0);
+    hipEvent_t pe = prof_start(precision, sched.rl == 3 ? EDX_KERNEL_SPMM_FAC
+                                          : sched.rl == 0 ? EDX_KERNEL_SPMM_IL
+                                                          : EDX_KERNEL_SPMM_RL,
+                               bytes);
     EDX_TRY(bspmm_launch(a, precision, K, X64, X32, e, sched));
     prof_stop(pe);
     return EDX_OK;

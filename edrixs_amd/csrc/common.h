@@ -108,6 +108,15 @@ struct edx_csr {
     int64_t *d_fr_rowptr;   // m + 1
     uint32_t *d_fr_pk;
     int64_t fr_nnz;
+    // typed blocks of the factored form: per valence row whose slot rows share
+    // their slot-keeping entries, its slot-changing entries grouped by
+    // (valence column, slot shift), one word each (valence column | type << 22);
+    // a type is the shift and each slot's value number (factor.hip)
+    int64_t *d_fb_rowptr;   // m / nc_slots + 1
+    uint32_t *d_fb_pk;
+    int64_t fb_nnz;
+    uint64_t *d_ftyp;       // nftyp types
+    int nftyp;
 };
 
 // Lookup descriptor for a column space basis.

@@ -643,6 +643,9 @@ void edx_csr_free(edx_csr *a) {
     edx::dfree(a->d_fv_pk);
     edx::dfree(a->d_fr_rowptr);
     edx::dfree(a->d_fr_pk);
+    edx::dfree(a->d_fb_rowptr);
+    edx::dfree(a->d_fb_pk);
+    edx::dfree(a->d_ftyp);
     delete a;
 }
 

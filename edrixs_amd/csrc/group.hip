@@ -55,11 +55,9 @@ struct GMember {
     int64_t vblk0, nvblk;    // vector blocks
     int pipe;                // SpMV blocks form a pipelined sub-grid (blk0, nblk multiples of 8)
     // slot-factored form (indexed precision, interleaved core-hole matrices):
-    // H_v words per valence row, remainder words per row; fac = core slots or 0
-    const int64_t *fvrow;
-    const uint32_t *fvpk;
-    const int64_t *frrow;
-    const uint32_t *frpk;
+    // its kernel arguments (H_v, typed blocks, remainder, tables); fac = core
+    // slots or 0
+    FacArgs fa;
     int fac;
 };
 
@@ -204,7 +202,7 @@ __device__ double2 g_spmv_pipe(const GMember &M, VP vals, const double2 *s_tab, 
 // wave (the product of k_spmv_fac): block l of the member takes valence rows
 // 4 (64 / NC) l ... + 4 (64 / NC) - 1.  Returns this thread's share of <x|y>.
 template <int NC>
-__device__ double2 g_spmv_fac(const GMember &M, const double2 *s_tab, double2 c1,
+__device__ double2 g_spmv_fac(const GMember &M, const double2 *s_tab, const uint64_t *s_typ, double2 c1,
                               const double2 *__restrict__ x, double2 *y, int64_t l) {
     constexpr int VPW = 64 / NC;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -213,10 +211,9 @@ __device__ double2 g_spmv_fac(const GMember &M, const double2 *s_tab, double2 c1
     const int64_t v = (l * 4 + wave) * VPW + vl;
     double2 dot = make_double2(0, 0);
     if (vl < VPW && v < mv) {
-        const FacArgs f{mv, M.fvrow, M.fvpk, M.frrow, M.frpk, M.tab, M.ntab, M.diag};
         const int64_t r = v * NC + s;
         const double2 dg = M.diag[r], xr = x[r], yr = y[r];
-        const double2 ax = cadd(fac_offdiag_sum<NC, 8>(f, x, s_tab, v, s, r), cmul(dg, xr));
+        const double2 ax = cadd(fac_offdiag_sum<NC, 8>(M.fa, x, s_tab, s_typ, v, s, r), cmul(dg, xr));
         const double2 out = cadd(ax, cmul(c1, yr));
         y[r] = out;
         dot = cconjmul(xr, out);
@@ -237,11 +234,10 @@ __global__ __launch_bounds__(256) void k_gspmv(const GMember *mem, int nmem, dou
     const double2 *x = (st.steps & 1) ? M.v1 : M.v0;
     double2 *y = (st.steps & 1) ? M.v0 : M.v1;
     if constexpr (std::is_same<VP, IdxPtr>::value) {
-        if (M.fac == 6) {   // the member's value table in LDS, then the factored product
+        if (M.fac == 6) {   // the member's value table and block types in LDS, then the factored product
             extern __shared__ double2 s_tab[];
-            for (int i = threadIdx.x; i < M.ntab; i += 256) s_tab[i] = M.tab[i];
-            __syncthreads();
-            double2 dot = g_spmv_fac<6>(M, s_tab, st.c1, x, y, bid - M.blk0);
+            const uint64_t *s_typ = fac_stage_lds(M.fa, s_tab);
+            double2 dot = g_spmv_fac<6>(M, s_tab, s_typ, st.c1, x, y, bid - M.blk0);
             dot = block_sum_256(dot);
             if (threadIdx.x == 0) partials[bid] = dot;
             return;
@@ -540,16 +536,15 @@ int group_layout(edx_group *g) {
         // the slot-factored product where the matrix has the form (its table
         // always fits LDS: at most 256 values)
         d.fac = g->precision == EDX_C128I && h->d_fr_rowptr && h->nc_slots == 6 ? 6 : 0;
-        d.fvrow = h->d_fv_rowptr;
-        d.fvpk = h->d_fv_pk;
-        d.frrow = h->d_fr_rowptr;
-        d.frpk = h->d_fr_pk;
+        if (d.fac) d.fa = edx::fac_args(h);
         d.pipe = !d.fac && h->m >= G_PIPE_MIN_ROWS;
         d.ntab = (d.fac || (d.pipe && g->precision == EDX_C128I && h->nvtab <= LDS_TAB_MAX &&
                             lds_tab_enabled()))
                      ? (int)h->nvtab
                      : 0;
-        lds_tab = std::max(lds_tab, d.ntab);
+        // LDS of the SpMV launch in double2 units: a factored member's block
+        // types follow its value table
+        lds_tab = std::max(lds_tab, d.ntab + (d.fac ? (d.fa.ntyp + 1) / 2 : 0));
         if (d.fac) {    // 4 (64 / 6) valence rows per block
             d.nblk = (h->m / 6 + 4 * (64 / 6) - 1) / (4 * (64 / 6));
         } else if (d.pipe) {   // a resident sub-grid starting at a multiple of 8

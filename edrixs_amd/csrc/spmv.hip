@@ -472,7 +472,7 @@ struct ProfSum {
     int64_t n = 0;
     double ms = 0.0, bytes = 0.0, flops = 0.0;
 };
-constexpr int PROF_KINDS = 8, PROF_KERNELS = 8;
+constexpr int PROF_KINDS = 8, PROF_KERNELS = 16;
 static bool g_prof = false;
 static std::vector<hipEvent_t> g_ev;      // pairs: 2k start, 2k+1 stop
 static std::vector<ProfRec> g_rec;        // PROF_CAP slots

@@ -156,10 +156,23 @@ int edx_csr_spmv_lanes(edx_csr *a, int precision, int set, int *lanes);
  * core slots, fewer than 2^24 rows, at most 256 distinct off-diagonal
  * values): the off-diagonal entries that are the same for every core slot of
  * a valence row (terms on valence orbitals only, src/ham.f90:419-446) stored
- * once per valence row, the rest per row.  *nc_slots = 0 when the form is
- * absent; *hv_nnz and *r_nnz count the two parts' entries.  No reference
+ * once per valence row, the entries that move the core hole as typed blocks
+ * (edx_csr_factored_blocks), and the valence rows whose slot rows differ
+ * whole per row (R).  *nc_slots = 0 when the form is absent; *hv_nnz counts
+ * the entries stored once per valence row, *r_nnz those stored per row.  No reference
  * counterpart (a storage form of the matrix of src/ham.f90:352-711). */
 int edx_csr_factored_info(const edx_csr *a, int *nc_slots, int64_t *hv_nnz, int64_t *r_nnz);
+
+/* The typed blocks of the slot-factored form: for each valence row whose
+ * slot rows share their slot-keeping entries, its entries that move the core
+ * hole grouped by (valence column, slot shift), one 32-bit word per block
+ * (valence column | type << 22); a type is the shift and the value number of
+ * each slot's entry.  Made with the form where there are fewer than 2^22
+ * valence rows, at most 255 distinct values and at most 1024 types
+ * (EDX_NO_FAC_BLOCKS=1 keeps those entries per row instead).  *blocks counts
+ * the block words, *types the distinct types; both 0 without blocks.  No
+ * reference counterpart (a storage form of the matrix of src/ham.f90:352-711). */
+int edx_csr_factored_blocks(const edx_csr *a, int64_t *blocks, int *types);
 
 /* Live profiling of SpMV/SpMM launches (no reference counterpart): while
  * enabled, every launch is bracketed by HIP events on its stream and its
@@ -179,7 +192,8 @@ enum edx_kernel {
     EDX_KERNEL_MF = 4,        /* k_mf: matrix-free apply (recorded as precision EDX_C128) */
     EDX_KERNEL_BPROJ = 5,     /* k_bproj: block projection V^H W on the matrix cores (EDX_C128) */
     EDX_KERNEL_BUPD = 6,      /* k_bupd: block update W - V C on the matrix cores (EDX_C128) */
-    EDX_KERNEL_SPMV_FAC = 7   /* k_spmv_fac: one vector, slot-factored form (EDX_C128I) */
+    EDX_KERNEL_SPMV_FAC = 7,  /* k_spmv_fac: one vector, slot-factored form (EDX_C128I) */
+    EDX_KERNEL_SPMM_FAC = 8   /* k_spmm_fac: 2-4 interleaved vectors, slot-factored form */
 };
 int edx_profile(int enable);
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,

@@ -1285,6 +1285,15 @@ def test_pipelined_rhs_parallel_spmm_matches_other_schedules(core, monkeypatch, 
     a0, b0, n0 = core.build_krylov_batch(A, V, 60, precision=precision)
     monkeypatch.setenv("EDX_FORCE_BSCHED", "2:4")
     a1, b1, n1 = core.build_krylov_batch(A, V, 60, precision=precision)
+    if precision == "c128i":
+        # schedule 4 (packed words, value table in LDS) makes the same sums in
+        # the same order as schedule 2 on the same grid
+        monkeypatch.setenv("EDX_FORCE_BSCHED", "4:4")
+        a4, b4, n4 = core.build_krylov_batch(A, V, 60, precision=precision)
+        monkeypatch.setenv("EDX_FORCE_BSCHED", "2:4")
+        for k in range(16):
+            assert np.array_equal(a4[k], a1[k]) and np.array_equal(b4[k], b1[k])
+        assert np.array_equal(n4, n1)
     # 60 steps of the plain recurrence amplify the summation-order rounding
     for k in range(16):
         np.testing.assert_allclose(a1[k], a0[k], rtol=1e-10)
@@ -1320,15 +1329,18 @@ def test_gs_sector_search_matches_dense_sectors(core):
     assert table[0][0] == n_gs and [t[1] for t in table] == sorted(t[1] for t in table)
 
 
-def test_slot_factored_form_matches_oracle(core):
+def test_slot_factored_form_matches_oracle(core, monkeypatch):
     # H_n in the interleaved core-hole layout: the off-diagonal entries that keep
     # the core slot come from valence-only terms and are the same for the six
     # slots of a valence row, so the factored form stores them once per valence
-    # row (H_v) and the rest per row (R).  The split is checked against counts
-    # taken from the oracle's CSR (reference layout), and the factored product
-    # (schedule 206) against the oracle's matvec: on the plain model, and on one
-    # with core-valence terms that keep the slot, so some valence rows differ
-    # between slots and stay whole in R.
+    # row (H_v); the entries that move the core hole go into typed blocks
+    # (valence column, slot shift), and valence rows whose slot rows differ stay
+    # whole per row (R).  The split is checked against counts taken from the
+    # oracle's CSR (reference layout), and the factored product (schedule 206)
+    # against the oracle's matvec: on the plain model, and on one with
+    # core-valence terms that keep the slot, so some valence rows differ
+    # between slots; each also built without typed blocks (slot-changing
+    # entries per row).
     from edrixs_amd import siam
     base = H.small_siam(nbath=4, seed=2)
     nv, nc = base.norb, base.ncore
@@ -1345,14 +1357,12 @@ def test_slot_factored_form_matches_oracle(core):
     pinv[p] = np.arange(len(p))
     rng = np.random.default_rng(23)
     for coul, all_same in variants:
-        A = core.build_ham_n(bn, nv, nc, base.hop_n, coul, layout="interleaved")
         R = o.build_ham_n(fn, nv, nc, base.hop_n, coul)
-        A.prepare("c128i")
-        f = A.factored()
-        assert f is not None and f[0] == nc
         # expected split from the oracle's entries: per valence row, the
-        # slot-keeping off-diagonal (valence column, value) lists of the six slots
+        # slot-keeping off-diagonal (valence column, value) lists of the six
+        # slots, and the slot-moving entries by (valence column, shift)
         lists = [[[] for _ in range(nc)] for _ in range(dv)]
+        moves = [dict() for _ in range(dv)]
         n_off = 0
         for r in range(R.m):
             s, v = divmod(r, dv)
@@ -1364,31 +1374,53 @@ def test_slot_factored_form_matches_oracle(core):
                 s2, v2 = divmod(c, dv)
                 if s2 == s:
                     lists[v][s].append((v2, R.vals[j].real, R.vals[j].imag))
+                else:
+                    bits = np.array([R.vals[j].real, R.vals[j].imag]).view(np.int64)
+                    moves[v].setdefault((v2, s2 - s), {})[s] = (int(bits[0]), int(bits[1]))
         same = [all(lists[v][s] == lists[v][0] for s in range(1, nc)) for v in range(dv)]
         hv = sum(len(lists[v][0]) for v in range(dv) if same[v])
         assert all(same) == all_same and any(same)
-        # rows of more than 8 remainder entries exist, so the loop past the 8
-        # words a lane prefetches runs
         n_off_row = np.diff(R.rowptr) - 1                  # every row has its diagonal
-        r_len = [n_off_row[s_ * dv + v] - (len(lists[v][s_]) if same[v] else 0)
-                 for s_ in range(nc) for v in range(dv)]
-        assert max(r_len) > 8 and sum(r_len) == f[2]
-        assert f[1] == hv and f[2] == n_off - nc * hv, (f, hv, n_off)
-        x = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
-        y0 = rng.normal(size=A.n) + 1j * rng.normal(size=A.n)
+        r_rows = sum(int(n_off_row[s_ * dv + v]) for s_ in range(nc) for v in range(dv)
+                     if not same[v])
+        blocks = sum(len(moves[v]) for v in range(dv) if same[v])
+        types = {(d, tuple(sorted(ent.items()))) for v in range(dv) if same[v]
+                 for (_, d), ent in moves[v].items()}
+        x = rng.normal(size=dv * nc) + 1j * rng.normal(size=dv * nc)
+        y0 = rng.normal(size=x.size) + 1j * rng.normal(size=x.size)
         ref = o.matvec(R, x[pinv], y0[pinv])[p]
         scale = np.abs(R.vals).max() * np.abs(x).max() * np.diff(R.rowptr).max()
-        for code in (206,):
-            A.spmv_lanes("c128i", set=code)
-            assert A.spmv_lanes("c128i") == code
+        for with_blocks in (True, False):
+            if with_blocks:
+                monkeypatch.delenv("EDX_NO_FAC_BLOCKS", raising=False)
+            else:
+                monkeypatch.setenv("EDX_NO_FAC_BLOCKS", "1")
+            A = core.build_ham_n(bn, nv, nc, base.hop_n, coul, layout="interleaved")
+            A.prepare("c128i")
+            f = A.factored()
+            assert f is not None and f[0] == nc and f[1] == hv, (f, hv)
+            if with_blocks:
+                assert f[2] == r_rows and f[3] == blocks and f[4] == len(types), \
+                    (f, r_rows, blocks, len(types))
+                assert blocks > 0 and 0 < f[4] <= 1024
+            else:
+                assert f[2] == n_off - nc * hv and f[3] == 0 and f[4] == 0, f
+                # rows of more than 8 remainder entries exist, so the loop past
+                # the 8 words a lane prefetches runs
+                r_len = [n_off_row[s_ * dv + v] - (len(lists[v][s_]) if same[v] else 0)
+                         for s_ in range(nc) for v in range(dv)]
+                assert max(r_len) > 8 and sum(r_len) == f[2]
+            A.spmv_lanes("c128i", set=206)
+            assert A.spmv_lanes("c128i") == 206
             y = A.spmv(_to_dev(core, x), _to_dev(core, y0), precision="c128i").cpu().numpy()
-            assert np.abs(y - ref).max() < 1e-14 * scale, code
+            assert np.abs(y - ref).max() < 1e-14 * scale, with_blocks
             # a Krylov recurrence (fused epilogue, dot partials) on the factored product
             a1, b1, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
             A.spmv_lanes("c128i", set=144)
             a2, b2, _ = core.build_krylov(A, _to_dev(core, x), 40, precision="c128i")
             np.testing.assert_allclose(a1, a2, rtol=1e-12)
             np.testing.assert_allclose(b1, b2, rtol=1e-12, atol=1e-12 * np.abs(a2).max())
+    monkeypatch.delenv("EDX_NO_FAC_BLOCKS", raising=False)
     # matrices outside the form: none is made, and the schedule refuses them
     Aref = core.build_ham_n(bn, nv, nc, base.hop_n, base.coul_n)      # reference layout
     Aref.prepare("c128i")
