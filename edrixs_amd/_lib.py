@@ -46,6 +46,7 @@ _SIGS = {
     "edx_last_error": (ctypes.c_char_p, []),
     "edx_init": (_I, [_I]),
     "edx_mem_in_use": (_I, [ctypes.POINTER(_I64)]),
+    "edx_set_stream": (_I, [_P]),
     "edx_basis_combinations": (_I, [_I, _I, _PP]),
     "edx_basis_from_words": (_I, [_P, _I64, _I, _PP]),
     "edx_basis_size": (_I, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I)]),

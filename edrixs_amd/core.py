@@ -743,6 +743,13 @@ def mem_in_use():
     return v.value
 
 
+def set_stream(stream=None):
+    """Queue this thread's library work on a torch stream (None: the legacy
+    default stream).  Pair it with ``torch.cuda.stream(stream)`` so torch's
+    work in the thread is ordered with the library's."""
+    _lib.call("edx_set_stream", ctypes.c_void_p(stream.cuda_stream if stream is not None else 0))
+
+
 def profile(enable=True):
     """Start (clearing) or stop recording SpMV/SpMM launch timings."""
     _lib.call("edx_profile", int(bool(enable)))

@@ -26,7 +26,9 @@ int ensure_binomials() {
             else table[n * 64 + k] = table[(n - 1) * 64 + k - 1] + table[(n - 1) * 64 + k];
         }
     }
-    EDX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), table, sizeof(table)));
+    EDX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_binom), table, sizeof(table), 0,
+                                   hipMemcpyHostToDevice, stream()));
+    EDX_HIP(hipStreamSynchronize(stream()));
     if (dev < 64) g_binom_ready[dev] = true;
     return EDX_OK;
 }
@@ -115,30 +117,30 @@ int edx_basis_from_words(const int64_t *words, int64_t n, int nbits, edx_basis *
     EDX_TRY(edx::ensure_binomials());
     int64_t *d = (int64_t *)edx::dmalloc((n ? n : 1) * sizeof(int64_t), "basis words");
     EDX_CHECK(d, EDX_ERR_NOMEM, "%s", edx_last_error());
-    if (n) EDX_HIP(hipMemcpy(d, words, n * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (n) EDX_HIP(edx::copy_sync(d, words, n * sizeof(int64_t), hipMemcpyHostToDevice));
     int nocc = n ? __builtin_popcountll((unsigned long long)words[0]) : 0;
     int complete = 0;
     if (n > 0 && n == edx::binom_host(nbits, nocc)) {
         int *flag = (int *)edx::dmalloc(sizeof(int), "flag");
         EDX_CHECK(flag, EDX_ERR_NOMEM, "%s", edx_last_error());
-        EDX_HIP(hipMemset(flag, 0, sizeof(int)));
+        EDX_HIP(edx::fill_sync(flag, 0, sizeof(int)));
         hipLaunchKernelGGL(k_check_complete, dim3(grid_for(n)), dim3(256), 0, edx::stream(), n,
                            nbits, nocc, d, flag);
         EDX_LAUNCH_CHECK();
         int h = 1;
-        EDX_HIP(hipMemcpy(&h, flag, sizeof(int), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(&h, flag, sizeof(int), hipMemcpyDeviceToHost));
         edx::dfree(flag);
         complete = (h == 0);
     }
     if (!complete && n > 1) {   // binary search needs strictly ascending words
         int *flag = (int *)edx::dmalloc(sizeof(int), "flag");
         EDX_CHECK(flag, EDX_ERR_NOMEM, "%s", edx_last_error());
-        EDX_HIP(hipMemset(flag, 0, sizeof(int)));
+        EDX_HIP(edx::fill_sync(flag, 0, sizeof(int)));
         hipLaunchKernelGGL(k_check_sorted, dim3(grid_for(n)), dim3(256), 0, edx::stream(), n, d,
                            flag);
         EDX_LAUNCH_CHECK();
         int h = 0;
-        EDX_HIP(hipMemcpy(&h, flag, sizeof(int), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(&h, flag, sizeof(int), hipMemcpyDeviceToHost));
         edx::dfree(flag);
         if (h) {
             edx::dfree(d);
@@ -158,7 +160,7 @@ int edx_basis_size(const edx_basis *b, int64_t *n, int *is_complete) {
 
 int edx_basis_download(const edx_basis *b, int64_t *words) {
     EDX_CHECK(b && words, EDX_ERR_ARG, "null argument");
-    if (b->n) EDX_HIP(hipMemcpy(words, b->d_words, b->n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (b->n) EDX_HIP(edx::copy_sync(words, b->d_words, b->n * sizeof(int64_t), hipMemcpyDeviceToHost));
     return EDX_OK;
 }
 
@@ -172,11 +174,11 @@ int edx_basis_lookup(const edx_basis *b, const int64_t *words, int64_t n, int64_
         edx::dfree(d_out);
         return EDX_ERR_NOMEM;
     }
-    EDX_HIP(hipMemcpy(d_in, words, n * sizeof(int64_t), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(d_in, words, n * sizeof(int64_t), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_lookup, dim3(grid_for(n)), dim3(256), 0, edx::stream(), view_of(b), d_in,
                        n, d_out);
     EDX_LAUNCH_CHECK();
-    EDX_HIP(hipMemcpy(index, d_out, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(index, d_out, n * sizeof(int64_t), hipMemcpyDeviceToHost));
     edx::dfree(d_in);
     edx::dfree(d_out);
     return EDX_OK;

@@ -1046,7 +1046,7 @@ int csr_add_split(edx_csr *a) {
     EDX_LAUNCH_CHECK();
     EDX_TRY(exclusive_scan(orow, m + 1));
     int64_t onnz = 0;
-    EDX_HIP(hipMemcpy(&onnz, orow + m, sizeof(int64_t), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(&onnz, orow + m, sizeof(int64_t), hipMemcpyDeviceToHost));
     int32_t *oc = (int32_t *)dmalloc((onnz ? onnz : 1) * sizeof(int32_t), "split columns");
     float2 *ov = (float2 *)dmalloc((onnz ? onnz : 1) * sizeof(float2), "split values");
     if (!oc || !ov) {
@@ -1096,7 +1096,7 @@ int csr_add_indexed(edx_csr *a) {
         hipLaunchKernelGGL(k_vt_insert, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
                            a->d_vals, fp, tab, flag);
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            edx::copy_sync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
             rc = set_error(EDX_ERR_HIP, "indexed form: kernels failed");
             goto fail;
         }
@@ -1109,7 +1109,7 @@ int csr_add_indexed(edx_csr *a) {
         hipLaunchKernelGGL(k_vt_index, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
                            a->d_vals, a->d_off_rowptr, fp, tab, idx, flag);
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            edx::copy_sync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
             rc = set_error(EDX_ERR_HIP, "indexed form: kernels failed");
             goto fail;
         }
@@ -1119,7 +1119,7 @@ int csr_add_indexed(edx_csr *a) {
         goto fail;
     }
     hfp.resize(VTAB_SLOTS);
-    if (hipMemcpy(hfp.data(), fp, VTAB_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
+    if (edx::copy_sync(hfp.data(), fp, VTAB_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
         hipSuccess) {
         rc = set_error(EDX_ERR_HIP, "indexed form: download failed");
         goto fail;
@@ -1138,7 +1138,7 @@ int csr_add_indexed(edx_csr *a) {
             rc = EDX_ERR_NOMEM;
             goto fail;
         }
-        bool ok = hipMemcpy(d_remap, remap.data(), VTAB_SLOTS * sizeof(uint16_t), hipMemcpyHostToDevice) ==
+        bool ok = edx::copy_sync(d_remap, remap.data(), VTAB_SLOTS * sizeof(uint16_t), hipMemcpyHostToDevice) ==
                   hipSuccess;
         if (ok) {
             hipLaunchKernelGGL(k_vt_compact, dim3(VTAB_SLOTS / 256), dim3(256), 0, stream(), fp, tab, d_remap,
@@ -1363,7 +1363,7 @@ int kb_advance(edx_krylov_batch *q, int nsteps) {
         q->done = i;
         if (i - checked >= 32 || i == end) {
             std::vector<int> r(K);
-            EDX_HIP(hipMemcpy(r.data(), st.run + (size_t)i * K, K * sizeof(int), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(r.data(), st.run + (size_t)i * K, K * sizeof(int), hipMemcpyDeviceToHost));
             checked = i;
             bool any = false;
             for (int k = 0; k < K; ++k) any |= r[k] != 0;
@@ -1385,12 +1385,12 @@ int kb_read(const edx_krylov_batch *q, int count, int *neff, double *alpha, doub
     std::vector<int> hneff(K);
     const BKrylov &st = q->st;
     if (c > 0) {
-        EDX_HIP(hipMemcpy(ha.data(), st.alpha, (size_t)c * K * sizeof(double), hipMemcpyDeviceToHost));
-        EDX_HIP(hipMemcpy(hb.data(), st.beta, (size_t)(c + 1) * K * sizeof(double),
+        EDX_HIP(edx::copy_sync(ha.data(), st.alpha, (size_t)c * K * sizeof(double), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(hb.data(), st.beta, (size_t)(c + 1) * K * sizeof(double),
                           hipMemcpyDeviceToHost));
     }
-    EDX_HIP(hipMemcpy(hn.data(), st.norm, K * sizeof(double), hipMemcpyDeviceToHost));
-    EDX_HIP(hipMemcpy(hneff.data(), st.neff, K * sizeof(int), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hn.data(), st.norm, K * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hneff.data(), st.neff, K * sizeof(int), hipMemcpyDeviceToHost));
     for (int k = 0; k < K; ++k) {
         if (neff) neff[k] = std::min(hneff[k], count);
         if (norm) norm[k] = hn[k];
@@ -1444,8 +1444,8 @@ int bminres_run(const edx_csr *h, int precision, int K, const double2 *sigma, co
     EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(maxit + 2) * K + K) * sizeof(int), s));
     {
         std::vector<double2> hneg(K, make_double2(-1.0, 0.0));
-        EDX_HIP(hipMemcpy(neg1, hneg.data(), K * sizeof(double2), hipMemcpyHostToDevice));
-        EDX_HIP(hipMemcpy(sig, sigma, K * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(neg1, hneg.data(), K * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(sig, sigma, K * sizeof(double2), hipMemcpyHostToDevice));
     }
     const int kb = (K + 63) / 64;
     ELAUNCH(k_cm_to_il, n, K, n, K, Bcm, v[0]);
@@ -1488,7 +1488,7 @@ int bminres_run(const edx_csr *h, int precision, int K, const double2 *sigma, co
     hipStream_t cap = nullptr;
     if (graphs_enabled() && !edx::g_prof && maxit >= 3) {
         EDX_TRY(stepb.alloc(sizeof(int), "minres step"));
-        EDX_HIP(hipMemcpy(stepb.p, &i, sizeof(int), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(stepb.p, &i, sizeof(int), hipMemcpyHostToDevice));
         BMinres sg = st;
         sg.istep = stepb.as<int>();
         EDX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -1530,7 +1530,7 @@ int bminres_run(const edx_csr *h, int precision, int K, const double2 *sigma, co
         }
         if (!rc && i - checked >= 16) {
             std::vector<int> r(K);
-            const hipError_t em = hipMemcpy(r.data(), st.run + (size_t)i * K, K * sizeof(int),
+            const hipError_t em = edx::copy_sync(r.data(), st.run + (size_t)i * K, K * sizeof(int),
                                             hipMemcpyDeviceToHost);
             if (em != hipSuccess) {
                 rc = edx::hip_fail(em, "hipMemcpy", __FILE__, __LINE__);
@@ -1559,9 +1559,9 @@ int bminres_run(const edx_csr *h, int precision, int K, const double2 *sigma, co
     ELAUNCH(k_il_to_cm, n, K, n, K, x, Xcm);
     EDX_LAUNCH_CHECK();
     std::vector<double2> hs(K);
-    EDX_HIP(hipMemcpy(hs.data(), sums, K * sizeof(double2), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hs.data(), sums, K * sizeof(double2), hipMemcpyDeviceToHost));
     std::vector<int> hit(K);
-    EDX_HIP(hipMemcpy(hit.data(), st.iters, K * sizeof(int), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hit.data(), st.iters, K * sizeof(int), hipMemcpyDeviceToHost));
     for (int k = 0; k < K; ++k) {
         if (iters) iters[k] = hit[k];
         if (final_res) final_res[k] = sqrt(fabs(hs[k].x));

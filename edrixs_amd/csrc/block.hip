@@ -279,7 +279,7 @@ int bl_proj(BlockLz &L, const double2 *Vb, int nblk, const double2 *X, std::vect
     EDX_LAUNCH_CHECK();
     if (hc) {
         hc->resize((size_t)nblk * 64);
-        EDX_HIP(hipMemcpy(hc->data(), L.dC, (size_t)nblk * 64 * sizeof(double2),
+        EDX_HIP(edx::copy_sync(hc->data(), L.dC, (size_t)nblk * 64 * sizeof(double2),
                           hipMemcpyDeviceToHost));
     }
     return EDX_OK;
@@ -367,7 +367,7 @@ int bl_cholqr_pass(BlockLz &L, double2 *Q, double shift_rel, const std::vector<c
         *piv_ratio = lo / hi;
     }
     bl_inv_upper(R, Ri);
-    EDX_HIP(hipMemcpy(L.dC, Ri, sizeof Ri, hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(L.dC, Ri, sizeof Ri, hipMemcpyHostToDevice));
     return bl_upd(L, Q, 1, L.dC, nullptr, Q, 1.0);
 }
 
@@ -558,7 +558,7 @@ int block_lanczos_impl(const edx_csr *h, int precision, int nev, int maxblocks, 
             for (int r = 0; r < BW; ++r)
                 for (int c = 0; c < BW && g * BW + c < nev; ++c)
                     S[(size_t)b * 64 + r * BW + c] = Z[(size_t)(b * BW + r) * nev + g * BW + c];
-        EDX_HIP(hipMemcpy(L.dC, S.data(), S.size() * sizeof(cd), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(L.dC, S.data(), S.size() * sizeof(cd), hipMemcpyHostToDevice));
         EDX_TRY(bl_upd(L, L.V, steps, L.dC, nullptr, L.W, 1.0));
         const int nc = std::min(BW, nvec - g * BW);
         hipLaunchKernelGGL(k_bl_to_cols, dim3((unsigned)vec_blocks(n)), dim3(VB), 0, s, n, L.W, nc,

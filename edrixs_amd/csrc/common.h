@@ -22,12 +22,23 @@ void *dmalloc(size_t bytes, const char *what);
 void dfree(void *p);
 int64_t mem_in_use();
 
-// The stream all library work is queued on: the legacy default stream, so it
-// is ordered with torch's default stream and with synchronous hipMemcpy --
-// except while a solver captures its iterations into a HIP graph, when the
-// launches of this thread go to the capture stream (StreamScope).
+// The stream all library work of the calling thread is queued on: by default
+// the legacy default stream, so it is ordered with torch's default stream; a
+// thread may choose another (edx_set_stream), and while a solver captures its
+// iterations into a HIP graph the launches of this thread go to the capture
+// stream (StreamScope).  Host-device copies and fills go through copy_sync /
+// fill_sync, ordered on the same stream, so the library's results are complete
+// when they return whichever stream the thread uses.
 extern thread_local hipStream_t g_launch_stream;
 inline hipStream_t stream() { return g_launch_stream; }
+inline hipError_t copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, g_launch_stream);
+    return e == hipSuccess ? hipStreamSynchronize(g_launch_stream) : e;
+}
+inline hipError_t fill_sync(void *dst, int value, size_t bytes) {
+    const hipError_t e = hipMemsetAsync(dst, value, bytes, g_launch_stream);
+    return e == hipSuccess ? hipStreamSynchronize(g_launch_stream) : e;
+}
 struct StreamScope {
     hipStream_t saved;
     explicit StreamScope(hipStream_t s) : saved(g_launch_stream) { g_launch_stream = s; }

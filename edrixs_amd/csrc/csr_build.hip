@@ -353,8 +353,8 @@ int upload_terms(const edx_tensor2 *hop, int64_t nhop, const edx_tensor4 *coul, 
     tt.d_ops = (uint32_t *)edx::dmalloc(ops.size() * sizeof(uint32_t), "term ops");
     tt.d_vals = (double2 *)edx::dmalloc(vals.size() * sizeof(double2), "term values");
     EDX_CHECK(tt.d_ops && tt.d_vals, EDX_ERR_NOMEM, "%s", edx_last_error());
-    EDX_HIP(hipMemcpy(tt.d_ops, ops.data(), ops.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    EDX_HIP(hipMemcpy(tt.d_vals, vals.data(), vals.size() * sizeof(double2),
+    EDX_HIP(edx::copy_sync(tt.d_ops, ops.data(), ops.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(tt.d_vals, vals.data(), vals.size() * sizeof(double2),
                       hipMemcpyHostToDevice));
     return EDX_OK;
 }
@@ -407,7 +407,7 @@ int build_csr(BuildParams p, int64_t ncols, int precision, edx_csr **out) {
         EDX_HIP(hipMemsetAsync(overflow, 0, sizeof(int), edx::stream()));
         if ((rc = launch_cap<RK, CK, false>(p, cap))) break;
         int h = 0;
-        EDX_HIP(hipMemcpy(&h, overflow, sizeof(int), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(&h, overflow, sizeof(int), hipMemcpyDeviceToHost));
         if (!h) break;
         if (cap == 4096) {
             rc = edx::set_error(EDX_ERR_CAPACITY,
@@ -419,8 +419,8 @@ int build_csr(BuildParams p, int64_t ncols, int precision, edx_csr **out) {
     if (!rc) rc = exclusive_scan(rowptr, p.M + 1);
     int64_t nnz = 0;
     if (!rc) {
-        hipError_t e = hipMemcpy(&nnz, rowptr + p.M, sizeof(int64_t), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = edx::hip_fail(e, "hipMemcpy(nnz)", __FILE__, __LINE__);
+        hipError_t e = edx::copy_sync(&nnz, rowptr + p.M, sizeof(int64_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = edx::hip_fail(e, "edx::copy_sync(nnz)", __FILE__, __LINE__);
     }
     int32_t *cols = nullptr;
     double2 *vals = nullptr;
@@ -580,12 +580,12 @@ int edx_csr_info(const edx_csr *a, int64_t *m, int64_t *n, int64_t *nnz, int *pr
 int edx_csr_download(const edx_csr *a, int64_t *rowptr, int32_t *cols, void *vals) {
     EDX_CHECK(a, EDX_ERR_ARG, "null matrix");
     if (rowptr)
-        EDX_HIP(hipMemcpy(rowptr, a->d_rowptr, (a->m + 1) * sizeof(int64_t),
+        EDX_HIP(edx::copy_sync(rowptr, a->d_rowptr, (a->m + 1) * sizeof(int64_t),
                           hipMemcpyDeviceToHost));
     if (cols && a->nnz)
-        EDX_HIP(hipMemcpy(cols, a->d_cols, a->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(cols, a->d_cols, a->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (vals && a->nnz)
-        EDX_HIP(hipMemcpy(vals, a->d_vals, a->nnz * sizeof(double2), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(vals, a->d_vals, a->nnz * sizeof(double2), hipMemcpyDeviceToHost));
     return EDX_OK;
 }
 
@@ -593,11 +593,11 @@ int edx_csr_download_values(const edx_csr *a, int precision, void *vals) {
     EDX_CHECK(a && vals, EDX_ERR_ARG, "null argument");
     if (!a->nnz) return EDX_OK;
     if (precision == EDX_C128) {
-        EDX_HIP(hipMemcpy(vals, a->d_vals, a->nnz * sizeof(double2), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(vals, a->d_vals, a->nnz * sizeof(double2), hipMemcpyDeviceToHost));
     } else {
         EDX_CHECK(precision == EDX_C64 && a->d_vals32, EDX_ERR_ARG,
                   "precision %d values not present", precision);
-        EDX_HIP(hipMemcpy(vals, a->d_vals32, a->nnz * sizeof(float2), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(vals, a->d_vals32, a->nnz * sizeof(float2), hipMemcpyDeviceToHost));
     }
     return EDX_OK;
 }

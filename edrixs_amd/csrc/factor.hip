@@ -603,9 +603,9 @@ int csr_add_factored(edx_csr *a) {
             goto fail;
         }
         if (!use_b) break;
-        if (hipMemcpy(hset.data(), tset, FAC_TYPE_SET * sizeof(uint64_t), hipMemcpyDeviceToHost) !=
+        if (edx::copy_sync(hset.data(), tset, FAC_TYPE_SET * sizeof(uint64_t), hipMemcpyDeviceToHost) !=
                 hipSuccess ||
-            hipMemcpy(&hover, overflow, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            edx::copy_sync(&hover, overflow, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
             rc = set_error(EDX_ERR_HIP, "factored form: type download failed");
             goto fail;
         }
@@ -633,9 +633,9 @@ int csr_add_factored(edx_csr *a) {
             rc = EDX_ERR_NOMEM;
             goto fail;
         }
-        if (hipMemcpy(tid, hid.data(), FAC_TYPE_SET * sizeof(int16_t), hipMemcpyHostToDevice) !=
+        if (edx::copy_sync(tid, hid.data(), FAC_TYPE_SET * sizeof(int16_t), hipMemcpyHostToDevice) !=
                 hipSuccess ||
-            (!sorted.empty() && hipMemcpy(typ, sorted.data(), sorted.size() * sizeof(uint64_t),
+            (!sorted.empty() && edx::copy_sync(typ, sorted.data(), sorted.size() * sizeof(uint64_t),
                                           hipMemcpyHostToDevice) != hipSuccess)) {
             rc = set_error(EDX_ERR_HIP, "factored form: type upload failed");
             goto fail;
@@ -644,9 +644,9 @@ int csr_add_factored(edx_csr *a) {
     if ((rc = exclusive_scan(vrow, mv + 1)) || (rc = exclusive_scan(brow, mv + 1)) ||
         (rc = exclusive_scan(rrow, m + 1)))
         goto fail;
-    if (hipMemcpy(&vn, vrow + mv, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&bn, brow + mv, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&rn, rrow + m, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (edx::copy_sync(&vn, vrow + mv, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        edx::copy_sync(&bn, brow + mv, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        edx::copy_sync(&rn, rrow + m, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess) {
         rc = set_error(EDX_ERR_HIP, "factored form: download failed");
         goto fail;
     }

@@ -376,7 +376,7 @@ int ed_driver() {   // src/ed_driver.f90:1-281
             for (int64_t r = 0; r < ndim; ++r)
                 hv[(size_t)j * ndim + r] = make_double2(Z[(size_t)r * nvector + j].real(),
                                                         Z[(size_t)r * nvector + j].imag());
-        EDX_HIP(hipMemcpy(vecs.p, hv.data(), hv.size() * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(vecs.p, hv.data(), hv.size() * sizeof(double2), hipMemcpyHostToDevice));
     } else {   // Lanczos (ed_solver=1) or, for ed_solver=2, thick-restart Lanczos with
                // ARPACK's ncv / maxiter / tol meaning (src/ed_driver.f90:154-172)
         int nev = std::max(neval, nvector);
@@ -393,7 +393,7 @@ int ed_driver() {   // src/ed_driver.f90:1-281
                                      lanczos_deflate(), nev, ev.data(), tmp.p, &iters));
         for (int k = 0; k < neval; ++k) evals[k] = ev[k];
         if (nvector)
-            EDX_HIP(hipMemcpy(vecs.p, tmp.p, (size_t)ndim * nvector * sizeof(double2),
+            EDX_HIP(edx::copy_sync(vecs.p, tmp.p, (size_t)ndim * nvector * sizeof(double2),
                               hipMemcpyDeviceToDevice));
     }
     {   // eigvals.dat, (i5, f20.10)   src/io.f90:587-607
@@ -409,7 +409,7 @@ int ed_driver() {   // src/ed_driver.f90:1-281
         EDX_TRY(edx_denmat(B.b, no, v, dm.data() + (size_t)k * no * no * 2));
         if (cfg.idump) {
             std::vector<double2> hv(ndim);
-            EDX_HIP(hipMemcpy(hv.data(), v, ndim * sizeof(double2), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(hv.data(), v, ndim * sizeof(double2), hipMemcpyDeviceToHost));
             EDX_TRY(write_eigvec("eigvec." + std::to_string(k + 1), evals[std::min(k, neval - 1)], hv));
         }
     }
@@ -456,8 +456,8 @@ int xas_driver() {   // src/xas_driver.f90:1-166
         std::vector<double2> hv;
         double eig = 0.0;
         EDX_TRY(read_eigvec("eigvec." + std::to_string(igs), ndim_i, eig, hv));
-        EDX_HIP(hipMemcpy(gs.p, hv.data(), ndim_i * sizeof(double2), hipMemcpyHostToDevice));
-        EDX_HIP(hipMemset(phi.p, 0, ndim_n * sizeof(double2)));
+        EDX_HIP(edx::copy_sync(gs.p, hv.data(), ndim_i * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(edx::fill_sync(phi.p, 0, ndim_n * sizeof(double2)));
         EDX_TRY(edx_spmv(D.a, EDX_C128, gs.p, phi.p));
         int neff = 0;
         double norm = 0.0;
@@ -504,8 +504,8 @@ int rixs_driver() {   // src/rixs_driver.f90:1-246
         std::vector<double2> hv;
         double eig = 0.0;
         EDX_TRY(read_eigvec("eigvec." + std::to_string(igs), ndim_i, eig, hv));
-        EDX_HIP(hipMemcpy(gs.p, hv.data(), ndim_i * sizeof(double2), hipMemcpyHostToDevice));
-        EDX_HIP(hipMemset(phi.p, 0, ndim_n * sizeof(double2)));
+        EDX_HIP(edx::copy_sync(gs.p, hv.data(), ndim_i * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(edx::fill_sync(phi.p, 0, ndim_n * sizeof(double2)));
         EDX_TRY(edx_spmv(Di.a, EDX_C128, gs.p, phi.p));
         // A = (omega_in + E_gs + i gamma_in) - H_n, omega last on the diagonal
         Csr A;
@@ -515,7 +515,7 @@ int rixs_driver() {   // src/rixs_driver.f90:1-246
         int iters = 0;
         double res = 0.0;
         EDX_TRY(edx_minres(A.a, exact_form(A.a), phi.p, x.p, cfg.linsys_tol, cfg.linsys_max, &iters, &res));
-        EDX_HIP(hipMemset(phif.p, 0, ndim_f * sizeof(double2)));
+        EDX_HIP(edx::fill_sync(phif.p, 0, ndim_f * sizeof(double2)));
         EDX_TRY(edx_spmv(Df.a, EDX_C128, x.p, phif.p));
         int neff = 0;
         double norm = 0.0;
@@ -550,11 +550,11 @@ int opavg_driver() {   // src/opavg_driver.f90:1-126
     for (int igs = 1; igs <= cfg.num_gs; ++igs) {
         std::vector<double2> hv;
         EDX_TRY(read_eigvec("eigvec." + std::to_string(igs), ndim, evals[igs - 1], hv));
-        EDX_HIP(hipMemcpy(v.p, hv.data(), ndim * sizeof(double2), hipMemcpyHostToDevice));
-        EDX_HIP(hipMemset(w.p, 0, ndim * sizeof(double2)));
+        EDX_HIP(edx::copy_sync(v.p, hv.data(), ndim * sizeof(double2), hipMemcpyHostToDevice));
+        EDX_HIP(edx::fill_sync(w.p, 0, ndim * sizeof(double2)));
         EDX_TRY(edx_spmv(H.a, EDX_C128, v.p, w.p));
         std::vector<double2> hw(ndim);
-        EDX_HIP(hipMemcpy(hw.data(), w.p, ndim * sizeof(double2), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(hw.data(), w.p, ndim * sizeof(double2), hipMemcpyDeviceToHost));
         std::complex<double> s = 0.0;
         for (int64_t k = 0; k < ndim; ++k)
             s += std::conj(std::complex<double>(hv[k].x, hv[k].y)) *
@@ -572,7 +572,7 @@ int opavg_driver() {   // src/opavg_driver.f90:1-126
 int run_rank0(int my_id, int (*fn)()) {
     if (my_id != 0) return EDX_OK;
     int rc = fn();
-    hipError_t e = hipDeviceSynchronize();
+    hipError_t e = hipStreamSynchronize(edx::stream());
     if (!rc && e != hipSuccess) rc = edx::hip_fail(e, "fsolver", __FILE__, __LINE__);
     return rc;
 }

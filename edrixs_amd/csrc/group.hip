@@ -572,7 +572,7 @@ int group_layout(edx_group *g) {
     EDX_TRY(grow((void **)&g->vpartials, &g->vpartials_cap,
                  (size_t)std::max<int64_t>(vblk, 1) * sizeof(double2), "group partials"));
     if (!hm.empty())
-        EDX_HIP(hipMemcpy(g->d_mem, hm.data(), hm.size() * sizeof(GMember), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(g->d_mem, hm.data(), hm.size() * sizeof(GMember), hipMemcpyHostToDevice));
     g->dirty = false;
     return EDX_OK;
 }
@@ -686,7 +686,7 @@ int edx_group_advance(edx_group *g, int nsteps, int *live) {
             nlive = 0;
             for (const auto &m : g->mems) {
                 GState st;
-                EDX_HIP(hipMemcpy(&st, m.st, sizeof(GState), hipMemcpyDeviceToHost));
+                EDX_HIP(edx::copy_sync(&st, m.st, sizeof(GState), hipMemcpyDeviceToHost));
                 nlive += st.run;
             }
         }
@@ -701,14 +701,14 @@ int edx_group_read(const edx_group *g, int member, int *steps, double *alpha, do
     for (const auto &m : g->mems) {
         if (m.id != member) continue;
         GState st;
-        EDX_HIP(hipMemcpy(&st, m.st, sizeof(GState), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(&st, m.st, sizeof(GState), hipMemcpyDeviceToHost));
         if (steps) *steps = st.steps;
         if (norm) *norm = st.norm;
         if (running) *running = st.run;
         if (alpha && st.steps)
-            EDX_HIP(hipMemcpy(alpha, m.ab, st.steps * sizeof(double), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(alpha, m.ab, st.steps * sizeof(double), hipMemcpyDeviceToHost));
         if (beta && st.steps)
-            EDX_HIP(hipMemcpy(beta, m.ab + g->cap, st.steps * sizeof(double), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(beta, m.ab + g->cap, st.steps * sizeof(double), hipMemcpyDeviceToHost));
         return EDX_OK;
     }
     return edx::set_error(EDX_ERR_ARG, "no member %d in the group", member);

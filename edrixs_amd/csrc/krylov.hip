@@ -147,16 +147,16 @@ int krylov_loop_in(int64_t n, Prod prod, const KrylovScratch &k, const double2 *
         std::swap(curr, prev);
         if (i - checked >= 32 && i < maxn) {   // stop launching once the recurrence broke down
             int r = 1;
-            EDX_HIP(hipMemcpy(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
             checked = i;
             if (!r) break;
         }
     }
     KrylovDev hs;
-    EDX_HIP(hipMemcpy(&hs, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
-    EDX_HIP(hipMemcpy(alpha, d_alpha, maxn * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(&hs, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(alpha, d_alpha, maxn * sizeof(double), hipMemcpyDeviceToHost));
     // beta_1 .. beta_maxn
-    EDX_HIP(hipMemcpy(beta, d_beta + 1, maxn * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(beta, d_beta + 1, maxn * sizeof(double), hipMemcpyDeviceToHost));
     *neff = hs.neff;
     *norm = hs.norm;
     return EDX_OK;
@@ -230,7 +230,7 @@ int minres_run(const edx_csr *a, int precision, const double2 *b, double2 *x, do
         std::swap(curr, prev);
         if (i + 1 - checked >= 16) {
             int r = 1;
-            EDX_HIP(hipMemcpy(&r, run + i + 1, sizeof(int), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(&r, run + i + 1, sizeof(int), hipMemcpyDeviceToHost));
             checked = i + 1;
             if (!r) break;
         }
@@ -245,9 +245,9 @@ int minres_run(const edx_csr *a, int precision, const double2 *b, double2 *x, do
                        part.as<double2>() + nbv, nullptr);
     EDX_LAUNCH_CHECK();
     double2 r2;
-    EDX_HIP(hipMemcpy(&r2, part.as<double2>() + nbv, sizeof(double2), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(&r2, part.as<double2>() + nbv, sizeof(double2), hipMemcpyDeviceToHost));
     MinresDev hs;
-    EDX_HIP(hipMemcpy(&hs, S, sizeof(MinresDev), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(&hs, S, sizeof(MinresDev), hipMemcpyDeviceToHost));
     if (iters) *iters = hs.iters;
     if (final_res) *final_res = sqrt(fabs(r2.x));
     return EDX_OK;
@@ -274,7 +274,7 @@ int host_dot(Lanczos &L, const double2 *a, const double2 *b, double2 *out) {
     hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(VB), 0, edx::stream(), L.part, L.nbv,
                        L.part + L.nbv, nullptr);
     EDX_LAUNCH_CHECK();
-    EDX_HIP(hipMemcpy(out, L.part + L.nbv, sizeof(double2), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(out, L.part + L.nbv, sizeof(double2), hipMemcpyDeviceToHost));
     return EDX_OK;
 }
 
@@ -291,7 +291,7 @@ int orthogonalize(Lanczos &L, int k, double2 *w, std::vector<double2> *hk) {
     EDX_LAUNCH_CHECK();
     if (hk) {
         hk->resize(k);
-        EDX_HIP(hipMemcpy(hk->data(), L.hvec, k * sizeof(double2), hipMemcpyDeviceToHost));
+        EDX_HIP(edx::copy_sync(hk->data(), L.hvec, k * sizeof(double2), hipMemcpyDeviceToHost));
     }
     return EDX_OK;
 }
@@ -457,7 +457,7 @@ int trl_run(Lanczos &L, int first, int nev, int m, int max_cycles, double tol,
         std::vector<double> S((size_t)keep * me);   // column i: Ritz vector i's coefficients
         for (int i = 0; i < keep; ++i)
             for (int c = 0; c < me; ++c) S[(size_t)i * me + c] = Z[(size_t)c * me + i].real();
-        EDX_HIP(hipMemcpy(dS, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice));
+        EDX_HIP(edx::copy_sync(dS, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice));
         const double2 *V = L.basis + (int64_t)first * L.ld;
         if (done) {
             hipLaunchKernelGGL(k_combine, dim3(L.nbv), dim3(VB), 0, s, n, V, L.ld, me, dS, keep, X, n);
@@ -585,7 +585,7 @@ int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double t
             std::vector<double> Sk((size_t)m * take);
             for (int j = 0; j < take; ++j)
                 for (int c = 0; c < m; ++c) Sk[(size_t)j * m + c] = S[(size_t)j * m + c];
-            EDX_HIP(hipMemcpy(tmpS.p, Sk.data(), Sk.size() * sizeof(double), hipMemcpyHostToDevice));
+            EDX_HIP(edx::copy_sync(tmpS.p, Sk.data(), Sk.size() * sizeof(double), hipMemcpyHostToDevice));
             hipLaunchKernelGGL(k_combine, dim3(nbv), dim3(VB), 0, s, n, L.basis + (int64_t)nlock * n,
                                n, m, tmpS.as<double>(), take, evec_all.as<double2>(), n);
             EDX_LAUNCH_CHECK();

@@ -236,7 +236,7 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
     hipLaunchKernelGGL(k_krylov_norm, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S);
     EDX_LAUNCH_CHECK();
     KrylovDev hk;
-    EDX_HIP(hipMemcpy(&hk, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(&hk, S, sizeof(KrylovDev), hipMemcpyDeviceToHost));
     if (!(hk.sqrt_norm > 0.0)) {   // b = 0: every solution is 0
         for (int k = 0; k < K; ++k) {
             iters[k] = 0;
@@ -282,14 +282,14 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         std::swap(curr, prev);
         if (i - checked >= 16 && i < maxit) {
             int r = 1;
-            EDX_HIP(hipMemcpy(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
+            EDX_HIP(edx::copy_sync(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
             checked = i;
             if (!r) break;
         }
     }
     hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(256), 0, s, S, d_beta, sh, K);
     EDX_LAUNCH_CHECK();
-    EDX_HIP(hipMemcpy(hs.data(), sh, K * sizeof(MsShift), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hs.data(), sh, K * sizeof(MsShift), hipMemcpyDeviceToHost));
     for (int k = 0; k < K; ++k) {
         iters[k] = hs[k].iters;
         final_res[k] = hs[k].res / hk.sqrt_norm;

@@ -77,12 +77,12 @@ int edx_denmat(const edx_basis *fock, int num_val_orbs, const void *d_vec, doubl
     DevBuf dp, part;
     EDX_TRY(dp.alloc(np * sizeof(int2), "pairs"));
     EDX_TRY(part.alloc((size_t)np * nchunk * 2 * sizeof(double2), "denmat partials"));
-    EDX_HIP(hipMemcpy(dp.p, pairs.data(), np * sizeof(int2), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(dp.p, pairs.data(), np * sizeof(int2), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_denmat_pairs, dim3(np, nchunk), dim3(256), 0, edx::stream(), view_of(fock),
                        no, (const double2 *)d_vec, dp.as<int2>(), nchunk, part.as<double2>());
     EDX_LAUNCH_CHECK();
     std::vector<double2> hp((size_t)np * nchunk * 2);
-    EDX_HIP(hipMemcpy(hp.data(), part.p, hp.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hp.data(), part.p, hp.size() * sizeof(double2), hipMemcpyDeviceToHost));
     double *dm = denmat;
     std::memset(dm, 0, sizeof(double) * 2 * no * no);
     for (int p = 0; p < np; ++p) {
@@ -113,15 +113,15 @@ int edx_cf_spectrum(int neff, const double *alpha, const double *beta, double no
     const size_t bytes = sizeof(double) * (2 * (size_t)neff + 3 * (size_t)nw);
     EDX_TRY(buf.alloc(bytes, "cf buffers"));
     double *da = buf.as<double>(), *db = da + neff, *dom = db + neff, *dg = dom + nw, *ds = dg + nw;
-    EDX_HIP(hipMemcpy(da, alpha, neff * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(hipMemcpy(db, beta, neff * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(hipMemcpy(dom, omega, nw * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(hipMemcpy(dg, gamma, nw * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(hipMemcpy(ds, spec, nw * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(da, alpha, neff * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(db, beta, neff * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(dom, omega, nw * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(dg, gamma, nw * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(ds, spec, nw * sizeof(double), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_cf, dim3((nw + 255) / 256), dim3(256), 0, edx::stream(), neff, da, db, norm,
                        e_gs, weight, nw, dom, dg, ds);
     EDX_LAUNCH_CHECK();
-    EDX_HIP(hipMemcpy(spec, ds, nw * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(spec, ds, nw * sizeof(double), hipMemcpyDeviceToHost));
     return EDX_OK;
 }
 

@@ -115,4 +115,9 @@ int edx_mem_in_use(int64_t *bytes) {
     return EDX_OK;
 }
 
+int edx_set_stream(void *stream) {
+    edx::g_launch_stream = (hipStream_t)stream;
+    return EDX_OK;
+}
+
 }  // extern "C"

@@ -18,9 +18,11 @@ count is set from its measured step time), so small requests are not held to
 the pace of the largest ones.
 
 Per-request work that is not a Krylov step -- CSR builds, the ground state
-(Lanczos), the edge estimate and the RIXS multi-shift solve -- runs between
-rounds of group steps, one request per round; the smallest queued request is
-set up first unless one has waited longer than ``max_wait_s``.
+(Lanczos), the edge estimate and the RIXS multi-shift solve -- runs on a
+worker thread with its own stream (default priority), one request at a time,
+while this thread keeps stepping the groups; the finished setup's recurrences
+join their group between rounds.  The smallest queued request is set up first
+unless one has waited longer than ``max_wait_s``.
 
 Results do not depend on the mix: a member's coefficients are the same in any
 group, and a spectrum is taken at the first multiple of ``check_every`` of
@@ -94,8 +96,10 @@ class Engine:
 
     def __init__(self, budget_gb=64.0, precision="c128i", check_every=100, chunk=25,
                  cf_tol=1e-6, min_steps=200, max_steps=4000, n_in=4, npol=1, nw=1000,
-                 rixs_tol=1e-6, rixs_maxit=4000, slice_s=0.1, max_wait_s=10.0):
+                 rixs_tol=1e-6, rixs_maxit=4000, slice_s=0.1, max_wait_s=10.0, overlap=True):
         self.budget = budget_gb * 1e9
+        self.overlap = overlap
+        self._stream = None     # the worker's stream for setups
         self.precision = precision
         self.check_every = check_every
         self.chunk = chunk
@@ -166,9 +170,10 @@ class Engine:
         return siam.SiamModel(nbath=req.norb - 10, nocc=req.norb // 2, seed=req.seed,
                               F=(8.0, f2, 0.62 * f2), ten_dq=req.ten_dq, V=req.V)
 
-    def setup(self, req):
-        """Builds, ground state and (RIXS) the intermediate-state solve; returns
-        the request's running state with its group members."""
+    def prepare(self, req):
+        """Builds, ground state and (RIXS) the intermediate-state solve: the
+        request's matrices and the start vectors of its recurrences.  Touches
+        no group, so it may run on a worker thread (its own stream)."""
         m = self._model(req)
         p = self.precision
         bi = core.Basis.combinations(m.norb, m.nocc)
@@ -179,13 +184,11 @@ class Engine:
         gs = vecs[0].contiguous()
         req.egs = egs
         Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout=LAYOUT)
-        group = self.group_for(req.norb)
-        st = {"req": req, "mats": [Hi, Hn], "members": [], "grid": None, "gamma": None,
-              "group": group}
+        st = {"req": req, "mats": [Hi, Hn], "starts": [], "grid": None, "gamma": None}
         if req.kind == "xas":
             for k in range(3):
                 D = core.build_transop_i(bn, bi, m.norb, m.ncore, m.trans_i[k], layout=LAYOUT)
-                st["members"].append(group.add(Hn, D.matvec(gs)))
+                st["starts"].append((Hn, D.matvec(gs)))
             st["grid"], st["gamma"] = self.omega, GAMMA_C
             st["combine"] = "average"
         else:
@@ -206,13 +209,47 @@ class Engine:
                                               tol=self.rixs_tol, maxit=self.rixs_maxit,
                                               precision=p)
                 for k in range(len(sigma)):
-                    st["members"].append(group.add(Hi, Y[k].contiguous()))
+                    st["starts"].append((Hi, Y[k].contiguous()))
             st["grid"], st["gamma"] = self.eloss, GAMMA_F
             st["combine"] = "each"
             st["mats"] = [Hi]          # H_n no longer needed
+        return st
+
+    def attach(self, st):
+        """The request's recurrences join its size class's group (main thread)."""
+        import torch
+        group = self.group_for(st["req"].norb)
+        st["group"] = group
+        st["members"] = [group.add(H, v) for H, v in st["starts"]]
+        # the group copied the start vectors on this thread's stream: let the
+        # copies finish before the vectors' memory can go back to the worker
+        torch.cuda.current_stream().synchronize()
+        st["starts"] = []
         st["prev"] = {mid: None for mid in st["members"]}
         st["checked"] = {mid: 0 for mid in st["members"]}
         st["result"] = {}
+        return st
+
+    def setup(self, req):
+        """prepare() and attach() on this thread."""
+        return self.attach(self.prepare(req))
+
+    def _prepare_on_stream(self, req):
+        """prepare() on the calling (worker) thread's own stream, finished
+        before it returns; also the device bytes the setup took."""
+        import torch
+        if self._stream is None:
+            self._stream = torch.cuda.Stream()     # default priority
+        s = self._stream
+        core.set_stream(s)
+        try:
+            with torch.cuda.stream(s):
+                before = self.device_bytes()
+                st = self.prepare(req)
+                s.synchronize()
+                st["took"] = max(0.0, self.device_bytes() - before)
+        finally:
+            core.set_stream(None)
         return st
 
     def poll(self, st):
@@ -255,67 +292,89 @@ class Engine:
     # -------------------------------------------------------------- the loop
     def run(self, requests, realtime=True, log=None):
         """Serve ``requests`` (arrival times in seconds from now).  With
-        realtime=False every request is present at the start (for tests)."""
+        realtime=False every request is present at the start (for tests).
+        With ``overlap`` (the default) one request's setup runs on a worker
+        thread and its own stream while the groups keep stepping; otherwise
+        setups run between rounds on this thread."""
         import torch
+        from concurrent.futures import ThreadPoolExecutor
         pending = sorted(requests, key=lambda r: r.arrival)
         queue, active, done = [], [], []
         t0 = time.perf_counter()
+        pool = ThreadPoolExecutor(max_workers=1) if self.overlap else None
+        inflight = None           # the setup running on the worker
 
         def now():
             return time.perf_counter() - t0
 
-        while pending or queue or active:
-            t = now()
-            while pending and (not realtime or pending[0].arrival <= t):
-                queue.append(pending.pop(0))
-            # admission against the HBM budget by the bytes the device holds
-            # now, one setup per round so running requests keep stepping
-            # between setups
-            req = self.pick(queue, len(active), now(), self.device_bytes()) if queue else None
-            if req is not None:
-                queue.remove(req)
-                req.admitted = now()
-                before = self.device_bytes()
-                st = self.setup(req)
-                torch.cuda.synchronize()
-                took = max(0.0, self.device_bytes() - before)
-                est = self.estimate_bytes(req)
-                if est > 0 and took > 0:
-                    self.scale[req.norb] = max(self.scale.get(req.norb, 0.0), took / est)
-                st["bytes"] = took
-                self.used += took
-                active.append(st)
-                if log:
-                    log("admit r%d %s norb=%d at %.2f s (%d running)" %
-                        (req.rid, req.kind, req.norb, req.admitted, len(active)))
-            if active:
-                # every size class gets about slice_s of device time per round:
-                # small requests take many cheap steps while large ones take few
-                for norb in sorted({st["req"].norb for st in active}):
-                    g = self.groups[norb]
-                    per = self.step_s.get(norb)
-                    n = self.chunk if per is None else int(min(1000, max(1, self.slice_s / per)))
-                    t1 = time.perf_counter()
-                    g.advance(n)
-                    dt = (time.perf_counter() - t1) / n
-                    self.step_s[norb] = dt if per is None else 0.5 * (per + dt)
-                still = []
-                for st in active:
-                    if self.poll(st):
-                        self.finish(st)
-                        torch.cuda.synchronize()
-                        st["req"].done = now()
-                        self.used -= st["bytes"]
-                        done.append(st["req"])
-                        if log:
-                            r = st["req"]
-                            log("done r%d %s norb=%d latency %.2f s" % (r.rid, r.kind, r.norb,
-                                                                       r.latency))
-                    else:
-                        still.append(st)
-                active = still
-            elif pending and not queue:
-                time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
+        def admit(st, took):
+            req = st["req"]
+            est = self.estimate_bytes(req)
+            if est > 0 and took > 0:
+                self.scale[req.norb] = max(self.scale.get(req.norb, 0.0), took / est)
+            st["bytes"] = took
+            self.used += took
+            active.append(self.attach(st))
+            if log:
+                log("admit r%d %s norb=%d at %.2f s, ready at %.2f s (%d running)" %
+                    (req.rid, req.kind, req.norb, req.admitted, now(), len(active)))
+
+        try:
+            while pending or queue or active or inflight is not None:
+                t = now()
+                while pending and (not realtime or pending[0].arrival <= t):
+                    queue.append(pending.pop(0))
+                # a finished setup joins its group; with nothing else to step,
+                # wait for it
+                if inflight is not None and (inflight.done() or not active):
+                    admit(inflight.result(), inflight.result()["took"])
+                    inflight = None
+                # admission against the HBM budget by the bytes the device
+                # holds now, one setup at a time
+                if inflight is None and queue:
+                    req = self.pick(queue, len(active), now(), self.device_bytes())
+                    if req is not None:
+                        queue.remove(req)
+                        req.admitted = now()
+                        if pool is not None:
+                            inflight = pool.submit(self._prepare_on_stream, req)
+                        else:
+                            before = self.device_bytes()
+                            st = self.prepare(req)
+                            torch.cuda.synchronize()
+                            admit(st, max(0.0, self.device_bytes() - before))
+                if active:
+                    # every size class gets about slice_s of device time per
+                    # round: small requests take many cheap steps while large
+                    # ones take few
+                    for norb in sorted({st["req"].norb for st in active}):
+                        g = self.groups[norb]
+                        per = self.step_s.get(norb)
+                        n = self.chunk if per is None else int(min(1000, max(1, self.slice_s / per)))
+                        t1 = time.perf_counter()
+                        g.advance(n)
+                        dt = (time.perf_counter() - t1) / n
+                        self.step_s[norb] = dt if per is None else 0.5 * (per + dt)
+                    still = []
+                    for st in active:
+                        if self.poll(st):
+                            self.finish(st)
+                            torch.cuda.current_stream().synchronize()
+                            st["req"].done = now()
+                            self.used -= st["bytes"]
+                            done.append(st["req"])
+                            if log:
+                                r = st["req"]
+                                log("done r%d %s norb=%d latency %.2f s" % (r.rid, r.kind, r.norb,
+                                                                           r.latency))
+                        else:
+                            still.append(st)
+                    active = still
+                elif inflight is None and pending and not queue:
+                    time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True)
         return done
 
 

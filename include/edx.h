@@ -64,6 +64,13 @@ const char *edx_last_error(void);
 int edx_init(int device);
 /* Device-memory bytes currently held by library objects. */
 int edx_mem_in_use(int64_t *bytes);
+/* The stream this thread's library work is queued on: a hipStream_t, or NULL
+ * for the legacy default stream (the initial choice of every thread).  All
+ * launches and host-device copies of the thread's later calls go to it, and a
+ * call that returns results to the host waits for that stream only, so two
+ * threads on two streams overlap.  Use streams of the default priority.  No
+ * reference counterpart (the reference runs one MPI rank per process). */
+int edx_set_stream(void *stream);
 
 /* --------------------------------------------------------- Fock basis */
 

@@ -56,3 +56,21 @@ def test_serving_report_and_budget_admission():
     assert rep["latency_p50_s"] <= rep["latency_p99_s"] <= rep["latency_max_s"]
     assert all(r.done >= r.admitted >= r.arrival for r in done)
     assert abs(eng.used) < 1.0 and not eng.members
+
+
+def test_serving_overlapped_setups_equal_serial_setups():
+    # setups on the worker thread's own stream against setups between rounds
+    # on the main thread: the same spectra, bit for bit, and the same steps
+    import copy
+    reqs = _requests()
+    serial = _engine()
+    serial.overlap = False
+    a = {r.rid: r for r in serial.run(copy.deepcopy(reqs), realtime=False)}
+    eng = _engine()
+    b = {r.rid: r for r in eng.run(copy.deepcopy(reqs), realtime=False)}
+    assert eng._stream is not None          # the worker's stream was used
+    assert sorted(a) == sorted(b) == sorted(r.rid for r in reqs)
+    for rid in a:
+        assert a[rid].steps == b[rid].steps
+        for x, y in zip(a[rid].spectra, b[rid].spectra):
+            assert np.array_equal(x, y)
