@@ -52,6 +52,18 @@ __device__ __forceinline__ double2 epilogue(const BEpi &e, int k, int64_t o, dou
     return out;
 }
 
+// The same with the addend read non-temporally (each element is read once).
+__device__ __forceinline__ double2 epilogue_nt(const BEpi &e, int k, int64_t o, double2 hx,
+                                               const double2 *X64) {
+    double2 out = e.c0 ? cmul(e.c0[k], hx) : hx;
+    if (e.z) {
+        const double2 z = nt_load2(e.z + o);
+        out = cadd(out, e.c1 ? cmul(e.c1[k], z) : z);
+    }
+    if (e.c2) out = cadd(out, cmul(e.c2[k], X64[o]));
+    return out;
+}
+
 __device__ __forceinline__ uint16_t shfl16(uint16_t v, int src) {
     return (uint16_t)__shfl((int)v, src, 16);
 }
@@ -149,12 +161,14 @@ __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_
                 }
             }
             if (kv) {
+                // single-use operands and the output non-temporal, so the
+                // reused lines of X stay in L2 (2 % faster, K=16 over H_i)
                 const int64_t o = r * K + k;
                 double2 hx = make_double2((double)sr, (double)si);
-                if (SPLIT) hx = cadd(hx, cmul(diag[r], X64[o]));
-                const double2 out = epilogue(e, k, o, hx, X64);
-                e.y[o] = out;
-                if (e.dvec) dot = cadd(dot, cconjmul(e.dvec[o], out));
+                if (SPLIT) hx = cadd(hx, cmul(nt_load2(diag + r), X64[o]));
+                const double2 out = epilogue_nt(e, k, o, hx, X64);
+                nt_store2(e.y + o, out);
+                if (e.dvec) dot = cadd(dot, cconjmul(nt_load2(e.dvec + o), out));
             }
         }
         r = rn;

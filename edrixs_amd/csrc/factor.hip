@@ -342,16 +342,18 @@ __global__ __launch_bounds__(256, WPE) void k_spmv_fac(FacArgs f, const double2 
     double2 dot = make_double2(0.0, 0.0);
     if (vl < VPW && v < f.mv) {
         const int64_t r = v * NC + s;
-        const double2 dg = f.diag[r], xr = x[r];
+        // the row's single-use operands and its output go non-temporal, so they
+        // do not push the reused x lines out of L2 (2 % faster on H_n, norb 24)
+        const double2 dg = nt_load2(f.diag + r), xr = x[r];
         double2 pz = make_double2(0, 0), pd = pz;
-        if (e.z) pz = e.z[r];
-        if (e.dvec) pd = e.dvec[r];
+        if (e.z) pz = nt_load2(e.z + r);
+        if (e.dvec) pd = nt_load2(e.dvec + r);
         const double2 ax = cadd(fac_offdiag_sum<NC, UR>(f, x, s_tab, s_typ, v, s, r), cmul(dg, xr));
         const double2 c0 = e.c0 ? *e.c0 : make_double2(1.0, 0.0);
         const double2 c1 = e.c1 ? *e.c1 : make_double2(1.0, 0.0);
         double2 out = cmul(c0, ax);
         if (e.z) out = cadd(out, cmul(c1, pz));
-        e.y[r] = out;
+        nt_store2(e.y + r, out);
         if (e.dvec) dot = cconjmul(pd, out);
     }
     if (e.dvec) {

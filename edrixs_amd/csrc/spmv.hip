@@ -23,6 +23,11 @@ __device__ __forceinline__ double2 nt_load2(const double2 *p) {
     return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
 
+__device__ __forceinline__ void nt_store2(double2 *p, double2 v) {
+    double *q = reinterpret_cast<double *>(p);
+    __builtin_nontemporal_store(v.x, q);
+    __builtin_nontemporal_store(v.y, q + 1);
+}
 __device__ __forceinline__ float2 nt_load2(const float2 *p) {
     const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(p));
     return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
