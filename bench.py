@@ -104,6 +104,11 @@ def parse():
                    help="run only the lowest-three-states comparison (single-vector vs block "
                         "Lanczos) on the norb=20 and norb=24 models")
     p.add_argument("--eigs-nbath", default="10,14")
+    p.add_argument("--cpu-check", action="store_true",
+                   help="host only: the reference's whole procedure by the oracle at "
+                        "--cpu-check-nbath against the cpu_baseline cost model")
+    p.add_argument("--cpu-check-nbath", type=int, default=8)
+    p.add_argument("--cpu-check-nin", type=int, default=2)
     p.add_argument("--probe", action="store_true",
                    help="run only a short fixed set of SpMV/SpMM launches (for counter runs)")
     p.add_argument("--probe-form", default="c128i")
@@ -374,16 +379,12 @@ def exact_reference(work, gs, results):
     return out
 
 
-def cpu_baseline(work, info, rows):
-    """Oracle (C restatement, its row loops over the OpenMP threads the host
-    gives, OMP_NUM_THREADS) on a bounded sample of the same step,
-    following the reference's procedure: one fsolver call per RIXS (w_in, pol)
-    that rebuilds H_n (shifted), the operators and H_f, runs MINRES and the
-    final Krylov; XAS rebuilds H_n per component.  Per-row build cost and
-    per-nonzero matvec cost are measured on row windows and scaled."""
+def cpu_costs(m, rows, reps=3):
+    """Oracle costs measured on row windows of H_i and H_n (the middle third of
+    each, ``rows`` rows): seconds per built row and per nonzero of a product,
+    with the oracle's row loops over the OpenMP threads the host gives."""
     from oracle import oracle as o
     from edrixs_amd import iostream
-    m, a = work.model, work.args
     fi = iostream.fock_words(m.norb, m.nocc)
     fn = iostream.fock_words(m.norb, m.nocc + 1)
     mi, mn = len(fi), len(fn) * m.ncore
@@ -396,7 +397,6 @@ def cpu_baseline(work, info, rows):
     An = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n, row_begin=lo_n, row_end=lo_n + rn)
     tbn = (time.perf_counter() - t) / rn
     xi, xn = np.ones(mi, np.complex128), np.ones(mn, np.complex128)
-    reps = 3
     t = time.perf_counter()
     for _ in range(reps):
         o.matvec(Ai, xi)
@@ -405,26 +405,113 @@ def cpu_baseline(work, info, rows):
     for _ in range(reps):
         o.matvec(An, xn)
     tmn = (time.perf_counter() - t) / reps / max(An.nnz, 1)
-    build_i, build_n = tbi * mi, tbn * mn
-    spmv_i, spmv_n = tmi * info["nnz_i"], tmn * info["nnz_n"]
-    t_ed = build_i + info["lanczos_iters"] * spmv_i
-    t_xas = 3 * (build_n + np.mean(info["xas_krylov_steps"]) * spmv_n)
+    return dict(mi=mi, mn=mn, ri=ri, rn=rn, reps=reps, tbi=tbi, tbn=tbn, tmi=tmi, tmn=tmn,
+                cores=int(o.lib().orc_num_threads()))
+
+
+def cpu_model(c, nnz_i, nnz_n, lanczos_iters, xas_steps, rixs_iters, rixs_krylov_steps):
+    """Seconds of the reference's procedure predicted from the window costs:
+    builds by rows, products by nonzeros times the iteration counts (ED, one
+    XAS component per Krylov run rebuilding H_n, one RIXS (w_in, pol) call per
+    pair rebuilding shifted H_n and H_f).  Vector operations are not counted."""
+    build_i, build_n = c["tbi"] * c["mi"], c["tbn"] * c["mn"]
+    spmv_i, spmv_n = c["tmi"] * nnz_i, c["tmn"] * nnz_n
+    t_ed = build_i + lanczos_iters * spmv_i
+    t_xas = sum(build_n + k * spmv_n for k in xas_steps)
+    t_rixs = sum(build_n + it * spmv_n + build_i + k * spmv_i
+                 for it, k in zip(rixs_iters, rixs_krylov_steps))
+    return t_ed, t_xas, t_rixs
+
+
+def cpu_baseline(work, info, rows):
+    """Oracle (C restatement, its row loops over the OpenMP threads the host
+    gives, OMP_NUM_THREADS) on a bounded sample of the same step,
+    following the reference's procedure: one fsolver call per RIXS (w_in, pol)
+    that rebuilds H_n (shifted), the operators and H_f, runs MINRES and the
+    final Krylov; XAS rebuilds H_n per component.  Per-row build cost and
+    per-nonzero matvec cost are measured on row windows and scaled
+    (cpu_model; ``--cpu-check`` compares that model with the whole procedure
+    run at a smaller size)."""
+    c = cpu_costs(work.model, rows)
     # the reference runs one MINRES per (w_in, pol); its iteration count is taken
     # as that of the Galerkin iterate the device solve stopped at for that shift
-    it = np.mean(info["rixs_iters"])
-    kst = np.mean(info["rixs_krylov_steps"])
-    t_rixs_each = build_n + it * spmv_n + build_i + kst * spmv_i
+    xs = [float(np.mean(info["xas_krylov_steps"]))] * 3
+    t_ed, t_xas, t_rixs = cpu_model(c, info["nnz_i"], info["nnz_n"], info["lanczos_iters"], xs,
+                                    info["rixs_iters"], info["rixs_krylov_steps"])
+    total = t_ed + t_xas + t_rixs
     n_rixs = len(info["rixs_iters"])
-    total = t_ed + t_xas + n_rixs * t_rixs_each
     nspec = 1 + n_rixs
-    cores = int(o.lib().orc_num_threads())
-    return {"value": nspec / total, "unit": "spectra/s", "cores": cores, "kind": "port",
+    return {"value": nspec / total, "unit": "spectra/s", "cores": c["cores"], "kind": "port",
             "sample": ("oracle (%d threads, rows in parallel) built %d H_i and %d H_n rows and ran %d row-block "
                        "matvecs on each; per-row and per-nonzero costs scaled to the step "
                        "(ED + 3 XAS components + %d RIXS (w_in, pol) calls, each rebuilding "
                        "its matrices as the reference driver does): %.0f s for %d spectra"
-                       % (cores, ri, rn, reps, n_rixs, total, nspec)),
+                       % (c["cores"], c["ri"], c["rn"], c["reps"], n_rixs, total, nspec)),
             "seconds_per_step": total}
+
+
+def cpu_check(args):
+    """The reference's whole procedure run by the oracle on the host (no GPU):
+    ED by Lanczos, XAS as three Krylov runs each rebuilding H_n, and RIXS as
+    one call per (w_in, pol) that rebuilds sigma - H_n, solves it by MINRES,
+    applies D_out and runs the final-state Krylov over a rebuilt H_f
+    (src/ed_driver.f90, src/xas_driver.f90, src/rixs_driver.f90), at
+    ``--cpu-check-nbath``; then the cost model cpu_baseline scales from row
+    windows, fed with this run's own iteration counts."""
+    from oracle import oracle as o
+    from edrixs_amd import iostream, siam
+    m = siam.SiamModel(nbath=args.cpu_check_nbath, seed=args.seed)
+    fi = iostream.fock_words(m.norb, m.nocc)
+    fn = iostream.fock_words(m.norb, m.nocc + 1)
+    rng = np.random.default_rng(args.seed)
+    omega = np.linspace(-10.0, 30.0, args.nw)
+    T = {}
+    t0 = time.perf_counter()
+    Hi = o.build_ham_i(fi, m.hop_i, m.coul_i)
+    v0 = rng.normal(size=Hi.m) + 1j * rng.normal(size=Hi.m)
+    ev, vecs, it_ed = o.lanczos(Hi, v0, 1, 2000, 1e-10, 1)
+    egs, gs = float(ev[0]), vecs[0]
+    T["ed"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xas, xsteps = 0.0, []
+    for k in range(3):
+        Hn = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n)
+        D = o.build_transop_i(fn, fi, m.norb, m.ncore, m.trans_i[k])
+        a, b, nrm = o.build_krylov(Hn, o.matvec(D, gs), args.nkryl)
+        xsteps.append(len(a))
+        xas = xas + o.cf_spectrum(a, b, nrm, egs, omega, Workload.GAMMA_C)
+    T["xas"] = time.perf_counter() - t0
+    nnz_n = Hn.nnz
+    del Hn
+    edge = omega[int(np.argmax(xas))]
+    w_in = np.linspace(edge - 2.0, edge + 8.0, args.cpu_check_nin)
+    t0 = time.perf_counter()
+    iters, ksteps = [], []
+    for w in w_in:
+        for ei, ef in Workload.POLS:
+            sig = w + egs + 1j * Workload.GAMMA_C
+            A = o.build_ham_n(fn, m.norb, m.ncore, m.hop_n, m.coul_n, omega=sig, ham_sign=-1.0)
+            tin, tout = m.pol_trans(ei)[0], m.pol_trans(ef)[1]
+            Din = o.build_transop_i(fn, fi, m.norb, m.ncore, tin)
+            Dout = o.build_transop_f(fi, fn, m.norb, m.ncore, tout)
+            bvec = o.matvec(Din, gs)
+            x, it, _ = o.pminres(A, bvec, args.rixs_tol * np.linalg.norm(bvec), args.rixs_maxit)
+            Hf = o.build_ham_i(fi, m.hop_i, m.coul_i)
+            a, b, nrm = o.build_krylov(Hf, o.matvec(Dout, x), args.nkryl)
+            iters.append(it)
+            ksteps.append(len(a))
+    T["rixs"] = time.perf_counter() - t0
+    measured = T["ed"] + T["xas"] + T["rixs"]
+    c = cpu_costs(m, args.cpu_rows)
+    t_ed, t_xas, t_rixs = cpu_model(c, Hi.nnz, nnz_n, it_ed, xsteps, iters, ksteps)
+    model = t_ed + t_xas + t_rixs
+    return {"norb": m.norb, "rows_i": Hi.m, "rows_n": c["mn"], "nnz_i": Hi.nnz, "nnz_n": nnz_n,
+            "cores": c["cores"], "window_rows": [c["ri"], c["rn"]],
+            "measured_s": {k: round(v, 2) for k, v in T.items()}, "measured_total_s": measured,
+            "model_s": {"ed": t_ed, "xas": t_xas, "rixs": t_rixs}, "model_total_s": model,
+            "model_over_measured": model / measured,
+            "counts": {"lanczos": it_ed, "xas_krylov": xsteps, "minres": iters,
+                       "rixs_krylov": ksteps}}
 
 
 def serve(args):
@@ -751,6 +838,9 @@ def main():
               "multi-GPU layout is a design note, not a measured path (got --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.cpu_check:
+        print(json.dumps({"cpu_check": cpu_check(args)}))
+        return
     import torch
     torch.zeros(1, device="cuda")
     if args.probe:

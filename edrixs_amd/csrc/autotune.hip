@@ -214,11 +214,11 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
     if (K >= 8) {
         for (int npl : {1, 2, 4, 8})
             if (16 * npl <= ((K + 15) / 16) * 16) cand.push_back(BSched{1, npl});
-        for (int bpc : {4, 8})   // pipelined RHS-parallel, workgroups per CU
+        for (int bpc : {4, 5, 8})   // pipelined RHS-parallel, workgroups per CU
             if (bsched_blocks(a, BSched{2, bpc}) <= bspmm_blocks(a->m))
                 cand.push_back(BSched{2, bpc});
         if (packed_spmm_ok(a, precision))   // the same over the packed words, table in LDS
-            for (int bpc : {4, 8})
+            for (int bpc : {4, 5, 8})
                 if (bsched_blocks(a, BSched{4, bpc}) <= bspmm_blocks(a->m))
                     cand.push_back(BSched{4, bpc});
     }

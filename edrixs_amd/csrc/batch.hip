@@ -139,7 +139,30 @@ __global__ __launch_bounds__(256) void k_spmm_rlp(int64_t m, int K, const int64_
             ACC sr = 0, si = 0;
             const int cnt = (int)(end - b);
             const int nf = cnt < 32 ? cnt : 32;
-            for (int t = 0; t < nf; ++t) {
+            // four nonzeros at a time: their columns and values by shuffles,
+            // then four gathers in flight before the sums (in nonzero order)
+            int t = 0;
+            for (; t + 4 <= nf; t += 4) {
+                int c[4];
+                decltype(value(ha)) a[4];
+                XT xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int tt = t + u, src = tt & 15;
+                    c[u] = __shfl(tt < 16 ? ca : cb, src, 16);
+                    a[u] = value(shfl16(tt < 16 ? ha : hb, src));
+                }
+                if (kv) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) xv[u] = X[(int64_t)c[u] * K + k];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        sr += (ACC)a[u].x * (ACC)xv[u].x - (ACC)a[u].y * (ACC)xv[u].y;
+                        si += (ACC)a[u].x * (ACC)xv[u].y + (ACC)a[u].y * (ACC)xv[u].x;
+                    }
+                }
+            }
+            for (; t < nf; ++t) {
                 const int src = t & 15;
                 const int c = __shfl(t < 16 ? ca : cb, src, 16);
                 const auto a = value(shfl16(t < 16 ? ha : hb, src));

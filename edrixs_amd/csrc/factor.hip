@@ -397,20 +397,59 @@ __global__ __launch_bounds__(256) void k_spmm_fac(FacArgs f, const double2 *__re
                     si[k] += a.x * xv.y + a.y * xv.x;
                 }
             };
+            // the values of K right-hand sides at one column, then their sums:
+            // several entries' gathers are issued before any is summed (the
+            // sums still run entry by entry)
+            auto acc = [&](double2 a, const double2 *xv) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    sr[k] += a.x * xv[k].x - a.y * xv[k].y;
+                    si[k] += a.x * xv[k].y + a.y * xv[k].x;
+                }
+            };
             const int64_t kb = f.vrow[v], ke = f.vrow[v + 1];
             int64_t q = kb;
-            for (; q + 2 <= ke; q += 2) {
-                const uint32_t w0 = f.vpk[q], w1 = f.vpk[q + 1];
-                add(s_tab[w0 >> PK_COL_BITS], ((int64_t)(w0 & FAC_COL_MASK) * NC + s) * K);
-                add(s_tab[w1 >> PK_COL_BITS], ((int64_t)(w1 & FAC_COL_MASK) * NC + s) * K);
+            for (; q + 4 <= ke; q += 4) {
+                uint32_t w[4];
+                double2 xv[4][K];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) w[u] = f.vpk[q + u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        xv[u][k] = X[((int64_t)(w[u] & FAC_COL_MASK) * NC + s) * K + k];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc(s_tab[w[u] >> PK_COL_BITS], xv[u]);
             }
-            if (q < ke) {
+            for (; q < ke; ++q) {
                 const uint32_t w = f.vpk[q];
                 add(s_tab[w >> PK_COL_BITS], ((int64_t)(w & FAC_COL_MASK) * NC + s) * K);
             }
             if (f.brow) {
                 const int64_t bb = f.brow[v], be = f.brow[v + 1];
-                for (int64_t p = bb; p < be; ++p) {
+                int64_t p = bb;
+                for (; p + 2 <= be; p += 2) {
+                    uint32_t nn[2];
+                    int64_t base[2];
+                    double2 xv[2][K];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const uint32_t w = f.bpk[p + u];
+                        const uint64_t d = s_typ[w >> FAC_TYPE_SHIFT];
+                        nn[u] = (uint32_t)(d >> (8 * s)) & 0xffu;
+                        base[u] = ((int64_t)(w & FAC_VCOL_MASK) * NC + s + (int)(d >> 48) - (NC - 1)) * K;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            xv[u][k] = nn[u] != 0xffu ? X[base[u] + k] : make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+                        if (nn[u] != 0xffu) acc(s_tab[nn[u]], xv[u]);
+                }
+                for (; p < be; ++p) {
                     const uint32_t w = f.bpk[p];
                     const uint64_t d = s_typ[w >> FAC_TYPE_SHIFT];
                     const uint32_t nn = (uint32_t)(d >> (8 * s)) & 0xffu;

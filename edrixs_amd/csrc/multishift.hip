@@ -38,11 +38,15 @@ struct MsShift {
     double res;        // residual estimate of the last finished iterate (absolute)
     int active;        // still iterating
     int iters;         // iterate the stored projection belongs to
+    // the first step of a pair, saved for the pair's update (k_ms_project)
+    double2 inv_eta_prev, zeta_prev;
+    int active_prev;
 };
 
 struct MsCtl {
     double tol_abs;
     int nactive;
+    int run_prev;      // the first step of the pending pair ran
 };
 
 __device__ __forceinline__ double2 crecip(double2 a) {
@@ -139,6 +143,97 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
     }
 }
 
+// Updates go in pairs of steps, so Q and out are read and written once per two
+// steps: the first step of a pair keeps w = P v_i (scaled) and its shift
+// scalars (k_ms_project), the second applies both steps (k_ms_update2) in the
+// order k_ms_update would, so the results are the same bit for bit.
+__global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *prow,
+                                                    const int32_t *pcol, const double2 *pval,
+                                                    const double2 *v, const double *vscale,
+                                                    double2 *wout, MsShift *sh, int K,
+                                                    const int *run, MsCtl *ctl) {
+    const int on = run[0] && ctl->nactive > 0;
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < K; k += 256) {
+            sh[k].inv_eta_prev = sh[k].inv_eta;
+            sh[k].zeta_prev = sh[k].zeta;
+            sh[k].active_prev = on ? sh[k].active : 0;
+        }
+        if (threadIdx.x == 0) ctl->run_prev = on;
+    }
+    if (!on) return;
+    const double vs = *vscale;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        double2 w;
+        if (prow) {
+            w = make_double2(0, 0);
+            for (int64_t j = prow[r]; j < prow[r + 1]; ++j) w = cadd(w, cmul(pval[j], v[pcol[j]]));
+        } else {
+            w = v[r];
+        }
+        wout[r] = make_double2(w.x * vs, w.y * vs);
+    }
+}
+
+// The pair's update: the saved first step (w1, beta_pp, the *_prev scalars)
+// where it ran, then the current step where with_curr and it runs.
+__global__ __launch_bounds__(256) void k_ms_update2(int64_t mP, const int64_t *prow,
+                                                    const int32_t *pcol, const double2 *pval,
+                                                    const double2 *v, const double *vscale,
+                                                    const double2 *w1in, const double *beta_pp,
+                                                    const double *beta_p, const MsShift *sh,
+                                                    int K, double2 *Q, double2 *out,
+                                                    const int *run, const MsCtl *ctl,
+                                                    int with_curr) {
+    const bool p1 = ctl->run_prev != 0;
+    const bool p2 = with_curr && run[0] && ctl->nactive > 0;
+    if (!p1 && !p2) return;
+    __shared__ double2 s_ie1[MS_MAXK], s_z1[MS_MAXK], s_ie2[MS_MAXK], s_z2[MS_MAXK];
+    __shared__ int s_a1[MS_MAXK], s_a2[MS_MAXK];
+    for (int k = threadIdx.x; k < K; k += 256) {
+        s_ie1[k] = sh[k].inv_eta_prev;
+        s_z1[k] = sh[k].zeta_prev;
+        s_a1[k] = p1 ? sh[k].active_prev : 0;
+        s_ie2[k] = sh[k].inv_eta;
+        s_z2[k] = sh[k].zeta;
+        s_a2[k] = p2 ? sh[k].active : 0;
+    }
+    __syncthreads();
+    const double bpp = p1 ? *beta_pp : 0.0;
+    const double bp = p2 ? *beta_p : 0.0;
+    const double vs = p2 ? *vscale : 0.0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const double2 w1 = p1 ? w1in[r] : make_double2(0, 0);
+        double2 w2 = make_double2(0, 0);
+        if (p2) {
+            if (prow) {
+                for (int64_t j = prow[r]; j < prow[r + 1]; ++j) w2 = cadd(w2, cmul(pval[j], v[pcol[j]]));
+            } else {
+                w2 = v[r];
+            }
+            w2 = make_double2(w2.x * vs, w2.y * vs);
+        }
+        for (int k = 0; k < K; ++k) {
+            if (!s_a1[k] && !s_a2[k]) continue;
+            const int64_t o = (int64_t)k * mP + r;
+            double2 q = Q[o];
+            double2 acc = out[o];
+            if (s_a1[k]) {
+                q = cmul(make_double2(w1.x + bpp * q.x, w1.y + bpp * q.y), s_ie1[k]);
+                acc = cadd(acc, cmul(s_z1[k], q));
+            }
+            if (s_a2[k]) {
+                q = cmul(make_double2(w2.x + bp * q.x, w2.y + bp * q.y), s_ie2[k]);
+                acc = cadd(acc, cmul(s_z2[k], q));
+            }
+            Q[o] = q;
+            out[o] = acc;
+        }
+    }
+}
+
 // The Lanczos vectors are kept unnormalized: u_i with v_i = u_i / s_i and
 // s_i = beta_i (s_1 = 1).  alpha_i = <u_i | y> / s_i for the product
 // y = (1/s_i) H u_i - (beta_i / s_{i-1}) u_{i-1}, and u_{i+1} = y - (alpha_i / s_i) u_i.
@@ -195,7 +290,7 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         const int64_t mp = proj[p] ? proj[p]->m : n;
         if (mp && K) EDX_HIP(hipMemsetAsync(out[p], 0, (size_t)mp * K * sizeof(double2), s));
     }
-    DevBuf vb, x32, part, st, ab, runf, shb, ctlb, qb;
+    DevBuf vb, x32, part, st, ab, runf, shb, ctlb, qb, wb;
     EDX_TRY(vb.alloc(2 * (size_t)n * sizeof(double2), "fom lanczos vectors"));
     if (precision == EDX_C64) EDX_TRY(x32.alloc((size_t)n * sizeof(float2), "fom x32"));
     const int nbv = vec_blocks(n);
@@ -213,6 +308,17 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         qtot += (size_t)(proj[p] ? proj[p]->m : n) * K;
     }
     EDX_TRY(qb.alloc((qtot ? qtot : 1) * sizeof(double2), "projected directions"));
+    // paired updates (k_ms_project / k_ms_update2) unless EDX_MS_UNPAIRED=1;
+    // wb keeps the first step's w = P v per projection
+    const char *unp = getenv("EDX_MS_UNPAIRED");
+    const bool paired = !(unp && unp[0] == '1');
+    size_t wtot = 0;
+    std::vector<size_t> woff(nproj);
+    for (int p = 0; p < nproj; ++p) {
+        woff[p] = wtot;
+        wtot += (size_t)(proj[p] ? proj[p]->m : n);
+    }
+    if (paired) EDX_TRY(wb.alloc((wtot ? wtot : 1) * sizeof(double2), "paired projections"));
     double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + n};
     double *d_alpha = ab.as<double>(), *d_beta = ab.as<double>() + maxit + 1;
     int *run = runf.as<int>();
@@ -249,7 +355,7 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
     hipLaunchKernelGGL(k_div_real, dim3(nbv), dim3(VB), 0, s, n, v[0], &S->sqrt_norm, nullptr);
     EDX_LAUNCH_CHECK();
 
-    int curr = 0, prev = 1, checked = 0;
+    int curr = 0, prev = 1, checked = 0, last = 0;
     for (int i = 1; i <= maxit; ++i) {
         SpmvEpi e{};
         e.y = v[prev];
@@ -268,12 +374,24 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         for (int p = 0; p < nproj; ++p) {
             const edx_csr *P = proj[p];
             const int64_t mp = P ? P->m : n;
-            hipLaunchKernelGGL(k_ms_update, dim3(vec_blocks(mp)), dim3(256), 0, s, mp,
-                               P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
-                               P ? P->d_vals : nullptr, v[curr], &S->coef[0].x,
-                               d_beta + (i - 1), sh, K,
-                               qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl);
+            const int64_t *pr = P ? P->d_rowptr : nullptr;
+            const int32_t *pc = P ? P->d_cols : nullptr;
+            const double2 *pv = P ? (const double2 *)P->d_vals : nullptr;
+            if (!paired)
+                hipLaunchKernelGGL(k_ms_update, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
+                                   pv, v[curr], &S->coef[0].x, d_beta + (i - 1), sh, K,
+                                   qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl);
+            else if (i % 2 == 1)
+                hipLaunchKernelGGL(k_ms_project, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
+                                   pv, v[curr], &S->coef[0].x, wb.as<double2>() + woff[p], sh, K,
+                                   run + (i - 1), ctl);
+            else
+                hipLaunchKernelGGL(k_ms_update2, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
+                                   pv, v[curr], &S->coef[0].x, wb.as<double2>() + woff[p],
+                                   d_beta + (i - 2), d_beta + (i - 1), sh, K,
+                                   qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl, 1);
         }
+        last = i;
         hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
                            &S->acoef, part.as<double2>(), run + (i - 1));
         hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(VB), 0, s, part.as<double2>(), nbv, S, d_beta,
@@ -285,6 +403,17 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
             EDX_HIP(edx::copy_sync(&r, run + i, sizeof(int), hipMemcpyDeviceToHost));
             checked = i;
             if (!r) break;
+        }
+    }
+    if (paired && last % 2 == 1) {   // the last pair has its first step only
+        for (int p = 0; p < nproj; ++p) {
+            const edx_csr *P = proj[p];
+            const int64_t mp = P ? P->m : n;
+            hipLaunchKernelGGL(k_ms_update2, dim3(vec_blocks(mp)), dim3(256), 0, s, mp,
+                               P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
+                               P ? (const double2 *)P->d_vals : nullptr, v[curr], &S->coef[0].x,
+                               wb.as<double2>() + woff[p], d_beta + (last - 1), d_beta + (last - 1),
+                               sh, K, qb.as<double2>() + qoff[p], out[p], run + (last - 1), ctl, 0);
         }
     }
     hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(256), 0, s, S, d_beta, sh, K);
