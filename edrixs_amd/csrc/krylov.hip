@@ -255,6 +255,8 @@ int minres_run(const edx_csr *a, int precision, const double2 *b, double2 *x, do
 
 // ------------------------------------------------------------- Lanczos
 
+constexpr int GEMV_NBX = 256;   // row workgroups per column group of the projections
+
 struct Lanczos {
     const edx_csr *h;
     int precision;
@@ -278,10 +280,12 @@ int host_dot(Lanczos &L, const double2 *a, const double2 *b, double2 *out) {
     return EDX_OK;
 }
 
-// w -= Q Q^H w over the first k basis columns; returns the coefficients in hk (host)
+// w -= Q Q^H w over the first k basis columns; returns the coefficients in hk (host).
+// The projections run on up to GEMV_NBX workgroups per group of 8 columns, so a
+// basis of a few dozen columns still gives the GPU thousands of workgroups.
 int orthogonalize(Lanczos &L, int k, double2 *w, std::vector<double2> *hk) {
     if (k <= 0) return EDX_OK;
-    const int nbx = std::min(L.nbv, 64);
+    const int nbx = std::min(L.nbv, GEMV_NBX);
     dim3 grid(nbx, (k + 7) / 8);
     hipLaunchKernelGGL(k_gemv_ct, grid, dim3(VB), 0, edx::stream(), L.n, L.basis, L.ld, k, w, L.part);
     hipLaunchKernelGGL(k_reduce_cols, dim3((k + 255) / 256), dim3(256), 0, edx::stream(), L.part,
@@ -546,7 +550,7 @@ int lanczos_impl(const edx_csr *h, int precision, int nev, int maxiter, double t
     if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "lanczos x32"));
     EDX_TRY(w.alloc(n * sizeof(double2), "lanczos w"));
     const int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc(((size_t)cap_cols * 64 + nbv + 64) * sizeof(double2), "partials"));
+    EDX_TRY(part.alloc(((size_t)cap_cols * GEMV_NBX + nbv + 64) * sizeof(double2), "partials"));
     EDX_TRY(hv.alloc((size_t)cap_cols * sizeof(double2), "coefficients"));
     EDX_TRY(tmpS.alloc((size_t)cap_cols * cap_cols * sizeof(double), "ritz vectors"));
     EDX_TRY(evec_all.alloc((size_t)n * nev * sizeof(double2), "eigenvectors"));
@@ -787,7 +791,7 @@ int edx_build_krylov_reorth(const edx_csr *h, int precision, const void *d_v0, i
     if (precision == EDX_C64) EDX_TRY(x32.alloc(n * sizeof(float2), "krylov x32"));
     EDX_TRY(w.alloc(n * sizeof(double2), "krylov w"));
     const int nbv = vec_blocks(n);
-    EDX_TRY(part.alloc(((size_t)cap_cols * 64 + nbv + 64) * sizeof(double2), "partials"));
+    EDX_TRY(part.alloc(((size_t)cap_cols * GEMV_NBX + nbv + 64) * sizeof(double2), "partials"));
     EDX_TRY(hv.alloc((size_t)cap_cols * sizeof(double2), "coefficients"));
     Lanczos L{h, precision, n, basis.as<double2>(), n, cap_cols, x32.as<float2>(), w.as<double2>(),
               part.as<double2>(), hv.as<double2>(), nbv};

@@ -102,6 +102,32 @@ __global__ __launch_bounds__(256) void k_ms_scalars(int i, const double *alpha, 
     if (threadIdx.x == 0) ctl->nactive = cnt[0];
 }
 
+// (P v)[r] for a CSR P (nullptr rowptr: the identity), four entries at a time
+// (their columns and values, then four gathers in flight), summed in entry
+// order.
+__device__ __forceinline__ double2 ms_proj_row(int64_t r, const int64_t *prow, const int32_t *pcol,
+                                               const double2 *pval, const double2 *v) {
+    if (!prow) return v[r];
+    double2 w = make_double2(0, 0);
+    int64_t j = prow[r];
+    const int64_t je = prow[r + 1];
+    for (; j + 4 <= je; j += 4) {
+        int c[4];
+        double2 a[4], x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = pcol[j + u];
+            a[u] = pval[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = v[c[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w = cadd(w, cmul(a[u], x[u]));
+    }
+    for (; j < je; ++j) w = cadd(w, cmul(pval[j], v[pcol[j]]));
+    return w;
+}
+
 // q_k = (P v + beta_{i-1} q_k) / eta_k and out_k += zeta_k q_k for the active
 // shifts; P is a CSR with FP64 values (nullptr rowptr: the identity).
 // Q and out are mP x K column-major.
@@ -124,13 +150,7 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
     const double vs = *vscale;   // v_i = vscale * u_i (the stored vector is unnormalized)
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
-        double2 w;
-        if (prow) {
-            w = make_double2(0, 0);
-            for (int64_t j = prow[r]; j < prow[r + 1]; ++j) w = cadd(w, cmul(pval[j], v[pcol[j]]));
-        } else {
-            w = v[r];
-        }
+        double2 w = ms_proj_row(r, prow, pcol, pval, v);
         w = make_double2(w.x * vs, w.y * vs);
         for (int k = 0; k < K; ++k) {
             if (!s_act[k]) continue;
@@ -165,13 +185,7 @@ __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *p
     const double vs = *vscale;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
-        double2 w;
-        if (prow) {
-            w = make_double2(0, 0);
-            for (int64_t j = prow[r]; j < prow[r + 1]; ++j) w = cadd(w, cmul(pval[j], v[pcol[j]]));
-        } else {
-            w = v[r];
-        }
+        const double2 w = ms_proj_row(r, prow, pcol, pval, v);
         wout[r] = make_double2(w.x * vs, w.y * vs);
     }
 }
@@ -208,28 +222,41 @@ __global__ __launch_bounds__(256) void k_ms_update2(int64_t mP, const int64_t *p
         const double2 w1 = p1 ? w1in[r] : make_double2(0, 0);
         double2 w2 = make_double2(0, 0);
         if (p2) {
-            if (prow) {
-                for (int64_t j = prow[r]; j < prow[r + 1]; ++j) w2 = cadd(w2, cmul(pval[j], v[pcol[j]]));
-            } else {
-                w2 = v[r];
-            }
+            w2 = ms_proj_row(r, prow, pcol, pval, v);
             w2 = make_double2(w2.x * vs, w2.y * vs);
         }
-        for (int k = 0; k < K; ++k) {
-            if (!s_a1[k] && !s_a2[k]) continue;
-            const int64_t o = (int64_t)k * mP + r;
-            double2 q = Q[o];
-            double2 acc = out[o];
-            if (s_a1[k]) {
-                q = cmul(make_double2(w1.x + bpp * q.x, w1.y + bpp * q.y), s_ie1[k]);
-                acc = cadd(acc, cmul(s_z1[k], q));
+        // four shifts at a time: their Q and out loads in flight together
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            double2 q[4], acc[4];
+            bool act[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = k0 + u;
+                act[u] = k < K && (s_a1[k] || s_a2[k]);
+                if (act[u]) {
+                    q[u] = Q[(int64_t)k * mP + r];
+                    acc[u] = out[(int64_t)k * mP + r];
+                }
             }
-            if (s_a2[k]) {
-                q = cmul(make_double2(w2.x + bp * q.x, w2.y + bp * q.y), s_ie2[k]);
-                acc = cadd(acc, cmul(s_z2[k], q));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = k0 + u;
+                if (!act[u]) continue;
+                if (s_a1[k]) {
+                    q[u] = cmul(make_double2(w1.x + bpp * q[u].x, w1.y + bpp * q[u].y), s_ie1[k]);
+                    acc[u] = cadd(acc[u], cmul(s_z1[k], q[u]));
+                }
+                if (s_a2[k]) {
+                    q[u] = cmul(make_double2(w2.x + bp * q[u].x, w2.y + bp * q[u].y), s_ie2[k]);
+                    acc[u] = cadd(acc[u], cmul(s_z2[k], q[u]));
+                }
             }
-            Q[o] = q;
-            out[o] = acc;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!act[u]) continue;
+                Q[(int64_t)(k0 + u) * mP + r] = q[u];
+                out[(int64_t)(k0 + u) * mP + r] = acc[u];
+            }
         }
     }
 }
