@@ -563,25 +563,7 @@ __global__ void k_baxpy_norm(int64_t n, int K, double2 *y, const double2 *x, con
     double2 s = make_double2(0, 0);
     if (kv) {
         const double al = alpha_i[k];
-        // four rows per pass (their loads in flight together), accumulated in
-        // row order as one row per pass would
-        int64_t r = rstart;
-        for (; r + 3 * rstride < n; r += 4 * rstride) {
-            double2 yv[4], xv[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int64_t o = (r + u * rstride) * K + k;
-                yv[u] = y[o];
-                xv[u] = x[o];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double2 v = make_double2(yv[u].x - al * xv[u].x, yv[u].y - al * xv[u].y);
-                y[(r + u * rstride) * K + k] = v;
-                s.x += v.x * v.x + v.y * v.y;
-            }
-        }
-        for (; r < n; r += rstride) {
+        for (int64_t r = rstart; r < n; r += rstride) {
             const int64_t o = r * K + k;
             const double2 v = make_double2(y[o].x - al * x[o].x, y[o].y - al * x[o].y);
             y[o] = v;

@@ -70,24 +70,7 @@ __global__ __launch_bounds__(VB) void k_axpy_norm(int64_t n, double2 *y, const d
     if (run && *run == 0) return;
     const double al = *a;
     double2 s = make_double2(0, 0);
-    const int64_t st = (int64_t)gridDim.x * VB;
-    int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x;
-    // four elements per pass (loads in flight together), summed in order
-    for (; i + 3 * st < n; i += 4 * st) {
-        double2 yv[4], xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            yv[u] = y[i + u * st];
-            xv[u] = x[i + u * st];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double2 v = make_double2(yv[u].x - al * xv[u].x, yv[u].y - al * xv[u].y);
-            y[i + u * st] = v;
-            s.x += v.x * v.x + v.y * v.y;
-        }
-    }
-    for (; i < n; i += st) {
+    for (int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) {
         double2 v = make_double2(y[i].x - al * x[i].x, y[i].y - al * x[i].y);
         y[i] = v;
         s.x += v.x * v.x + v.y * v.y;
