@@ -119,7 +119,7 @@ def rixs_siam(model, ominc, eloss, pol_vectors, workdir=".", gamma_c=0.5, gamma_
 
 
 def search_gs_sector(norb, hop, coul, imp_orbs, precision="c128i", tol=1e-10, maxiter=500,
-                     log=None):
+                     log=None, concurrent=True):
     """Particle-number sector of the ground state: the ``do_ed=0`` search of
     ``ed_siam_fort`` (edrixs/solvers.py:2156-2263), in memory on the device.
 
@@ -132,42 +132,94 @@ def search_gs_sector(norb, hop, coul, imp_orbs, precision="c128i", tol=1e-10, ma
     table a list of (N, E0, impurity occupation) sorted by energy, as the
     reference writes to search_result.dat; the impurity occupation is the trace
     of the ground state's density matrix over the orbitals ``imp_orbs``.  With
-    ``log`` (a writable file) each sector is printed as it is evaluated, like
-    search_gs.log."""
+    ``log`` (a writable file) each sector is printed as the walk takes it, like
+    search_gs.log.
+
+    With ``concurrent`` the first three sectors, and then the two sectors of
+    each step while both directions are open, are evaluated at once on two
+    worker threads with their own streams; the walk then takes them in the
+    reference's order and drops a sector it would not have evaluated, so the
+    result and the log are those of the sequential walk."""
     from . import core
 
     imp = np.asarray(list(imp_orbs), dtype=int)
+    done = {}
 
     def sector(n):
         b = core.Basis.combinations(norb, n)
         H = core.build_ham_i(b, hop, coul)
         ev, vecs, _ = core.ground_state(H, nev=1, precision=precision, maxiter=maxiter, tol=tol)
         occ = float(np.real(np.trace(core.denmat(b, norb, vecs[0].contiguous())[np.ix_(imp, imp)])))
-        row = (n, float(ev[0]), occ)
+        return (n, float(ev[0]), occ)
+
+    pool = None
+    if concurrent:
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+        import torch as _torch
+        local = threading.local()
+
+        def on_stream(n):
+            if not hasattr(local, "stream"):
+                local.stream = _torch.cuda.Stream()     # default priority
+            s_ = local.stream
+            core.set_stream(s_)
+            try:
+                with _torch.cuda.stream(s_):
+                    r = sector(n)
+                    s_.synchronize()
+            finally:
+                core.set_stream(None)
+            return r
+
+        pool = ThreadPoolExecutor(max_workers=2)
+
+    def prefetch(ns):
+        """Start the sectors of ns not yet evaluated (concurrent mode)."""
+        if pool is None:
+            return
+        for n in ns:
+            if 0 <= n <= norb and n not in done:
+                done[n] = pool.submit(on_stream, n)
+
+    def take(n):
+        """The walk evaluates sector n (logging it in walk order)."""
+        if pool is None:
+            r = sector(n)
+        else:
+            prefetch([n])
+            r = done[n].result()
         if log is not None:
-            print("%d %.10f %.10f" % row, file=log, flush=True)
-        return row
+            print("%d %.10f %.10f" % r, file=log, flush=True)
+        return r
 
     n0 = norb // 2
-    res = [sector(n0)]
-    n_gs, e_gs = n0, res[0][1]
-    up = down = True
-    for i in range(norb // 2):
-        if up:
-            r = sector(n0 + i + 1)
-            res.append(r)
-            if r[1] > e_gs:
-                up = False
-            else:
-                down = False
-                n_gs, e_gs = r[0], r[1]
-        if down:
-            r = sector(n0 - i - 1)
-            res.append(r)
-            if r[1] > e_gs:
-                down = False
-            else:
-                up = False
-                n_gs, e_gs = r[0], r[1]
+    try:
+        prefetch([n0, n0 + 1, n0 - 1])
+        res = [take(n0)]
+        n_gs, e_gs = n0, res[0][1]
+        up = down = True
+        for i in range(norb // 2):
+            if up and down:
+                prefetch([n0 + i + 1, n0 - i - 1])
+            if up:
+                r = take(n0 + i + 1)
+                res.append(r)
+                if r[1] > e_gs:
+                    up = False
+                else:
+                    down = False
+                    n_gs, e_gs = r[0], r[1]
+            if down:
+                r = take(n0 - i - 1)
+                res.append(r)
+                if r[1] > e_gs:
+                    down = False
+                else:
+                    up = False
+                    n_gs, e_gs = r[0], r[1]
+    finally:
+        if pool is not None:
+            pool.shutdown(wait=True)
     res.sort(key=lambda t: t[1])
     return n_gs, e_gs, res

@@ -1350,7 +1350,19 @@ def test_gs_sector_search_matches_dense_sectors(core):
         fock = H.fock_words(m.norb, n)
         e_dense[n] = float(np.linalg.eigvalsh(o.build_ham_i(fock, hop, coul).todense())[0])
     n_best = min(e_dense, key=e_dense.get)
-    n_gs, e_gs, table = solvers.search_gs_sector(m.norb, hop, coul, imp_orbs=range(10))
+    import io
+    logs = {}
+    for conc in (True, False):
+        logs[conc] = io.StringIO()
+        out = solvers.search_gs_sector(m.norb, hop, coul, imp_orbs=range(10), log=logs[conc],
+                                       concurrent=conc)
+        if conc:
+            n_gs, e_gs, table = out
+        else:
+            # the sectors evaluated two at a time on their own streams give the
+            # sequential walk's result and log exactly
+            assert out == (n_gs, e_gs, table)
+            assert logs[True].getvalue() == logs[False].getvalue()
     assert n_gs == n_best and n_gs != m.norb // 2
     assert abs(e_gs - e_dense[n_best]) < 1e-9
     for n, e, occ in table:
