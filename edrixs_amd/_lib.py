@@ -70,6 +70,7 @@ _SIGS = {
     "edx_krylov_batch_start": (_I, [_P, _I, _I, _P, _I, _PP]),
     "edx_krylov_batch_advance": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "edx_krylov_batch_read": (_I, [_P, _I, _P, _P, _P, _P]),
+    "edx_krylov_batch_keep": (_I, [_P, _I, _P]),
     "edx_krylov_batch_free": (None, [_P]),
     "edx_mf_ham_i": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _D, _PP]),
     "edx_mf_ham_n": (_I, [_P, _I, _I, _P, _I64, _P, _I64, _D, _D, _D, _PP]),

@@ -476,6 +476,13 @@ class KrylovBatch:
         return ([alpha[k, :neff[k]].copy() for k in range(K)],
                 [beta[k, :neff[k]].copy() for k in range(K)], norm)
 
+    def keep(self, indices):
+        """Drop every recurrence but the rows ``indices`` (increasing) of the
+        current batch; the kept ones continue unchanged as rows 0..len-1."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int32).reshape(-1))
+        _lib.call("edx_krylov_batch_keep", self._h, len(idx), _ptr(idx))
+        self.K = len(idx)
+
     def close(self):
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None and _lib._lib is not None:
@@ -610,7 +617,7 @@ SPLIT_BLOCK_BYTES = 192 << 20
 
 
 def converged_spectra(H, V0, e_gs, grid, gamma, precision="c128", tol=1e-8, chunk=250,
-                      min_steps=250, max_steps=4000):
+                      min_steps=250, max_steps=4000, compact=True):
     """Continued-fraction spectra of the K start vectors (rows of V0), each taken at
     the first multiple of ``chunk`` steps (>= min_steps) where its relative L2
     change since the previous multiple is <= tol, or where its recurrence broke
@@ -619,7 +626,9 @@ def converged_spectra(H, V0, e_gs, grid, gamma, precision="c128", tol=1e-8, chun
     8 vectors whose interleaved block exceeds SPLIT_BLOCK_BYTES advance as
     separate single-vector recurrences, except 2 to 4 vectors over a matrix
     with the slot-factored form in the indexed precision, whose block product
-    gathers contiguous segments of all vectors."""
+    gathers contiguous segments of all vectors.  With ``compact`` a block
+    drops the recurrences whose spectra are final (KrylovBatch.keep), so the
+    later products cover only the vectors still running."""
     V0 = V0.contiguous()
     K = V0.shape[0]
     # a block of up to 4 vectors over a matrix with the slot-factored form stays
@@ -642,7 +651,8 @@ def converged_spectra(H, V0, e_gs, grid, gamma, precision="c128", tol=1e-8, chun
     try:
         while True:
             target = min(max(m + chunk, min_steps), max_steps)
-            for g, kb in zip(groups, kbs):
+            for gi, kb in enumerate(kbs):
+                g = groups[gi]
                 if all(out[k] is not None for k in g):
                     continue
                 got = kb.advance(target - kb.done)
@@ -661,6 +671,10 @@ def converged_spectra(H, V0, e_gs, grid, gamma, precision="c128", tol=1e-8, chun
                         if ended:
                             change[k] = 0.0
                     prev[k] = s
+                live = [j for j, k in enumerate(g) if out[k] is None]
+                if compact and live and len(live) < len(g):
+                    kb.keep(live)
+                    groups[gi] = [g[j] for j in live]
             m = target
             if all(o is not None for o in out):
                 break

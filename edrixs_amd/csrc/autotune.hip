@@ -211,17 +211,18 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
         }
     }
     std::vector<BSched> cand = {BSched{0, 8}, BSched{0, 16}};
-    if (K >= 8) {
+    if (K >= 8)
         for (int npl : {1, 2, 4, 8})
             if (16 * npl <= ((K + 15) / 16) * 16) cand.push_back(BSched{1, npl});
-        for (int bpc : {4, 5, 8})   // pipelined RHS-parallel, workgroups per CU
-            if (bsched_blocks(a, BSched{2, bpc}) <= bspmm_blocks(a->m))
-                cand.push_back(BSched{2, bpc});
-        if (packed_spmm_ok(a, precision))   // the same over the packed words, table in LDS
-            for (int bpc : {4, 5, 8})
-                if (bsched_blocks(a, BSched{4, bpc}) <= bspmm_blocks(a->m))
-                    cand.push_back(BSched{4, bpc});
-    }
+    // the pipelined RHS-parallel schedules for every K: a block compacted below
+    // 8 vectors keeps them (lanes past K of each 16-lane group idle)
+    for (int bpc : {4, 5, 8})   // workgroups per CU
+        if (bsched_blocks(a, BSched{2, bpc}) <= bspmm_blocks(a->m))
+            cand.push_back(BSched{2, bpc});
+    if (packed_spmm_ok(a, precision))   // the same over the packed words, table in LDS
+        for (int bpc : {4, 5, 8})
+            if (bsched_blocks(a, BSched{4, bpc}) <= bspmm_blocks(a->m))
+                cand.push_back(BSched{4, bpc});
     if (fac_spmm_ok(a, precision, K))   // slot-factored, workgroups per CU
         for (int bpc : {4, 8})
             if (bsched_blocks(a, BSched{3, bpc}) <= bspmm_blocks(a->m))

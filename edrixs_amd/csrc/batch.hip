@@ -491,6 +491,20 @@ __global__ void k_il_to_cm(int64_t n, int K, const double2 *il, double2 *cm) {
     for (int64_t r = rstart; r < n; r += rstride) cm[(int64_t)k * n + r] = il[r * K + k];
 }
 
+// interleaved block of K (n rows) -> interleaved block of the Kn columns
+// keep[0..Kn): launched with elaunch(n, Kn)
+__global__ void k_il_keep(int64_t n, int K, int Kn, const int *keep, const double2 *in,
+                          double2 *out) {
+    const int kte = blockDim.x / ESLOTS;
+    const int kk = threadIdx.x % kte, slot = threadIdx.x / kte;
+    const int k = blockIdx.y * 16 + kk;
+    if (k >= Kn) return;
+    const int src = keep[k];
+    const int64_t rstride = (int64_t)gridDim.x * ESLOTS;
+    for (int64_t r = (int64_t)blockIdx.x * ESLOTS + slot; r < n; r += rstride)
+        out[r * Kn + k] = in[r * K + src];
+}
+
 // a -= b
 __global__ void k_bsub(int64_t n, int K, double2 *a, const double2 *b) {
     ELEM_SETUP
@@ -880,16 +894,15 @@ int bsched_blocks(const edx_csr *a, BSched sched);
 // One launch of the multi-RHS product with an explicit schedule: row-split
 // (sched.rl == 0: sched.p lanes per row, tiles of BKT right-hand sides) or
 // RHS-parallel (sched.rl == 1: tiles of 16 * sched.p right-hand sides).
-// EDX_FORCE_BSCHED="rl:p" replaces the chosen multi-RHS schedule of K >= 8
-// products (rl 0 row-split with p lanes, 1 RHS-parallel with tiles of 16 p, 2
-// pipelined RHS-parallel with p workgroups per CU): for tests that exercise
-// every schedule on small matrices.
+// EDX_FORCE_BSCHED="rl:p" replaces the chosen multi-RHS schedule (rl 0
+// row-split with p lanes, 1 RHS-parallel with tiles of 16 p for K >= 8, 2 and
+// 4 pipelined RHS-parallel with p workgroups per CU, 3 slot-factored for
+// K < 8): for tests that exercise every schedule on small matrices.
 void forced_bsched(int K, BSched *sched) {
     const char *f = getenv("EDX_FORCE_BSCHED");
     int rl = 0, p = 0;
-    if (K >= 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && (rl <= 2 || rl == 4) && rl >= 0 && p > 0)
-        *sched = BSched{rl, p};
-    if (K < 8 && f && sscanf(f, "%d:%d", &rl, &p) == 2 && (rl == 0 || rl == 3) && p > 0)
+    if (!f || sscanf(f, "%d:%d", &rl, &p) != 2 || p <= 0) return;
+    if (rl == 0 || rl == 2 || rl == 4 || (K >= 8 && rl == 1) || (K < 8 && rl == 3))
         *sched = BSched{rl, p};
 }
 
@@ -1413,6 +1426,79 @@ int kb_advance(edx_krylov_batch *q, int nsteps) {
     return EDX_OK;
 }
 
+// Keep the recurrences keep[0..nkeep) (strictly increasing) and drop the rest:
+// both vector blocks are gathered on the device into blocks of nkeep, the
+// scalar histories of the steps taken are rearranged on the host.  A kept
+// recurrence continues with the same vectors and scalars, so with the same
+// product schedule its later steps equal those of the uncompacted batch.
+int kb_keep(edx_krylov_batch *q, int nkeep, const int *keep) {
+    const int K = q->K;
+    for (int j = 0; j < nkeep; ++j)
+        EDX_CHECK(keep[j] >= 0 && keep[j] < K && (j == 0 || keep[j] > keep[j - 1]), EDX_ERR_ARG,
+                  "keep[%d] = %d: indices must increase strictly within [0, %d)", j, keep[j], K);
+    if (nkeep == K) return EDX_OK;
+    const int64_t n = q->n;
+    const int done = q->done;
+    Dbuf vb, x32, kd;
+    EDX_TRY(vb.alloc((size_t)2 * n * nkeep * sizeof(double2), "batched krylov vectors"));
+    if (q->precision == EDX_C64)
+        EDX_TRY(x32.alloc((size_t)n * nkeep * sizeof(float2), "batched x32"));
+    EDX_TRY(kd.alloc(nkeep * sizeof(int), "kept columns"));
+    EDX_HIP(edx::copy_sync(kd.p, keep, nkeep * sizeof(int), hipMemcpyHostToDevice));
+    const double2 *src = q->vb.as<double2>();
+    double2 *dst = vb.as<double2>();
+    for (int h = 0; h < 2; ++h)
+        ELAUNCH(k_il_keep, n, nkeep, n, K, nkeep, kd.as<int>(), src + (size_t)h * n * K,
+                dst + (size_t)h * n * nkeep);
+    EDX_LAUNCH_CHECK();
+    // the history of the steps taken: alpha_1..done, beta_0..done, run_0..done
+    const BKrylov &st = q->st;
+    std::vector<double> ha((size_t)done * K), hb((size_t)(done + 1) * K), ht((size_t)BK_TAIL * K);
+    std::vector<int> hr((size_t)(done + 1) * K), hn(K);
+    if (done > 0)
+        EDX_HIP(edx::copy_sync(ha.data(), st.alpha, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hb.data(), st.beta, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(ht.data(), st.norm, ht.size() * sizeof(double), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hr.data(), st.run, hr.size() * sizeof(int), hipMemcpyDeviceToHost));
+    EDX_HIP(edx::copy_sync(hn.data(), st.neff, K * sizeof(int), hipMemcpyDeviceToHost));
+    auto pick = [&](const auto &in, int rows, int width) {
+        std::vector<typename std::decay_t<decltype(in)>::value_type> out((size_t)rows * nkeep * width);
+        for (int i = 0; i < rows; ++i)
+            for (int j = 0; j < nkeep; ++j)
+                for (int t = 0; t < width; ++t)
+                    out[((size_t)i * nkeep + j) * width + t] = in[((size_t)i * K + keep[j]) * width + t];
+        return out;
+    };
+    const std::vector<double> na = pick(ha, done, 1), nb = pick(hb, done + 1, 1);
+    const std::vector<int> nr = pick(hr, done + 1, 1), nn = pick(hn, 1, 1);
+    // the tail holds per-RHS fields of 1, 2, 2, 2, 1 and 1 doubles in turn
+    std::vector<double> nt((size_t)BK_TAIL * nkeep);
+    size_t off_old = 0, off_new = 0;
+    for (int w : {1, 2, 2, 2, 1, 1}) {
+        for (int j = 0; j < nkeep; ++j)
+            for (int t = 0; t < w; ++t)
+                nt[off_new + (size_t)j * w + t] = ht[off_old + (size_t)keep[j] * w + t];
+        off_old += (size_t)w * K;
+        off_new += (size_t)w * nkeep;
+    }
+    q->K = nkeep;
+    const int rc = kb_layout(q, q->cap, nullptr, nullptr, 0);
+    if (rc) {
+        q->K = K;
+        return rc;
+    }
+    const BKrylov &ns = q->st;
+    if (done > 0)
+        EDX_HIP(edx::copy_sync(ns.alpha, na.data(), na.size() * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(ns.beta, nb.data(), nb.size() * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(ns.norm, nt.data(), nt.size() * sizeof(double), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(ns.run, nr.data(), nr.size() * sizeof(int), hipMemcpyHostToDevice));
+    EDX_HIP(edx::copy_sync(ns.neff, nn.data(), nkeep * sizeof(int), hipMemcpyHostToDevice));
+    std::swap(q->vb.p, vb.p);        // the old blocks are freed with the locals
+    std::swap(q->x32.p, x32.p);
+    return EDX_OK;
+}
+
 // alpha / beta of the first count steps, nrhs x count row-major (beta_1..beta_count)
 int kb_read(const edx_krylov_batch *q, int count, int *neff, double *alpha, double *beta,
             double *norm) {
@@ -1658,6 +1744,11 @@ int edx_krylov_batch_advance(edx_krylov_batch *q, int nsteps, int *steps_done) {
     EDX_TRY(kb_advance(q, nsteps));
     if (steps_done) *steps_done = q->done;
     return EDX_OK;
+}
+
+int edx_krylov_batch_keep(edx_krylov_batch *q, int nkeep, const int *keep) {
+    EDX_CHECK(q && keep && nkeep >= 1 && nkeep <= q->K, EDX_ERR_ARG, "bad argument");
+    return kb_keep(q, nkeep, keep);
 }
 
 int edx_krylov_batch_read(const edx_krylov_batch *q, int count, int *neff, double *alpha,

@@ -344,6 +344,12 @@ int edx_krylov_batch_start(const edx_csr *h, int precision, int nrhs, const void
 int edx_krylov_batch_advance(edx_krylov_batch *state, int nsteps, int *steps_done);
 int edx_krylov_batch_read(const edx_krylov_batch *state, int count, int *neff, double *alpha,
                           double *beta, double *norm);
+/* Keep the recurrences keep[0..nkeep) (indices into the current columns,
+ * strictly increasing) and drop the others, so that a caller whose spectra
+ * of some columns have converged stops paying for their products; the kept
+ * recurrences continue unchanged and become columns 0..nkeep-1 (no reference
+ * counterpart: build_krylov_mp, src/lanczos.f90:266-298, runs one vector). */
+int edx_krylov_batch_keep(edx_krylov_batch *state, int nkeep, const int *keep);
 void edx_krylov_batch_free(edx_krylov_batch *state);
 
 /* Grouped recurrences for a serving loop (no reference counterpart): many

@@ -675,6 +675,66 @@ def test_krylov_batch_resumes_bit_exact(core):
     kb.close()
 
 
+@pytest.mark.parametrize("precision", ["c128", "c64", "c128i"])
+def test_krylov_batch_keep_continues_the_kept_recurrences(core, monkeypatch, precision):
+    # a batch compacted to some of its vectors (6 -> 3 -> 1) continues each
+    # kept recurrence exactly as the full batch does: with one product schedule
+    # for every K (forced: the pipelined RHS-parallel one, whose per-vector sums
+    # and dot partials do not depend on K), the coefficients are bit-identical
+    m, fn, bn, A, R = _siam_hn(core)
+    A.prepare(precision)
+    rng = np.random.default_rng(33)
+    V0 = rng.normal(size=(6, A.n)) + 1j * rng.normal(size=(6, A.n))
+    Vd = _to_dev(core, V0)
+    monkeypatch.setenv("EDX_FORCE_BSCHED", "2:4")
+    full = core.KrylovBatch(A, Vd, precision, capacity=16)
+    assert full.advance(80) == 80
+    af, bf, nf = full.coefficients()
+    full.close()
+    kb = core.KrylovBatch(A, Vd, precision, capacity=16)
+    assert kb.advance(30) == 30
+    kb.keep([1, 3, 4])
+    assert kb.K == 3
+    a1, b1, n1 = kb.coefficients()
+    for j, k in enumerate([1, 3, 4]):
+        assert np.array_equal(a1[j], af[k][:30]) and np.array_equal(b1[j], bf[k][:30])
+        assert n1[j] == nf[k]
+    with pytest.raises(Exception, match="increase strictly"):
+        kb.keep([1, 0])                  # refused before anything changes
+    assert kb.K == 3
+    assert kb.advance(30) == 60          # grows its history after the compaction
+    kb.keep([2])
+    assert kb.advance(20) == 80
+    a2, b2, n2 = kb.coefficients()
+    assert np.array_equal(a2[0], af[4]) and np.array_equal(b2[0], bf[4]) and n2[0] == nf[4]
+    kb.close()
+    monkeypatch.delenv("EDX_FORCE_BSCHED")
+
+
+def test_converged_spectra_compaction_changes_nothing(core, monkeypatch):
+    # converged_spectra drops the recurrences whose spectra are final; with one
+    # product schedule for every K the spectra and step counts equal those of
+    # the uncompacted block bit for bit
+    m, fn, bn, A, R = _siam_hn(core)
+    A.prepare("c128i")
+    ev, vecs, _ = core.lanczos_eigs(A, 1, tol=1e-12)
+    rng = np.random.default_rng(35)
+    V0 = rng.normal(size=(8, A.n)) + 1j * rng.normal(size=(8, A.n))
+    V0[5] = vecs[0].cpu().numpy()         # a near-eigenvector: its spectrum is final first
+    V0[2] = V0[5] + 1e-3 * V0[2]
+    Vd = _to_dev(core, V0)
+    om = np.linspace(ev[0] - 5.0, ev[0] + 40.0, 301)
+    monkeypatch.setenv("EDX_FORCE_BSCHED", "2:4")
+    runs = [core.converged_spectra(A, Vd, ev[0], om, 0.3, precision="c128i", tol=1e-9, chunk=20,
+                                   min_steps=20, max_steps=400, compact=c) for c in (False, True)]
+    monkeypatch.delenv("EDX_FORCE_BSCHED")
+    (s0, st0, c0), (s1, st1, c1) = runs
+    assert st0 == st1 and c0 == c1
+    assert len(set(st0)) > 1                                  # members stopped apart
+    for k in range(8):
+        assert np.array_equal(s0[k], s1[k]), k
+
+
 def test_converged_spectra_equal_pole_sum(core):
     m = H.small_siam(nbath=2, seed=8)
     fi = H.fock_words(m.norb, m.nocc)
