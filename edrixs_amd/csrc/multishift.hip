@@ -38,15 +38,11 @@ struct MsShift {
     double res;        // residual estimate of the last finished iterate (absolute)
     int active;        // still iterating
     int iters;         // iterate the stored projection belongs to
-    // the first step of a pair, saved for the pair's update (k_ms_project)
-    double2 inv_eta_prev, zeta_prev;
-    int active_prev;
 };
 
 struct MsCtl {
     double tol_abs;
     int nactive;
-    int run_prev;      // the first step of the pending pair ran
 };
 
 __device__ __forceinline__ double2 crecip(double2 a) {
@@ -128,59 +124,28 @@ __device__ __forceinline__ double2 ms_proj_row(int64_t r, const int64_t *prow, c
     return w;
 }
 
-// q_k = (P v + beta_{i-1} q_k) / eta_k and out_k += zeta_k q_k for the active
-// shifts; P is a CSR with FP64 values (nullptr rowptr: the identity).
-// Q and out are mP x K column-major.
-__global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *prow,
-                                                   const int32_t *pcol, const double2 *pval,
-                                                   const double2 *v, const double *vscale,
-                                                   const double *beta_prev,
-                                                   const MsShift *sh, int K, double2 *Q,
-                                                   double2 *out, const int *run, const MsCtl *ctl) {
-    if (!run[0] || ctl->nactive == 0) return;
-    __shared__ double2 s_ie[MS_MAXK], s_z[MS_MAXK];
-    __shared__ int s_act[MS_MAXK];
-    for (int k = threadIdx.x; k < K; k += 256) {
-        s_ie[k] = sh[k].inv_eta;
-        s_z[k] = sh[k].zeta;
-        s_act[k] = sh[k].active;
-    }
-    __syncthreads();
-    const double bp = *beta_prev;
-    const double vs = *vscale;   // v_i = vscale * u_i (the stored vector is unnormalized)
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        double2 w = ms_proj_row(r, prow, pcol, pval, v);
-        w = make_double2(w.x * vs, w.y * vs);
-        for (int k = 0; k < K; ++k) {
-            if (!s_act[k]) continue;
-            const int64_t o = (int64_t)k * mP + r;
-            const double2 qo = Q[o];
-            const double2 q = cmul(make_double2(w.x + bp * qo.x, w.y + bp * qo.y), s_ie[k]);
-            Q[o] = q;
-            out[o] = cadd(out[o], cmul(s_z[k], q));
-        }
-    }
-}
+// Projected updates go in windows of W steps, so Q and out are read and
+// written once per window: each of the first W-1 steps keeps w = s P v_i in a
+// slot with its shift scalars (k_ms_project), the last applies every step of
+// the window (k_ms_update), per shift in step order, as a per-step update
+// would: q_k = (w + beta_{i-1} q_k) / eta_k, out_k += zeta_k q_k for the
+// shifts active at that step.  P is a CSR with FP64 values (nullptr rowptr:
+// the identity); Q and out are mP x K column-major.
+struct MsSlot {
+    double2 inv_eta, zeta;
+    int active;      // the shift took this step (0 where the step did not run)
+};
 
-// Updates go in pairs of steps, so Q and out are read and written once per two
-// steps: the first step of a pair keeps w = P v_i (scaled) and its shift
-// scalars (k_ms_project), the second applies both steps (k_ms_update2) in the
-// order k_ms_update would, so the results are the same bit for bit.
 __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *prow,
                                                     const int32_t *pcol, const double2 *pval,
                                                     const double2 *v, const double *vscale,
-                                                    double2 *wout, MsShift *sh, int K,
-                                                    const int *run, MsCtl *ctl) {
+                                                    double2 *wout, MsSlot *slot,
+                                                    const MsShift *sh, int K, const int *run,
+                                                    const MsCtl *ctl) {
     const int on = run[0] && ctl->nactive > 0;
-    if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < K; k += 256) {
-            sh[k].inv_eta_prev = sh[k].inv_eta;
-            sh[k].zeta_prev = sh[k].zeta;
-            sh[k].active_prev = on ? sh[k].active : 0;
-        }
-        if (threadIdx.x == 0) ctl->run_prev = on;
-    }
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < K; k += 256)
+            slot[k] = MsSlot{sh[k].inv_eta, sh[k].zeta, on ? sh[k].active : 0};
     if (!on) return;
     const double vs = *vscale;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
@@ -190,40 +155,61 @@ __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *p
     }
 }
 
-// The pair's update: the saved first step (w1, beta_pp, the *_prev scalars)
-// where it ran, then the current step where with_curr and it runs.
-__global__ __launch_bounds__(256) void k_ms_update2(int64_t mP, const int64_t *prow,
-                                                    const int32_t *pcol, const double2 *pval,
-                                                    const double2 *v, const double *vscale,
-                                                    const double2 *w1in, const double *beta_pp,
-                                                    const double *beta_p, const MsShift *sh,
-                                                    int K, double2 *Q, double2 *out,
-                                                    const int *run, const MsCtl *ctl,
-                                                    int with_curr) {
-    const bool p1 = ctl->run_prev != 0;
-    const bool p2 = with_curr && run[0] && ctl->nactive > 0;
-    if (!p1 && !p2) return;
-    __shared__ double2 s_ie1[MS_MAXK], s_z1[MS_MAXK], s_ie2[MS_MAXK], s_z2[MS_MAXK];
-    __shared__ int s_a1[MS_MAXK], s_a2[MS_MAXK];
-    for (int k = threadIdx.x; k < K; k += 256) {
-        s_ie1[k] = sh[k].inv_eta_prev;
-        s_z1[k] = sh[k].zeta_prev;
-        s_a1[k] = p1 ? sh[k].active_prev : 0;
-        s_ie2[k] = sh[k].inv_eta;
-        s_z2[k] = sh[k].zeta;
-        s_a2[k] = p2 ? sh[k].active : 0;
+// The window's update: the NS saved steps (slots 0..NS-1, their w in
+// wslots[j * mP ..]) and then the current step where with_curr and it runs;
+// beta[j] = beta_{i_j - 1} of the window's step j (j = NS: the current step).
+// Dynamic LDS: (NS + 1) K entries of inv_eta, zeta and active.
+template <int NS>
+__global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *prow,
+                                                   const int32_t *pcol, const double2 *pval,
+                                                   const double2 *v, const double *vscale,
+                                                   const double2 *wslots, const MsSlot *slots,
+                                                   const double *beta, const MsShift *sh, int K,
+                                                   double2 *Q, double2 *out, const int *run,
+                                                   const MsCtl *ctl, int with_curr) {
+    extern __shared__ double2 s_lds[];
+    double2 *s_ie = s_lds;                        // [(NS + 1) K]
+    double2 *s_z = s_ie + (NS + 1) * K;           // [(NS + 1) K]
+    int *s_a = (int *)(s_z + (NS + 1) * K);       // [(NS + 1) K]
+    int *s_any = s_a + (NS + 1) * K;              // [K]: some step of the window moves shift k
+    __shared__ int s_step[NS + 1];                // some shift takes step j
+    __shared__ double s_b[NS + 1];
+    const bool cur = with_curr && run[0] && ctl->nactive > 0;
+    if (threadIdx.x <= NS) s_step[threadIdx.x] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < (NS + 1) * K; t += 256) {
+        const int j = t / K, k = t - j * K;
+        MsSlot q;
+        if (j < NS) q = slots[t];
+        else q = MsSlot{sh[k].inv_eta, sh[k].zeta, cur ? sh[k].active : 0};
+        s_ie[t] = q.inv_eta;
+        s_z[t] = q.zeta;
+        s_a[t] = q.active;
+        if (q.active) atomicOr(&s_step[j], 1);
     }
     __syncthreads();
-    const double bpp = p1 ? *beta_pp : 0.0;
-    const double bp = p2 ? *beta_p : 0.0;
-    const double vs = p2 ? *vscale : 0.0;
+    for (int k = threadIdx.x; k < K; k += 256) {
+        int a = 0;
+        for (int j = 0; j <= NS; ++j) a |= s_a[j * K + k];
+        s_any[k] = a;
+    }
+    if (threadIdx.x <= NS) s_b[threadIdx.x] = s_step[threadIdx.x] ? beta[threadIdx.x] : 0.0;
+    __syncthreads();
+    int anyw = 0;
+#pragma unroll
+    for (int j = 0; j <= NS; ++j) anyw |= s_step[j];
+    if (!anyw) return;
+    const double vs = s_step[NS] ? *vscale : 0.0;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const double2 w1 = p1 ? w1in[r] : make_double2(0, 0);
-        double2 w2 = make_double2(0, 0);
-        if (p2) {
-            w2 = ms_proj_row(r, prow, pcol, pval, v);
-            w2 = make_double2(w2.x * vs, w2.y * vs);
+        double2 w[NS + 1];
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+            w[j] = s_step[j] ? wslots[(int64_t)j * mP + r] : make_double2(0, 0);
+        w[NS] = make_double2(0, 0);
+        if (s_step[NS]) {
+            const double2 p = ms_proj_row(r, prow, pcol, pval, v);
+            w[NS] = make_double2(p.x * vs, p.y * vs);
         }
         // four shifts at a time: their Q and out loads in flight together
         for (int k0 = 0; k0 < K; k0 += 4) {
@@ -232,23 +218,21 @@ __global__ __launch_bounds__(256) void k_ms_update2(int64_t mP, const int64_t *p
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k = k0 + u;
-                act[u] = k < K && (s_a1[k] || s_a2[k]);
+                act[u] = k < K && s_any[k];
                 if (act[u]) {
                     q[u] = Q[(int64_t)k * mP + r];
                     acc[u] = out[(int64_t)k * mP + r];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int k = k0 + u;
-                if (!act[u]) continue;
-                if (s_a1[k]) {
-                    q[u] = cmul(make_double2(w1.x + bpp * q[u].x, w1.y + bpp * q[u].y), s_ie1[k]);
-                    acc[u] = cadd(acc[u], cmul(s_z1[k], q[u]));
-                }
-                if (s_a2[k]) {
-                    q[u] = cmul(make_double2(w2.x + bp * q[u].x, w2.y + bp * q[u].y), s_ie2[k]);
-                    acc[u] = cadd(acc[u], cmul(s_z2[k], q[u]));
+            for (int j = 0; j <= NS; ++j) {
+                const double bj = s_b[j];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = j * K + k0 + u;
+                    if (!act[u] || !s_a[t]) continue;
+                    q[u] = cmul(make_double2(w[j].x + bj * q[u].x, w[j].y + bj * q[u].y), s_ie[t]);
+                    acc[u] = cadd(acc[u], cmul(s_z[t], q[u]));
                 }
             }
 #pragma unroll
@@ -308,6 +292,43 @@ __global__ void k_ms_finish(const KrylovDev *S, const double *beta, MsShift *sh,
     }
 }
 
+// Steps per windowed update: 8 (EDX_MS_WINDOW=1..8 overrides; 1 updates every
+// step), fewer where the window's shift scalars would not fit 24 KB of LDS.
+int ms_window(int K) {
+    int w = 8;
+    if (const char *e = getenv("EDX_MS_WINDOW")) w = std::min(8, std::max(1, atoi(e)));
+    while (w > 1 && (size_t)w * K * (2 * sizeof(double2) + sizeof(int)) + K * sizeof(int) > 24576)
+        --w;
+    return w;
+}
+
+int ms_update_launch(int ns, int64_t mp, const int64_t *pr, const int32_t *pc, const double2 *pv,
+                     const double2 *v, const double *vscale, const double2 *wslots,
+                     const MsSlot *slots, const double *beta, const MsShift *sh, int K, double2 *Q,
+                     double2 *out, const int *run, const MsCtl *ctl, int with_curr) {
+    const size_t lds = (size_t)(ns + 1) * K * (2 * sizeof(double2) + sizeof(int)) + K * sizeof(int);
+    const dim3 g(vec_blocks(mp)), b(256);
+    hipStream_t s = edx::stream();
+#define EDX_MSU(NS)                                                                               \
+    case NS:                                                                                      \
+        hipLaunchKernelGGL(k_ms_update<NS>, g, b, lds, s, mp, pr, pc, pv, v, vscale, wslots,      \
+                           slots, beta, sh, K, Q, out, run, ctl, with_curr);                      \
+        break;
+    switch (ns) {
+        EDX_MSU(0)
+        EDX_MSU(1)
+        EDX_MSU(2)
+        EDX_MSU(3)
+        EDX_MSU(4)
+        EDX_MSU(5)
+        EDX_MSU(6)
+        EDX_MSU(7)
+        default: return edx::set_error(EDX_ERR_ARG, "window of %d saved steps", ns);
+    }
+#undef EDX_MSU
+    return EDX_OK;
+}
+
 int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, const double *sigma,
                     int nproj, const edx_csr *const *proj, double2 *const *out, double tol,
                     int maxit, int *iters, double *final_res) {
@@ -317,7 +338,7 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         const int64_t mp = proj[p] ? proj[p]->m : n;
         if (mp && K) EDX_HIP(hipMemsetAsync(out[p], 0, (size_t)mp * K * sizeof(double2), s));
     }
-    DevBuf vb, x32, part, st, ab, runf, shb, ctlb, qb, wb;
+    DevBuf vb, x32, part, st, ab, runf, shb, ctlb, qb, wb, slb;
     EDX_TRY(vb.alloc(2 * (size_t)n * sizeof(double2), "fom lanczos vectors"));
     if (precision == EDX_C64) EDX_TRY(x32.alloc((size_t)n * sizeof(float2), "fom x32"));
     const int nbv = vec_blocks(n);
@@ -335,17 +356,19 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         qtot += (size_t)(proj[p] ? proj[p]->m : n) * K;
     }
     EDX_TRY(qb.alloc((qtot ? qtot : 1) * sizeof(double2), "projected directions"));
-    // paired updates (k_ms_project / k_ms_update2) unless EDX_MS_UNPAIRED=1;
-    // wb keeps the first step's w = P v per projection
-    const char *unp = getenv("EDX_MS_UNPAIRED");
-    const bool paired = !(unp && unp[0] == '1');
+    // windowed updates (k_ms_project / k_ms_update): win steps per pass over Q
+    // and out; wb keeps the first win-1 steps' w = s P v per projection
+    const int win = ms_window(K);
     size_t wtot = 0;
     std::vector<size_t> woff(nproj);
     for (int p = 0; p < nproj; ++p) {
         woff[p] = wtot;
-        wtot += (size_t)(proj[p] ? proj[p]->m : n);
+        wtot += (size_t)(proj[p] ? proj[p]->m : n) * (win - 1);
     }
-    if (paired) EDX_TRY(wb.alloc((wtot ? wtot : 1) * sizeof(double2), "paired projections"));
+    if (win > 1) {
+        EDX_TRY(wb.alloc((wtot ? wtot : 1) * sizeof(double2), "windowed projections"));
+        EDX_TRY(slb.alloc((size_t)(win - 1) * K * sizeof(MsSlot), "windowed shift scalars"));
+    }
     double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + n};
     double *d_alpha = ab.as<double>(), *d_beta = ab.as<double>() + maxit + 1;
     int *run = runf.as<int>();
@@ -398,25 +421,23 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
                            d_alpha, i, run);
         hipLaunchKernelGGL(k_ms_scalars, dim3(1), dim3(256), 0, s, i, d_alpha, d_beta, S, sh, K,
                            ctl, run);
+        const int j = (i - 1) % win;   // the step's place in its window
         for (int p = 0; p < nproj; ++p) {
             const edx_csr *P = proj[p];
             const int64_t mp = P ? P->m : n;
             const int64_t *pr = P ? P->d_rowptr : nullptr;
             const int32_t *pc = P ? P->d_cols : nullptr;
             const double2 *pv = P ? (const double2 *)P->d_vals : nullptr;
-            if (!paired)
-                hipLaunchKernelGGL(k_ms_update, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
-                                   pv, v[curr], &S->coef[0].x, d_beta + (i - 1), sh, K,
-                                   qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl);
-            else if (i % 2 == 1)
+            if (j < win - 1)
                 hipLaunchKernelGGL(k_ms_project, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
-                                   pv, v[curr], &S->coef[0].x, wb.as<double2>() + woff[p], sh, K,
-                                   run + (i - 1), ctl);
+                                   pv, v[curr], &S->coef[0].x,
+                                   wb.as<double2>() + woff[p] + (size_t)j * mp,
+                                   slb.as<MsSlot>() + (size_t)j * K, sh, K, run + (i - 1), ctl);
             else
-                hipLaunchKernelGGL(k_ms_update2, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
-                                   pv, v[curr], &S->coef[0].x, wb.as<double2>() + woff[p],
-                                   d_beta + (i - 2), d_beta + (i - 1), sh, K,
-                                   qb.as<double2>() + qoff[p], out[p], run + (i - 1), ctl, 1);
+                EDX_TRY(ms_update_launch(win - 1, mp, pr, pc, pv, v[curr], &S->coef[0].x,
+                                         wb.as<double2>() + woff[p], slb.as<MsSlot>(),
+                                         d_beta + (i - win), sh, K, qb.as<double2>() + qoff[p],
+                                         out[p], run + (i - 1), ctl, 1));
         }
         last = i;
         hipLaunchKernelGGL(k_axpy_norm, dim3(nbv), dim3(VB), 0, s, n, v[prev], v[curr],
@@ -432,15 +453,16 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
             if (!r) break;
         }
     }
-    if (paired && last % 2 == 1) {   // the last pair has its first step only
+    const int nrem = last % win;   // the last window's saved steps, not yet applied
+    if (nrem) {
         for (int p = 0; p < nproj; ++p) {
             const edx_csr *P = proj[p];
             const int64_t mp = P ? P->m : n;
-            hipLaunchKernelGGL(k_ms_update2, dim3(vec_blocks(mp)), dim3(256), 0, s, mp,
-                               P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
-                               P ? (const double2 *)P->d_vals : nullptr, v[curr], &S->coef[0].x,
-                               wb.as<double2>() + woff[p], d_beta + (last - 1), d_beta + (last - 1),
-                               sh, K, qb.as<double2>() + qoff[p], out[p], run + (last - 1), ctl, 0);
+            EDX_TRY(ms_update_launch(nrem, mp, P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
+                                     P ? (const double2 *)P->d_vals : nullptr, v[curr],
+                                     &S->coef[0].x, wb.as<double2>() + woff[p], slb.as<MsSlot>(),
+                                     d_beta + (last - nrem), sh, K, qb.as<double2>() + qoff[p],
+                                     out[p], run + (last - 1), ctl, 0));
         }
     }
     hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(256), 0, s, S, d_beta, sh, K);

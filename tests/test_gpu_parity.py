@@ -626,34 +626,35 @@ def test_shifted_fom_edge_cases(core):
         np.testing.assert_allclose(X.cpu().numpy()[k], ev / (sig[k] - w), atol=1e-12)
 
 
-def test_shifted_fom_paired_updates_equal_per_step_updates(core, monkeypatch):
-    # the projected updates made in pairs of steps (one pass over the shifts'
-    # vectors per two steps) against one pass per step, for a run whose shifts
-    # stop at different steps and for runs capped at an odd and an even step
-    # count (the last pair then has one step or two).  The same scalars, so the
-    # same iteration counts and residuals; the solutions to 1e-13 of their
-    # scale (the paired form stores the first step's scaled projection, a
-    # rounding the per-step form may fuse into its next multiply-add)
+def test_shifted_fom_windowed_updates_equal_per_step_updates(core, monkeypatch):
+    # the projected updates made in windows of 2, 3 and 8 steps (one pass over
+    # the shifts' vectors per window) against one pass per step, for a run
+    # whose shifts stop at different steps and for runs capped at 7, 8 and 13
+    # steps (the last window then partly filled or full).  The same scalars, so
+    # the same iteration counts and residuals; the solutions to 1e-13 of their
+    # scale (a window stores its earlier steps' scaled projections, a rounding
+    # the per-step form may fuse into its next multiply-add)
     m, fi, fn, Rn, Dout, b, sig = _rixs_case()
     bi, bn = core.Basis.from_words(fi, m.norb), core.Basis.from_words(fn, m.norb)
     Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
     Df = core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1])
     bd = _to_dev(core, b)
-    for tol, maxit in ((1e-10, 4000), (0.0, 7), (0.0, 8)):
+    for tol, maxit in ((1e-10, 4000), (0.0, 7), (0.0, 8), (0.0, 13)):
         out = []
-        for unpaired in ("1", "0"):
-            monkeypatch.setenv("EDX_MS_UNPAIRED", unpaired)
+        for window in ("1", "2", "3", "8"):
+            monkeypatch.setenv("EDX_MS_WINDOW", window)
             (X, Y), it, res = core.shifted_fom(Hn, bd, sig, projections=(None, Df), tol=tol,
                                                maxit=maxit, precision="c128")
             out.append((X.cpu().numpy(), Y.cpu().numpy(), it, res))
-        (X0, Y0, it0, r0), (X1, Y1, it1, r1) = out
-        for k in range(len(sig)):
-            assert np.abs(X1[k] - X0[k]).max() <= 1e-13 * np.abs(X0[k]).max(), (tol, maxit, k)
-            assert np.abs(Y1[k] - Y0[k]).max() <= 1e-13 * np.abs(Y0[k]).max(), (tol, maxit, k)
-        assert np.array_equal(it0, it1) and np.array_equal(r0, r1)
+        X0, Y0, it0, r0 = out[0]
+        for X1, Y1, it1, r1 in out[1:]:
+            for k in range(len(sig)):
+                assert np.abs(X1[k] - X0[k]).max() <= 1e-13 * np.abs(X0[k]).max(), (tol, maxit, k)
+                assert np.abs(Y1[k] - Y0[k]).max() <= 1e-13 * np.abs(Y0[k]).max(), (tol, maxit, k)
+            assert np.array_equal(it0, it1) and np.array_equal(r0, r1)
         if tol > 0:
             assert len(set(int(v) for v in it0)) > 1      # shifts stopped at different steps
-    monkeypatch.delenv("EDX_MS_UNPAIRED")
+    monkeypatch.delenv("EDX_MS_WINDOW")
 
 
 # ------------------------------------------------- resumable Krylov, converged CF
