@@ -86,6 +86,7 @@ _SIGS = {
                             ctypes.POINTER(_I)]),
     "edx_group_free": (None, [_P]),
     "edx_shifted_fom_project": (_I, [_P, _I, _P, _I, _P, _I, _PP, _PP, _D, _I, _P, _P]),
+    "edx_shifted_fom_project_batch": (_I, [_P, _I, _I, _P, _I, _P, _PP, _PP, _D, _I, _P, _P]),
     "edx_csr_free": (None, [_P]),
     "edx_csr_spmv_lanes": (_I, [_P, _I, _I, ctypes.POINTER(_I)]),
     "edx_csr_factored_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I64),

@@ -721,6 +721,29 @@ def shifted_fom(H, b, sigma, projections=(None,), tol=1e-8, maxit=2000, precisio
     return outs, it, res
 
 
+def shifted_fom_batch(H, B, sigma, projections, tol=1e-8, maxit=2000, precision="c128"):
+    """shifted_fom for the rows b_r of B (R, n) in one interleaved recurrence, the
+    same shifts for every row: ``projections[r]`` (``None`` = identity) projects
+    row r.  Returns ``(outs, iters, res)``: outs[r] is (K, rows(P_r)), iters and
+    res are (R, K)."""
+    T = torch()
+    B = B.contiguous()
+    R = B.shape[0]
+    assert B.dim() == 2 and B.shape[1] == H.n and B.dtype == T.complex128
+    assert len(projections) == R
+    sig = np.ascontiguousarray(np.atleast_1d(np.asarray(sigma, dtype=np.complex128)))
+    K = len(sig)
+    outs = [T.zeros((K, P.m if P is not None else H.n), dtype=T.complex128, device=B.device)
+            for P in projections]
+    hp = (ctypes.c_void_p * R)(*[P._h if P is not None else None for P in projections])
+    op = (ctypes.c_void_p * R)(*[o.data_ptr() for o in outs])
+    it = np.zeros((R, K), dtype=np.int32)
+    res = np.zeros((R, K))
+    _lib.call("edx_shifted_fom_project_batch", H._h, _prec(precision), R, _dptr(B), K,
+              _ptr(sig.view(np.float64)), hp, op, float(tol), int(maxit), _ptr(it), _ptr(res))
+    return outs, it, res
+
+
 def minres(A, b, tol=1e-10, maxit=500, precision="c128"):
     T = torch()
     _check_vec(b, A.n)

@@ -53,25 +53,32 @@ __device__ __forceinline__ double2 crecip(double2 a) {
 __device__ __forceinline__ double cabs2(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 
 // Step i (1-based) of the shift recurrences, after alpha_i is known.  A shift
-// whose iterate i-1 already meets the tolerance is frozen there.
+// whose iterate i-1 already meets the tolerance is frozen there.  Block r
+// serves right-hand side r of R: alpha_i, beta_{i-1} and the run flag of step
+// i-1 at [(i - 1) R + r], its K shifts at sh[r K ..], its control at ctl[r].
 __global__ __launch_bounds__(256) void k_ms_scalars(int i, const double *alpha, const double *beta,
-                                                    const KrylovDev *S, MsShift *sh, int K,
-                                                    MsCtl *ctl, const int *run) {
-    if (!run[i - 1]) return;
+                                                    int R, const double *sqrt_norm, MsShift *sh,
+                                                    int K, MsCtl *ctl, const int *run) {
+    const int rr = blockIdx.x;
+    const int64_t o = (int64_t)(i - 1) * R + rr;
+    if (!run[o]) return;
+    sh += (int64_t)rr * K;
+    ctl += rr;
     __shared__ int cnt[256];
     int c = 0;
     const double tol = ctl->tol_abs;
+    const double nb = sqrt_norm[rr];
     for (int k = threadIdx.x; k < K; k += 256) {
         MsShift s = sh[k];
-        const double a = alpha[i - 1];
+        const double a = alpha[o];
         if (i == 1) {
             s.eta = make_double2(s.sigma.x - a, s.sigma.y);
-            s.zeta = make_double2(S->sqrt_norm, 0.0);
+            s.zeta = make_double2(nb, 0.0);
             s.active = 1;
             s.iters = 0;
-            s.res = S->sqrt_norm;
+            s.res = nb;
         } else if (s.active) {
-            const double bp = beta[i - 1];
+            const double bp = beta[o];
             const double2 ratio = cmul(s.zeta, s.inv_eta);   // zeta_{i-1} / eta_{i-1}
             s.res = bp * cabs2(ratio);
             if (s.res <= tol) {
@@ -100,10 +107,12 @@ __global__ __launch_bounds__(256) void k_ms_scalars(int i, const double *alpha, 
 
 // (P v)[r] for a CSR P (nullptr rowptr: the identity), four entries at a time
 // (their columns and values, then four gathers in flight), summed in entry
-// order.
+// order; element c of v is v[c * vs + vo] (column vo of an interleaved block
+// of vs vectors).
 __device__ __forceinline__ double2 ms_proj_row(int64_t r, const int64_t *prow, const int32_t *pcol,
-                                               const double2 *pval, const double2 *v) {
-    if (!prow) return v[r];
+                                               const double2 *pval, const double2 *v, int vs,
+                                               int vo) {
+    if (!prow) return v[r * vs + vo];
     double2 w = make_double2(0, 0);
     int64_t j = prow[r];
     const int64_t je = prow[r + 1];
@@ -116,11 +125,11 @@ __device__ __forceinline__ double2 ms_proj_row(int64_t r, const int64_t *prow, c
             a[u] = pval[j + u];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = v[c[u]];
+        for (int u = 0; u < 4; ++u) x[u] = v[(int64_t)c[u] * vs + vo];
 #pragma unroll
         for (int u = 0; u < 4; ++u) w = cadd(w, cmul(a[u], x[u]));
     }
-    for (; j < je; ++j) w = cadd(w, cmul(pval[j], v[pcol[j]]));
+    for (; j < je; ++j) w = cadd(w, cmul(pval[j], v[(int64_t)pcol[j] * vs + vo]));
     return w;
 }
 
@@ -138,8 +147,9 @@ struct MsSlot {
 
 __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *prow,
                                                     const int32_t *pcol, const double2 *pval,
-                                                    const double2 *v, const double *vscale,
-                                                    double2 *wout, MsSlot *slot,
+                                                    const double2 *v, int vs, int vo,
+                                                    const double *vscale, double2 *wout,
+                                                    MsSlot *slot,
                                                     const MsShift *sh, int K, const int *run,
                                                     const MsCtl *ctl) {
     const int on = run[0] && ctl->nactive > 0;
@@ -147,26 +157,75 @@ __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *p
         for (int k = threadIdx.x; k < K; k += 256)
             slot[k] = MsSlot{sh[k].inv_eta, sh[k].zeta, on ? sh[k].active : 0};
     if (!on) return;
-    const double vs = *vscale;
+    const double sc = *vscale;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const double2 w = ms_proj_row(r, prow, pcol, pval, v);
-        wout[r] = make_double2(w.x * vs, w.y * vs);
+        const double2 w = ms_proj_row(r, prow, pcol, pval, v, vs, vo);
+        wout[r] = make_double2(w.x * sc, w.y * sc);
     }
+}
+
+// The projections of one step for up to MS_PGROUP (vector column, projection)
+// entries in one launch: a row's gathers for the entries (the right-hand sides
+// of an interleaved block share cache lines) come from one thread in turn
+// (measured faster than workgroups of different entries side by side).
+constexpr int MS_PGROUP = 8;
+struct MsProjSet {
+    const int64_t *prow[MS_PGROUP];
+    const int32_t *pcol[MS_PGROUP];
+    const double2 *pval[MS_PGROUP];
+    int64_t mp[MS_PGROUP];
+    double2 *wout[MS_PGROUP];       // this step's slot of w = s P v
+    MsSlot *slot[MS_PGROUP];        // this step's slot of shift scalars [K]
+    const MsShift *sh[MS_PGROUP];
+    const MsCtl *ctl[MS_PGROUP];
+    const double *vscale[MS_PGROUP];
+    const int *run[MS_PGROUP];      // the entry's run flag of this step
+    int vo[MS_PGROUP];              // the entry's column of v
+    int n;
+    int64_t mmax;
+};
+
+__global__ __launch_bounds__(256) void k_ms_project_set(MsProjSet ps, const double2 *v, int vs,
+                                                        int K) {
+    __shared__ int s_on[MS_PGROUP];
+    __shared__ double s_sc[MS_PGROUP];
+    if (threadIdx.x < ps.n) {
+        const int e = threadIdx.x;
+        const int on = ps.run[e][0] && ps.ctl[e]->nactive > 0;
+        s_on[e] = on;
+        s_sc[e] = on ? *ps.vscale[e] : 0.0;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int e = 0; e < ps.n; ++e)
+            for (int k = threadIdx.x; k < K; k += 256)
+                ps.slot[e][k] = MsSlot{ps.sh[e][k].inv_eta, ps.sh[e][k].zeta,
+                                       s_on[e] ? ps.sh[e][k].active : 0};
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < ps.mmax;
+         r += (int64_t)gridDim.x * blockDim.x)
+        for (int e = 0; e < ps.n; ++e) {
+            if (!s_on[e] || r >= ps.mp[e]) continue;
+            const double2 w = ms_proj_row(r, ps.prow[e], ps.pcol[e], ps.pval[e], v, vs, ps.vo[e]);
+            ps.wout[e][r] = make_double2(w.x * s_sc[e], w.y * s_sc[e]);
+        }
 }
 
 // The window's update: the NS saved steps (slots 0..NS-1, their w in
 // wslots[j * mP ..]) and then the current step where with_curr and it runs;
-// beta[j] = beta_{i_j - 1} of the window's step j (j = NS: the current step).
+// beta[j bstride] = beta_{i_j - 1} of the window's step j (j = NS: the current
+// step); v, vs, vo as in ms_proj_row.
 // Dynamic LDS: (NS + 1) K entries of inv_eta, zeta and active.
 template <int NS>
 __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *prow,
                                                    const int32_t *pcol, const double2 *pval,
-                                                   const double2 *v, const double *vscale,
+                                                   const double2 *v, int vs, int vo,
+                                                   const double *vscale,
                                                    const double2 *wslots, const MsSlot *slots,
-                                                   const double *beta, const MsShift *sh, int K,
-                                                   double2 *Q, double2 *out, const int *run,
-                                                   const MsCtl *ctl, int with_curr) {
+                                                   const double *beta, int bstride,
+                                                   const MsShift *sh, int K, double2 *Q,
+                                                   double2 *out, const int *run, const MsCtl *ctl,
+                                                   int with_curr) {
     extern __shared__ double2 s_lds[];
     double2 *s_ie = s_lds;                        // [(NS + 1) K]
     double2 *s_z = s_ie + (NS + 1) * K;           // [(NS + 1) K]
@@ -193,13 +252,14 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
         for (int j = 0; j <= NS; ++j) a |= s_a[j * K + k];
         s_any[k] = a;
     }
-    if (threadIdx.x <= NS) s_b[threadIdx.x] = s_step[threadIdx.x] ? beta[threadIdx.x] : 0.0;
+    if (threadIdx.x <= NS)
+        s_b[threadIdx.x] = s_step[threadIdx.x] ? beta[(int64_t)threadIdx.x * bstride] : 0.0;
     __syncthreads();
     int anyw = 0;
 #pragma unroll
     for (int j = 0; j <= NS; ++j) anyw |= s_step[j];
     if (!anyw) return;
-    const double vs = s_step[NS] ? *vscale : 0.0;
+    const double sc = s_step[NS] ? *vscale : 0.0;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
         double2 w[NS + 1];
@@ -208,8 +268,8 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
             w[j] = s_step[j] ? wslots[(int64_t)j * mP + r] : make_double2(0, 0);
         w[NS] = make_double2(0, 0);
         if (s_step[NS]) {
-            const double2 p = ms_proj_row(r, prow, pcol, pval, v);
-            w[NS] = make_double2(p.x * vs, p.y * vs);
+            const double2 p = ms_proj_row(r, prow, pcol, pval, v, vs, vo);
+            w[NS] = make_double2(p.x * sc, p.y * sc);
         }
         // four shifts at a time: their Q and out loads in flight together
         for (int k0 = 0; k0 < K; k0 += 4) {
@@ -279,13 +339,16 @@ __global__ __launch_bounds__(VB) void k_ms_beta(const double2 *partials, int nb,
     }
 }
 
-// shifts still active when the recurrence ends keep iterate neff
-__global__ void k_ms_finish(const KrylovDev *S, const double *beta, MsShift *sh, int K) {
-    const int m = S->neff;
+// shifts still active when the recurrence ends keep iterate neff (block r:
+// right-hand side r of R, beta_m at beta[m R + r])
+__global__ void k_ms_finish(const int *neff, const double *beta, int R, MsShift *sh, int K) {
+    const int rr = blockIdx.x;
+    const int m = neff[rr];
+    sh += (int64_t)rr * K;
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
         MsShift s = sh[k];
         if (s.active && m > 0) {
-            s.res = beta[m] * cabs2(cmul(s.zeta, s.inv_eta));
+            s.res = beta[(int64_t)m * R + rr] * cabs2(cmul(s.zeta, s.inv_eta));
             s.iters = m;
         }
         sh[k] = s;
@@ -293,26 +356,29 @@ __global__ void k_ms_finish(const KrylovDev *S, const double *beta, MsShift *sh,
 }
 
 // Steps per windowed update: 8 (EDX_MS_WINDOW=1..8 overrides; 1 updates every
-// step), fewer where the window's shift scalars would not fit 24 KB of LDS.
+// step), fewer where the window's shift scalars (and one more step's) would
+// not fit 24 KB of LDS.
 int ms_window(int K) {
     int w = 8;
     if (const char *e = getenv("EDX_MS_WINDOW")) w = std::min(8, std::max(1, atoi(e)));
-    while (w > 1 && (size_t)w * K * (2 * sizeof(double2) + sizeof(int)) + K * sizeof(int) > 24576)
+    while (w > 1 &&
+           (size_t)(w + 1) * K * (2 * sizeof(double2) + sizeof(int)) + K * sizeof(int) > 24576)
         --w;
     return w;
 }
 
 int ms_update_launch(int ns, int64_t mp, const int64_t *pr, const int32_t *pc, const double2 *pv,
-                     const double2 *v, const double *vscale, const double2 *wslots,
+                     const double2 *v, int vs, int vo, const double *vscale, const double2 *wslots,
                      const MsSlot *slots, const double *beta, const MsShift *sh, int K, double2 *Q,
-                     double2 *out, const int *run, const MsCtl *ctl, int with_curr) {
+                     double2 *out, const int *run, const MsCtl *ctl, int with_curr,
+                     int bstride = 1) {
     const size_t lds = (size_t)(ns + 1) * K * (2 * sizeof(double2) + sizeof(int)) + K * sizeof(int);
     const dim3 g(vec_blocks(mp)), b(256);
     hipStream_t s = edx::stream();
 #define EDX_MSU(NS)                                                                               \
     case NS:                                                                                      \
-        hipLaunchKernelGGL(k_ms_update<NS>, g, b, lds, s, mp, pr, pc, pv, v, vscale, wslots,      \
-                           slots, beta, sh, K, Q, out, run, ctl, with_curr);                      \
+        hipLaunchKernelGGL(k_ms_update<NS>, g, b, lds, s, mp, pr, pc, pv, v, vs, vo, vscale,      \
+                           wslots, slots, beta, bstride, sh, K, Q, out, run, ctl, with_curr);     \
         break;
     switch (ns) {
         EDX_MSU(0)
@@ -323,6 +389,7 @@ int ms_update_launch(int ns, int64_t mp, const int64_t *pr, const int32_t *pc, c
         EDX_MSU(5)
         EDX_MSU(6)
         EDX_MSU(7)
+        EDX_MSU(8)
         default: return edx::set_error(EDX_ERR_ARG, "window of %d saved steps", ns);
     }
 #undef EDX_MSU
@@ -419,8 +486,8 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         EDX_TRY(apply(h, precision, v[curr], x32.as<float2>(), e, &nb, run + (i - 1)));
         hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(VB), 0, s, part.as<double2>(), nb, S,
                            d_alpha, i, run);
-        hipLaunchKernelGGL(k_ms_scalars, dim3(1), dim3(256), 0, s, i, d_alpha, d_beta, S, sh, K,
-                           ctl, run);
+        hipLaunchKernelGGL(k_ms_scalars, dim3(1), dim3(256), 0, s, i, d_alpha, d_beta, 1,
+                           &S->sqrt_norm, sh, K, ctl, run);
         const int j = (i - 1) % win;   // the step's place in its window
         for (int p = 0; p < nproj; ++p) {
             const edx_csr *P = proj[p];
@@ -430,11 +497,11 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
             const double2 *pv = P ? (const double2 *)P->d_vals : nullptr;
             if (j < win - 1)
                 hipLaunchKernelGGL(k_ms_project, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
-                                   pv, v[curr], &S->coef[0].x,
+                                   pv, v[curr], 1, 0, &S->coef[0].x,
                                    wb.as<double2>() + woff[p] + (size_t)j * mp,
                                    slb.as<MsSlot>() + (size_t)j * K, sh, K, run + (i - 1), ctl);
             else
-                EDX_TRY(ms_update_launch(win - 1, mp, pr, pc, pv, v[curr], &S->coef[0].x,
+                EDX_TRY(ms_update_launch(win - 1, mp, pr, pc, pv, v[curr], 1, 0, &S->coef[0].x,
                                          wb.as<double2>() + woff[p], slb.as<MsSlot>(),
                                          d_beta + (i - win), sh, K, qb.as<double2>() + qoff[p],
                                          out[p], run + (i - 1), ctl, 1));
@@ -459,19 +526,203 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
             const edx_csr *P = proj[p];
             const int64_t mp = P ? P->m : n;
             EDX_TRY(ms_update_launch(nrem, mp, P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
-                                     P ? (const double2 *)P->d_vals : nullptr, v[curr],
+                                     P ? (const double2 *)P->d_vals : nullptr, v[curr], 1, 0,
                                      &S->coef[0].x, wb.as<double2>() + woff[p], slb.as<MsSlot>(),
                                      d_beta + (last - nrem), sh, K, qb.as<double2>() + qoff[p],
                                      out[p], run + (last - 1), ctl, 0));
         }
     }
-    hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(256), 0, s, S, d_beta, sh, K);
+    hipLaunchKernelGGL(k_ms_finish, dim3(1), dim3(256), 0, s, &S->neff, d_beta, 1, sh, K);
     EDX_LAUNCH_CHECK();
     EDX_HIP(edx::copy_sync(hs.data(), sh, K * sizeof(MsShift), hipMemcpyDeviceToHost));
     for (int k = 0; k < K; ++k) {
         iters[k] = hs[k].iters;
         final_res[k] = hs[k].res / hk.sqrt_norm;
     }
+    return EDX_OK;
+}
+
+
+// Batched form: R right-hand sides in one interleaved Lanczos recurrence (the
+// product is the multi-RHS SpMM; scalars, run flags and shift states are per
+// RHS), each with the same K shifts and its own projection proj[r] (nullptr:
+// the identity) into out[r].  A right-hand side also stops once none of its
+// shifts is active.
+__global__ void k_msb_init(int R, int K, const double *norm, double *sqrt_norm, MsCtl *ctl,
+                           double tol) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const double nb = sqrt(norm[r]);
+    sqrt_norm[r] = nb;
+    ctl[r] = MsCtl{tol * nb, K};
+}
+
+__global__ void k_msb_stop(int i, int R, int *run, const MsCtl *ctl) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R && ctl[r].nactive == 0) run[(int64_t)i * R + r] = 0;
+}
+
+int shifted_fom_batch_run(const edx_csr *h, int precision, int R, const double2 *Bcm, int K,
+                          const double *sigma, const edx_csr *const *proj, double2 *const *out,
+                          double tol, int maxit, int *iters, double *final_res) {
+    const int64_t n = h->n;
+    hipStream_t s = edx::stream();
+    for (int r = 0; r < R; ++r) {
+        const int64_t mp = proj[r] ? proj[r]->m : n;
+        if (mp) EDX_HIP(hipMemsetAsync(out[r], 0, (size_t)mp * K * sizeof(double2), s));
+    }
+    const size_t nr = (size_t)n * R;
+    Dbuf vb, x32, part, sc, flags, shb, ctlb, qb, wb, slb, nrm;
+    EDX_TRY(vb.alloc(2 * nr * sizeof(double2), "batched fom lanczos vectors"));
+    if (precision == EDX_C64) EDX_TRY(x32.alloc(nr * sizeof(float2), "batched fom x32"));
+    const int nbs = bspmm_blocks(n), nbe = elem_grid_x(n);
+    EDX_TRY(part.alloc(std::max<size_t>((size_t)std::max(nbs, nbe) * R + 64,
+                                        (size_t)edx::spmv_partials_len(h)) * sizeof(double2),
+                       "partials"));
+    // alpha [maxit R], beta [(maxit + 1) R], the per-RHS tail; run [(maxit + 1) R], neff [R]
+    const size_t nsc = (size_t)maxit * R + (size_t)(maxit + 1) * R + (size_t)BK_TAIL * R;
+    EDX_TRY(sc.alloc(nsc * sizeof(double), "batched fom scalars"));
+    EDX_TRY(flags.alloc(((size_t)(maxit + 1) * R + R) * sizeof(int), "batched fom flags"));
+    EDX_TRY(shb.alloc((size_t)R * K * sizeof(MsShift), "shift states"));
+    EDX_TRY(ctlb.alloc((size_t)R * sizeof(MsCtl), "shift controls"));
+    EDX_TRY(nrm.alloc((size_t)R * sizeof(double), "rhs norms"));
+    std::vector<size_t> qoff(R), woff(R);
+    size_t qtot = 0, wtot = 0;
+    const int win = ms_window(K);
+    for (int r = 0; r < R; ++r) {
+        const size_t mp = (size_t)(proj[r] ? proj[r]->m : n);
+        qoff[r] = qtot;
+        qtot += mp * K;
+        woff[r] = wtot;
+        wtot += mp * win;
+    }
+    // every step's projections go to slots (one fused launch per step for up
+    // to MS_PGROUP right-hand sides), the window's last step applies them all
+    EDX_TRY(qb.alloc((qtot ? qtot : 1) * sizeof(double2), "projected directions"));
+    EDX_TRY(wb.alloc((wtot ? wtot : 1) * sizeof(double2), "windowed projections"));
+    EDX_TRY(slb.alloc((size_t)R * win * K * sizeof(MsSlot), "windowed shift scalars"));
+    BKrylov st;
+    st.alpha = sc.as<double>();
+    st.beta = st.alpha + (size_t)maxit * R;
+    st.norm = st.beta + (size_t)(maxit + 1) * R;
+    st.c1 = (double2 *)(st.norm + R);    // 16-byte aligned: 2 maxit R + R doubles precede it
+    st.sums = st.c1 + R;
+    st.c0 = st.sums + R;
+    st.acoef = (double *)(st.c0 + R);
+    st.scur = st.acoef + R;
+    st.run = flags.as<int>();
+    st.neff = st.run + (size_t)(maxit + 1) * R;
+    MsShift *sh = shb.as<MsShift>();
+    MsCtl *ctl = ctlb.as<MsCtl>();
+    double *sqn = nrm.as<double>();
+    std::vector<MsShift> hs((size_t)R * K);
+    for (int r = 0; r < R; ++r)
+        for (int k = 0; k < K; ++k) {
+            hs[(size_t)r * K + k] = MsShift{};
+            hs[(size_t)r * K + k].sigma = make_double2(sigma[2 * k], sigma[2 * k + 1]);
+        }
+    EDX_HIP(hipMemcpyAsync(sh, hs.data(), hs.size() * sizeof(MsShift), hipMemcpyHostToDevice, s));
+    EDX_HIP(hipMemsetAsync(sc.p, 0, nsc * sizeof(double), s));
+    EDX_HIP(hipMemsetAsync(flags.p, 0, ((size_t)(maxit + 1) * R + R) * sizeof(int), s));
+    EDX_HIP(hipMemsetAsync(qb.p, 0, (qtot ? qtot : 1) * sizeof(double2), s));
+    EDX_HIP(hipMemsetAsync(vb.p, 0, 2 * nr * sizeof(double2), s));
+    double2 *v[2] = {vb.as<double2>(), vb.as<double2>() + nr};
+    const int kb = (R + 63) / 64;
+    ELAUNCH(k_cm_to_il, n, R, n, R, Bcm, v[0]);
+    ELAUNCH(k_bdot, n, R, n, R, v[0], v[0], part.as<double2>());
+    hipLaunchKernelGGL(k_breduce, dim3(R), dim3(256), 0, s, part.as<double2>(), nbe, R, st.sums);
+    hipLaunchKernelGGL(k_bk_init, dim3(kb), dim3(64), 0, s, st, R, st.sums);
+    hipLaunchKernelGGL(k_msb_init, dim3(kb), dim3(64), 0, s, R, K, st.norm, sqn, ctl, tol);
+    ELAUNCH(k_bdiv_sqrt_norm, n, R, n, R, v[0], st.norm);
+    EDX_LAUNCH_CHECK();
+
+    int curr = 0, prev = 1, checked = 0, last = 0;
+    for (int i = 1; i <= maxit; ++i) {
+        if (precision == EDX_C64) ELAUNCH(k_il_to_c64, n, R, n, R, v[curr], x32.as<float2>());
+        BEpi e{};   // y = (1/s_i) H u_i - (beta_i/s_{i-1}) u_{i-1}; partial <u_i|y>
+        e.y = v[prev];
+        e.z = v[prev];
+        e.c0 = st.c0;
+        e.c1 = st.c1;
+        e.dvec = v[curr];
+        e.partials = part.as<double2>();
+        int nb = 0;
+        EDX_TRY(edx::bspmm(h, precision, R, v[curr], x32.as<float2>(), e, &nb));
+        hipLaunchKernelGGL(k_breduce, dim3(R), dim3(256), 0, s, part.as<double2>(), nb, R, st.sums);
+        hipLaunchKernelGGL(k_bk_alpha, dim3(kb), dim3(64), 0, s, st, R, i, st.sums);
+        hipLaunchKernelGGL(k_ms_scalars, dim3(R), dim3(256), 0, s, i, st.alpha, st.beta, R, sqn, sh,
+                           K, ctl, st.run);
+        const int j = (i - 1) % win;
+        for (int r0 = 0; r0 < R; r0 += MS_PGROUP) {
+            MsProjSet ps{};
+            ps.n = std::min(MS_PGROUP, R - r0);
+            for (int e = 0; e < ps.n; ++e) {
+                const int r = r0 + e;
+                const edx_csr *P = proj[r];
+                ps.prow[e] = P ? P->d_rowptr : nullptr;
+                ps.pcol[e] = P ? P->d_cols : nullptr;
+                ps.pval[e] = P ? (const double2 *)P->d_vals : nullptr;
+                ps.mp[e] = P ? P->m : n;
+                ps.mmax = std::max(ps.mmax, ps.mp[e]);
+                ps.wout[e] = wb.as<double2>() + woff[r] + (size_t)j * ps.mp[e];
+                ps.slot[e] = slb.as<MsSlot>() + ((size_t)r * win + j) * K;
+                ps.sh[e] = sh + (size_t)r * K;
+                ps.ctl[e] = ctl + r;
+                ps.vscale[e] = &st.c0[r].x;
+                ps.run[e] = st.run + (size_t)(i - 1) * R + r;
+                ps.vo[e] = r;
+            }
+            hipLaunchKernelGGL(k_ms_project_set, dim3(vec_blocks(ps.mmax)), dim3(256), 0, s, ps,
+                               v[curr], R, K);
+        }
+        if (j == win - 1)
+            for (int r = 0; r < R; ++r) {
+                const int64_t mp = proj[r] ? proj[r]->m : n;
+                EDX_TRY(ms_update_launch(win, mp, nullptr, nullptr, nullptr, nullptr, R, r,
+                                         nullptr, wb.as<double2>() + woff[r],
+                                         slb.as<MsSlot>() + (size_t)r * win * K,
+                                         st.beta + (size_t)(i - win) * R + r, sh + (size_t)r * K,
+                                         K, qb.as<double2>() + qoff[r], out[r],
+                                         st.run + (size_t)(i - 1) * R + r, ctl + r, 0, R));
+            }
+        last = i;
+        // u_{i+1} = y - (alpha_i/s_i) u_i, kept unnormalized (s_{i+1} = beta_{i+1})
+        ELAUNCH(k_baxpy_norm, n, R, n, R, v[prev], v[curr], st.acoef, part.as<double2>());
+        hipLaunchKernelGGL(k_breduce, dim3(R), dim3(256), 0, s, part.as<double2>(), nbe, R, st.sums);
+        hipLaunchKernelGGL(k_bk_beta, dim3(kb), dim3(64), 0, s, st, R, i, st.sums);
+        hipLaunchKernelGGL(k_msb_stop, dim3(kb), dim3(64), 0, s, i, R, st.run, ctl);
+        EDX_LAUNCH_CHECK();
+        std::swap(curr, prev);
+        if (i - checked >= 16 && i < maxit) {
+            std::vector<int> rf(R);
+            EDX_HIP(edx::copy_sync(rf.data(), st.run + (size_t)i * R, R * sizeof(int),
+                                   hipMemcpyDeviceToHost));
+            checked = i;
+            bool any = false;
+            for (int r = 0; r < R; ++r) any |= rf[r] != 0;
+            if (!any) break;
+        }
+    }
+    const int nrem = last % win;   // the last window's saved steps, not yet applied
+    for (int r = 0; r < R && nrem; ++r) {
+        const int64_t mp = proj[r] ? proj[r]->m : n;
+        EDX_TRY(ms_update_launch(nrem, mp, nullptr, nullptr, nullptr, nullptr, R, r, nullptr,
+                                 wb.as<double2>() + woff[r], slb.as<MsSlot>() + (size_t)r * win * K,
+                                 st.beta + (size_t)(last - nrem) * R + r, sh + (size_t)r * K, K,
+                                 qb.as<double2>() + qoff[r], out[r],
+                                 st.run + (size_t)(last - 1) * R + r, ctl + r, 0, R));
+    }
+    hipLaunchKernelGGL(k_ms_finish, dim3(R), dim3(256), 0, s, st.neff, st.beta, R, sh, K);
+    EDX_LAUNCH_CHECK();
+    EDX_HIP(edx::copy_sync(hs.data(), sh, hs.size() * sizeof(MsShift), hipMemcpyDeviceToHost));
+    std::vector<double> hn(R);
+    EDX_HIP(edx::copy_sync(hn.data(), sqn, R * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; ++r)
+        for (int k = 0; k < K; ++k) {
+            const MsShift &q = hs[(size_t)r * K + k];
+            iters[(size_t)r * K + k] = hn[r] > 0.0 ? q.iters : 0;
+            final_res[(size_t)r * K + k] = hn[r] > 0.0 ? q.res / hn[r] : 0.0;
+        }
     return EDX_OK;
 }
 
@@ -507,6 +758,36 @@ int edx_shifted_fom_project(const edx_csr *h, int precision, const void *d_b, in
     for (int p = 0; p < nproj; ++p) outs[p] = (double2 *)d_out[p];
     return shifted_fom_run(h, precision, (const double2 *)d_b, nshift, sigma, nproj, proj,
                            outs.data(), tol, maxit, iters, final_res);
+}
+
+int edx_shifted_fom_project_batch(const edx_csr *h, int precision, int nrhs, const void *d_B,
+                                  int nshift, const double *sigma, const edx_csr *const *proj,
+                                  void *const *d_out, double tol, int maxit, int *iters,
+                                  double *final_res) {
+    EDX_TRY(check_square(h, precision));
+    EDX_CHECK(h->n > 0, EDX_ERR_ARG, "empty matrix");
+    EDX_CHECK(nrhs >= 1 && nrhs <= 64, EDX_ERR_ARG, "nrhs %d not in [1, 64]", nrhs);
+    EDX_CHECK(nshift >= 1 && nshift <= MS_MAXK, EDX_ERR_ARG, "nshift %d not in [1, %d]", nshift,
+              MS_MAXK);
+    EDX_CHECK(proj && d_out && d_B && sigma && iters && final_res, EDX_ERR_ARG, "null argument");
+    EDX_CHECK(maxit >= 1, EDX_ERR_ARG, "maxit %d < 1", maxit);
+    EDX_CHECK(tol >= 0.0, EDX_ERR_ARG, "negative tolerance");
+    for (int k = 0; k < nshift; ++k)
+        EDX_CHECK(sigma[2 * k + 1] != 0.0 && std::isfinite(sigma[2 * k]) &&
+                      std::isfinite(sigma[2 * k + 1]),
+                  EDX_ERR_ARG, "shift %d = (%g, %g): needs a finite nonzero imaginary part", k,
+                  sigma[2 * k], sigma[2 * k + 1]);
+    for (int r = 0; r < nrhs; ++r) {
+        EDX_CHECK(d_out[r], EDX_ERR_ARG, "null output %d", r);
+        if (proj[r])
+            EDX_CHECK(proj[r]->n == h->n, EDX_ERR_ARG,
+                      "projection %d has %lld columns, matrix has %lld rows", r,
+                      (long long)proj[r]->n, (long long)h->n);
+    }
+    std::vector<double2 *> outs(nrhs);
+    for (int r = 0; r < nrhs; ++r) outs[r] = (double2 *)d_out[r];
+    return shifted_fom_batch_run(h, precision, nrhs, (const double2 *)d_B, nshift, sigma, proj,
+                                 outs.data(), tol, maxit, iters, final_res);
 }
 
 }  // extern "C"

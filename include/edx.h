@@ -389,6 +389,19 @@ int edx_shifted_fom_project(const edx_csr *h, int precision, const void *d_b, in
                             const double *sigma, int nproj, const edx_csr *const *proj,
                             void *const *d_out, double tol, int maxit, int *iters,
                             double *final_res);
+/* The same for nrhs right-hand sides at once (column r of d_B, n x nrhs
+ * column-major; nrhs <= 64), all with the same nshift shifts, in one
+ * interleaved recurrence whose product is the multi-RHS SpMM: proj[r] (NULL:
+ * the identity) projects right-hand side r into d_out[r] (rows(proj[r]) x
+ * nshift column-major); iters and final_res are nrhs x nshift, row r for
+ * right-hand side r.  Each right-hand side gets the solution of its own
+ * edx_shifted_fom_project call up to rounding (the reference's RIXS driver
+ * solves the incident polarizations one after another, src/rixs_driver.f90:
+ * 127-150). */
+int edx_shifted_fom_project_batch(const edx_csr *h, int precision, int nrhs, const void *d_B,
+                                  int nshift, const double *sigma, const edx_csr *const *proj,
+                                  void *const *d_out, double tol, int maxit, int *iters,
+                                  double *final_res);
 int edx_minres_shifted_batch(const edx_csr *h, int precision, int nrhs, const double *sigma,
                              const void *d_B, void *d_X, double tol, int maxit, int *iters,
                              double *final_res);

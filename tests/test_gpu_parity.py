@@ -626,6 +626,53 @@ def test_shifted_fom_edge_cases(core):
         np.testing.assert_allclose(X.cpu().numpy()[k], ev / (sig[k] - w), atol=1e-12)
 
 
+@pytest.mark.parametrize("precision", ["c128", "c64d", "c128i"])
+def test_shifted_fom_batch_matches_dense_and_single_solves(core, precision):
+    # three right-hand sides in one interleaved recurrence (the second a
+    # different incident polarization, the third zero), each projected by its
+    # own operator: the dense solutions, and the per-RHS recurrence to the
+    # rounding of the two summation orders
+    m, fi, fn, Rn, Dout, b, sig = _rixs_case()
+    bi, bn = core.Basis.from_words(fi, m.norb), core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    Df = core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1])
+    Ri = o.build_ham_i(fi, m.hop_i, m.coul_i).todense()
+    e, v = np.linalg.eigh(Ri)
+    b2 = o.build_transop_i(fn, fi, m.norb, m.ncore, m.trans_i[2]).todense() @ v[:, 0]
+    Bm = np.stack([b, b2, np.zeros_like(b)])
+    tol = 1e-10 if precision != "c64d" else 1e-8
+    outs, it, res = core.shifted_fom_batch(Hn, _to_dev(core, Bm), sig, (Df, None, Df), tol=tol,
+                                           maxit=4000, precision=precision)
+    Y0, X1, Y2 = (t.cpu().numpy() for t in outs)
+    assert it.shape == (3, len(sig)) and res.shape == (3, len(sig))
+    assert np.all(it[2] == 0) and np.all(res[2] == 0) and not Y2.any()
+    A = Rn.todense()
+    Dd = Dout.todense()
+    bound = 1e-7 if precision != "c64d" else 1e-5
+    for k in range(len(sig)):
+        S = sig[k] * np.eye(A.shape[0]) - A
+        x0, x1 = np.linalg.solve(S, b), np.linalg.solve(S, b2)
+        assert res[0, k] <= tol and res[1, k] <= tol and 0 < it[0, k] < 4000
+        assert np.linalg.norm(Y0[k] - Dd @ x0) < bound * np.linalg.norm(Dd @ x0)
+        assert np.linalg.norm(X1[k] - x1) < bound * np.linalg.norm(x1)
+    # 40 steps of every shift: the per-RHS recurrence to the rounding of the
+    # two summation orders (a converged run's step count is not comparable:
+    # without reorthogonalization the rounding moves it by a few per cent)
+    outs, it, res = core.shifted_fom_batch(Hn, _to_dev(core, Bm[:2]), sig, (Df, None), tol=0.0,
+                                           maxit=40, precision=precision)
+    for r, P in ((0, Df), (1, None)):
+        (Ys,), its, rss = core.shifted_fom(Hn, _to_dev(core, Bm[r]), sig, projections=(P,),
+                                           tol=0.0, maxit=40, precision=precision)
+        Z, Ys = outs[r].cpu().numpy(), Ys.cpu().numpy()
+        assert np.all(its == 40) and np.all(it[r] == 40)
+        np.testing.assert_allclose(res[r], rss, rtol=1e-8)
+        for k in range(len(sig)):
+            assert np.linalg.norm(Z[k] - Ys[k]) < 1e-11 * np.linalg.norm(Ys[k]), (r, k)
+    with pytest.raises(Exception, match="nrhs"):
+        core.shifted_fom_batch(Hn, _to_dev(core, np.zeros((65, Hn.n), complex)), sig,
+                               [None] * 65, precision=precision)
+
+
 def test_shifted_fom_windowed_updates_equal_per_step_updates(core, monkeypatch):
     # the projected updates made in windows of 2, 3 and 8 steps (one pass over
     # the shifts' vectors per window) against one pass per step, for a run
