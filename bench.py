@@ -333,6 +333,13 @@ def timed(work, precision, steps, warmup, reserve=lambda t: 0.0):
     dt = time.perf_counter() - t0
     prof = {k: c.profile_read(precision, k) for k in c.KERNELS}
     prof["all"] = c.profile_read(precision)
+    # the same per count of right-hand sides (one template instance each)
+    prof["by_rhs"] = {}
+    for k in c.KERNELS:
+        for r in range(1, 18):
+            got = c.profile_read(precision, k, r)
+            if got[0]:
+                prof["by_rhs"][(k, r)] = got
     c.profile(False)
     return dt, done, out, prof
 
@@ -767,6 +774,12 @@ def config2(args):
     return out
 
 
+def traffic_key(kernel, nrhs):
+    """Name of a kernel instance in the traffic summaries: the kernel, with
+    _k<K> for a block of K right-hand sides."""
+    return kernel if nrhs == 1 else "%s_k%d" % (kernel, nrhs)
+
+
 def measured_traffic(kernel, form, norb):
     """HBM bytes per launch of a kernel as two rocprofv3 counter runs measured
     it (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
@@ -784,7 +797,8 @@ def measured_traffic(kernel, form, norb):
 def probe(args):
     """A short fixed workload for hardware-counter runs: the SpMV/SpMM launches
     of the step at full size (single-vector SpMV over H_n with the Krylov
-    epilogue, the K=3 block over H_n, the K=16 final-state block over H_i),
+    epilogue, the K=2 and K=3 blocks over H_n, the K=16 final-state block over
+    H_i),
     each ``--launches`` times, plus a plain device copy of 1 GiB as the
     calibration kernel of known traffic."""
     import torch
@@ -812,11 +826,12 @@ def probe(args):
     kb.close()
     torch.cuda.synchronize()
     log("probe: %d k_spmv Krylov steps on H_n done" % args.launches)
-    kb = c.KrylovBatch(Hn, rnd(3, Hn.n), f, capacity=args.launches + 1)
-    kb.advance(args.launches)
-    kb.close()
-    torch.cuda.synchronize()
-    log("probe: %d K=3 steps on H_n done" % args.launches)
+    for K in (2, 3):   # the RIXS solves' and the XAS vectors' blocks
+        kb = c.KrylovBatch(Hn, rnd(K, Hn.n), f, capacity=args.launches + 1)
+        kb.advance(args.launches)
+        kb.close()
+        torch.cuda.synchronize()
+        log("probe: %d K=%d steps on H_n done" % (args.launches, K))
     kb = c.KrylovBatch(Hi, rnd(16, Hi.n), f, capacity=args.launches + 1)
     kb.advance(args.launches)
     kb.close()
@@ -888,9 +903,11 @@ def main():
     forms = [f for f in forms if f in results]
     nspec = 1 + 2 * args.n_in
     t_main = times[main_form]
-    # the dominant kernel: the SpMV/SpMM variant with the most kernel time
-    kname = max(work.core.KERNELS, key=lambda k: profs[main_form][k][1])
-    n, ms, by = profs[main_form][kname]
+    # the dominant kernel: the SpMV/SpMM variant and count of right-hand sides
+    # (one template instance) with the most kernel time
+    by_rhs = profs[main_form]["by_rhs"]
+    kname, krhs = max(by_rhs, key=lambda kr: by_rhs[kr][1])
+    n, ms, by = by_rhs[(kname, krhs)]
     gbs = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     m = work.model
     forms_out = {}
@@ -898,14 +915,23 @@ def main():
     for f in forms:
         r = results[f]
         kern = {}
-        for k, (pn, pms, pby) in profs[f].items():
+        for k, v in profs[f].items():
+            if k == "by_rhs":
+                continue
+            pn, pms, pby = v
             if pn:
                 kern[k] = {"launches": pn, "total_s": pms * 1e-3, "avg_us": pms / pn * 1e3,
                            "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
+        inst = {}
+        for (k, nr), (pn, pms, pby) in sorted(profs[f]["by_rhs"].items()):
+            inst[traffic_key(k, nr)] = {"launches": pn, "total_s": pms * 1e-3,
+                                        "avg_us": pms / pn * 1e3,
+                                        "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
         d = {"spectra_per_s": nspec * nsteps[f] / times[f],
              "ms_per_step": times[f] / nsteps[f] * 1e3, "steps": nsteps[f],
              "phases_s": r["info"]["phases_s"],
              "spmv_kernels": kern,
+             "spmv_instances": inst,
              "lanczos_iters": r["info"]["lanczos_iters"],
              "lowest_states_eV": r["info"]["lowest_states"],
              "rixs_iters_max": max(r["info"]["rixs_iters"]),
@@ -959,8 +985,10 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": gbs / HBM_PEAK_GBS,
-            "traffic": measured_traffic(kname, main_form, m.norb),
-            "kernel": "%s (%s form), every launch of the timed region" % (kname, main_form),
+            "traffic": measured_traffic(traffic_key(kname, krhs), main_form, m.norb),
+            "kernel": "%s, %d right-hand side%s (%s form), every such launch of the timed region"
+                      % (kname, krhs, "" if krhs == 1 else "s", main_form),
+            "nrhs": krhs,
             "launches": n, "avg_us": ms / max(n, 1) * 1e3, "bytes_per_launch": by / max(n, 1),
             "bytes_counted": ("the stored form's bytes: index words, row pointers, value table, "
                               "diagonal, x once, y written, the epilogue's addend and dot vector"),
@@ -968,22 +996,23 @@ def main():
         "forms": forms_out,
     }
     inf = results[main_form]["info"]
-    if kname == "k_spmv_fac" and inf.get("fac_n") and ms > 0:
+    if kname in ("k_spmv_fac", "k_spmm_fac") and inf.get("fac_n") and ms > 0:
         # the same launches priced as the complex128 CSR the reference's
-        # matvec_csr streams (src/spmv.f90:73-103): 16-byte value and 4-byte
-        # column per nonzero (diagonal included), 8-byte row pointers, the same
-        # vector terms
+        # matvec_csr streams (src/spmv.f90:73-103) once per vector: 16-byte value
+        # and 4-byte column per nonzero (diagonal included), 8-byte row
+        # pointers, the same vector terms
         mh, (nc, hv, rr, nb, nt) = inf["m_n"], inf["fac_n"]
         mv = mh // nc
         stored = ((hv + rr + nb) * 4.0 + (mv + 1) * 8.0 + ((mv + 1) * 8.0 if nb else 0.0) +
                   ((mh + 1) * 8.0 if rr else 0.0) + nt * 8.0 + mh * 16.0)
-        csr = inf["nnz_n"] * 20.0 + (mh + 1) * 8.0
+        csr = krhs * (inf["nnz_n"] * 20.0 + (mh + 1) * 8.0)
         cby = by + n * (csr - stored)
         line["roofline"]["csr_equivalent"] = {
             "achieved": cby / (ms * 1e-3) / 1e9, "unit": "GB/s", "bytes_per_launch": cby / max(n, 1),
-            "bytes_counted": ("the complex128 CSR of H_n (20 bytes per nonzero, row pointers) in "
-                              "place of the factored form's words, pointers and diagonal; the "
-                              "same vector terms")}
+            "bytes_counted": ("the complex128 CSR of H_n (20 bytes per nonzero, row pointers), "
+                              "read once per vector as the reference's single-vector product "
+                              "does, in place of the factored form's words, pointers and "
+                              "diagonal; the same vector terms")}
     if "c128" in results:
         line["speedup_vs_fp64"] = (times["c128"] / nsteps["c128"]) / (t_main / nsteps[main_form])
     if not args.no_cpu:

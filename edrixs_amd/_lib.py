@@ -97,6 +97,8 @@ _SIGS = {
     "edx_profile": (_I, [_I]),
     "edx_profile_read": (_I, [_I, _I, ctypes.POINTER(_I64), ctypes.POINTER(_D),
                               ctypes.POINTER(_D)]),
+    "edx_profile_read_rhs": (_I, [_I, _I, _I, ctypes.POINTER(_I64), ctypes.POINTER(_D),
+                                  ctypes.POINTER(_D)]),
     "edx_profile_read_flops": (_I, [_I, _I, ctypes.POINTER(_D)]),
     "edx_set_corehole_layout": (_I, [_I, ctypes.POINTER(_I)]),
     "edx_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,

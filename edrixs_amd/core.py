@@ -797,15 +797,17 @@ KERNELS = {"k_spmv": 0, "k_spmm_il": 1, "k_spmm_rl": 2, "k_spmm": 3, "k_mf": 4, 
 BLOCK_KERNELS = {"k_bproj": 5, "k_bupd": 6}   # matrix-core kernels (recorded as c128)
 
 
-def profile_read(precision="c128", kernel=None):
+def profile_read(precision="c128", kernel=None, nrhs=None):
     """(launches, kernel milliseconds, algorithmic bytes) recorded for one
-    precision and one SpMV/SpMM kernel (a name of KERNELS; None = all)."""
+    precision and one SpMV/SpMM kernel (a name of KERNELS; None = all), and
+    with ``nrhs`` the launches with that many right-hand sides only (above 16:
+    every launch of more than 16)."""
     n = ctypes.c_int64()
     ms = ctypes.c_double()
     by = ctypes.c_double()
     k = -1 if kernel is None else {**KERNELS, **BLOCK_KERNELS}[kernel]
-    _lib.call("edx_profile_read", _prec(precision), k, ctypes.byref(n), ctypes.byref(ms),
-              ctypes.byref(by))
+    _lib.call("edx_profile_read_rhs", _prec(precision), k, -1 if nrhs is None else int(nrhs),
+              ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by))
     return n.value, ms.value, by.value
 
 

@@ -776,17 +776,18 @@ __device__ __forceinline__ uint64_t value_fp(double2 v) {
     return f ? f : 1ull;
 }
 
-// claim (or find) the slot of every off-diagonal value by its 64-bit
-// fingerprint (compare-and-swap on an empty slot, linear probing) and store
-// the value there; flag bit 1 = table full
+// claim (or find) the slot of every off-diagonal value (every value when
+// skip_diag is 0) by its 64-bit fingerprint (compare-and-swap on an empty
+// slot, linear probing) and store the value there; flag bit 1 = table full
 __global__ void k_vt_insert(int64_t m, const int64_t *rowptr, const int32_t *cols,
-                            const double2 *vals, unsigned long long *fp, double2 *tab, int *flag) {
+                            const double2 *vals, unsigned long long *fp, double2 *tab, int *flag,
+                            int skip_diag) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * blockDim.x) {
         if (*(volatile int *)flag) return;   // already failed: stop early
         unsigned long long last = 0;         // the value this thread placed last
         for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            if (cols[j] == r) continue;
+            if (skip_diag && cols[j] == r) continue;
             const double2 v = vals[j];
             const unsigned long long f = value_fp(v);
             if (f == last) continue;
@@ -816,17 +817,18 @@ __global__ void k_vt_insert(int64_t m, const int64_t *rowptr, const int32_t *col
     }
 }
 
-// the slot number of every off-diagonal value, in the split form's order;
-// flag bit 2 = two different values share a fingerprint
+// the slot number of every off-diagonal value (every value when skip_diag is
+// 0), row r's from idx[orow[r]] on; flag bit 2 = two different values share a
+// fingerprint
 __global__ void k_vt_index(int64_t m, const int64_t *rowptr, const int32_t *cols,
                            const double2 *vals, const int64_t *orow,
                            const unsigned long long *fp, const double2 *tab, uint16_t *idx,
-                           int *flag) {
+                           int *flag, int skip_diag) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * blockDim.x) {
         int64_t o = orow[r];
         for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            if (cols[j] == r) continue;
+            if (skip_diag && cols[j] == r) continue;
             const double2 v = vals[j];
             const unsigned long long f = value_fp(v);
             unsigned h = (unsigned)f & (VTAB_SLOTS - 1);
@@ -1073,7 +1075,7 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
     hipEvent_t pe = prof_start(precision, sched.rl == 3 ? EDX_KERNEL_SPMM_FAC
                                           : sched.rl == 0 ? EDX_KERNEL_SPMM_IL
                                                           : EDX_KERNEL_SPMM_RL,
-                               bytes);
+                               bytes, 0.0, K);
     EDX_TRY(bspmm_launch(a, precision, K, X64, X32, e, sched));
     prof_stop(pe);
     return EDX_OK;
@@ -1120,13 +1122,17 @@ int csr_add_split(edx_csr *a) {
 
 // The indexed form: the split form's pattern and diagonal, and per
 // off-diagonal nonzero the table slot of its exact value.
-int csr_add_indexed(edx_csr *a) {
-    if (a->d_off_idx) return EDX_OK;
-    EDX_TRY(csr_add_split(a));
-    const int64_t m = a->m;
+// Numbers the distinct values of a CSR (its off-diagonal ones with skip_diag)
+// densely, 0 .. ntab-1 in hash-slot order: *tab_out gets the ntab values,
+// *idx_out one number per numbered entry (row r's from orow[r] on, onnz in
+// all).  EDX_ERR_CAPACITY when the values do not fit the VTAB_SLOTS-slot
+// table or two of them share a fingerprint; nothing is kept then.
+int value_numbering(int64_t m, const int64_t *rowptr, const int32_t *cols, const double2 *vals,
+                    const int64_t *orow, int64_t onnz, int skip_diag, double2 **tab_out,
+                    uint16_t **idx_out, int64_t *ntab_out) {
     auto *fp = (unsigned long long *)dmalloc(VTAB_SLOTS * sizeof(unsigned long long), "value fingerprints");
     auto *tab = (double2 *)dmalloc(VTAB_SLOTS * sizeof(double2), "value table");
-    auto *idx = (uint16_t *)dmalloc((a->off_nnz ? a->off_nnz : 1) * sizeof(uint16_t), "value numbers");
+    auto *idx = (uint16_t *)dmalloc((onnz ? onnz : 1) * sizeof(uint16_t), "value numbers");
     int *flag = (int *)dmalloc(sizeof(int), "index flag");
     int rc = EDX_OK;
     std::vector<unsigned long long> hfp;
@@ -1143,8 +1149,8 @@ int csr_add_indexed(edx_csr *a) {
     }
     {
         const int g = elem_blocks(m);
-        hipLaunchKernelGGL(k_vt_insert, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
-                           a->d_vals, fp, tab, flag);
+        hipLaunchKernelGGL(k_vt_insert, dim3(g), dim3(256), 0, stream(), m, rowptr, cols, vals,
+                           fp, tab, flag, skip_diag);
         if (hipGetLastError() != hipSuccess ||
             edx::copy_sync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
             rc = set_error(EDX_ERR_HIP, "indexed form: kernels failed");
@@ -1152,12 +1158,12 @@ int csr_add_indexed(edx_csr *a) {
         }
         if (hflag & 1) {
             rc = set_error(EDX_ERR_CAPACITY,
-                           "too many distinct off-diagonal values for the %d-slot table: no indexed form",
-                           VTAB_SLOTS);
+                           "too many distinct %svalues for the %d-slot table: no indexed form",
+                           skip_diag ? "off-diagonal " : "", VTAB_SLOTS);
             goto fail;
         }
-        hipLaunchKernelGGL(k_vt_index, dim3(g), dim3(256), 0, stream(), m, a->d_rowptr, a->d_cols,
-                           a->d_vals, a->d_off_rowptr, fp, tab, idx, flag);
+        hipLaunchKernelGGL(k_vt_index, dim3(g), dim3(256), 0, stream(), m, rowptr, cols, vals,
+                           orow, fp, tab, idx, flag, skip_diag);
         if (hipGetLastError() != hipSuccess ||
             edx::copy_sync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
             rc = set_error(EDX_ERR_HIP, "indexed form: kernels failed");
@@ -1174,12 +1180,11 @@ int csr_add_indexed(edx_csr *a) {
         rc = set_error(EDX_ERR_HIP, "indexed form: download failed");
         goto fail;
     }
-    {   // renumber the values 0 .. nvtab-1 in slot order: a dense table of nvtab entries
+    {   // renumber the values 0 .. ntab-1 in slot order: a dense table of ntab entries
         std::vector<uint16_t> remap(VTAB_SLOTS, 0);
         int64_t k = 0;
         for (int sl = 0; sl < VTAB_SLOTS; ++sl)
             if (hfp[sl]) remap[sl] = (uint16_t)k++;
-        a->nvtab = k;
         auto *d_remap = (uint16_t *)dmalloc(VTAB_SLOTS * sizeof(uint16_t), "value renumbering");
         auto *tabc = (double2 *)dmalloc((size_t)std::max<int64_t>(k, 1) * sizeof(double2), "value table");
         if (!d_remap || !tabc) {
@@ -1193,9 +1198,9 @@ int csr_add_indexed(edx_csr *a) {
         if (ok) {
             hipLaunchKernelGGL(k_vt_compact, dim3(VTAB_SLOTS / 256), dim3(256), 0, stream(), fp, tab, d_remap,
                                tabc);
-            if (a->off_nnz)
-                hipLaunchKernelGGL(k_vt_renumber, dim3(elem_blocks(a->off_nnz)), dim3(256), 0, stream(),
-                                   a->off_nnz, d_remap, idx);
+            if (onnz)
+                hipLaunchKernelGGL(k_vt_renumber, dim3(elem_blocks(onnz)), dim3(256), 0, stream(),
+                                   onnz, d_remap, idx);
             ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(stream()) == hipSuccess;
         }
         dfree(d_remap);
@@ -1205,9 +1210,30 @@ int csr_add_indexed(edx_csr *a) {
             rc = set_error(EDX_ERR_HIP, "indexed form: renumbering failed");
             goto fail;
         }
+        *ntab_out = k;
     }
     dfree(fp);
     dfree(flag);
+    *tab_out = tab;
+    *idx_out = idx;
+    return EDX_OK;
+fail:
+    dfree(fp);
+    dfree(tab);
+    dfree(idx);
+    dfree(flag);
+    return rc;
+}
+
+int csr_add_indexed(edx_csr *a) {
+    if (a->d_off_idx) return EDX_OK;
+    EDX_TRY(csr_add_split(a));
+    double2 *tab = nullptr;
+    uint16_t *idx = nullptr;
+    int64_t ntab = 0;
+    EDX_TRY(value_numbering(a->m, a->d_rowptr, a->d_cols, a->d_vals, a->d_off_rowptr, a->off_nnz,
+                            1, &tab, &idx, &ntab));
+    a->nvtab = ntab;
     a->d_vtab = tab;
     a->d_off_idx = idx;
     // few values and columns: one 32-bit word per nonzero for the single-vector product
@@ -1225,12 +1251,43 @@ int csr_add_indexed(edx_csr *a) {
         }
     }
     return EDX_OK;
-fail:
-    dfree(fp);
-    dfree(tab);
+}
+
+// Packed form of a whole (rectangular) CSR, for the projections of the
+// multi-shift solve: one 32-bit word per nonzero, column | value number << 24,
+// with its table of distinct values.  Optional: a matrix with 2^24 or more
+// columns, more than 256 distinct values, or without the memory keeps
+// none, and its users read the complex128 CSR.
+int csr_add_packed_all(edx_csr *a) {
+    if (a->d_apk || a->nnz == 0 || a->n >= (int64_t(1) << PK_COL_BITS)) return EDX_OK;
+    const char *off = getenv("EDX_NO_PACKED_PROJ");
+    if (off && off[0] == '1') return EDX_OK;
+    double2 *tab = nullptr;
+    uint16_t *idx = nullptr;
+    int64_t ntab = 0;
+    if (value_numbering(a->m, a->d_rowptr, a->d_cols, a->d_vals, a->d_rowptr, a->nnz, 0, &tab,
+                        &idx, &ntab) != EDX_OK)
+        return EDX_OK;
+    auto *pk = ntab <= (int64_t(1) << (32 - PK_COL_BITS))
+                   ? (uint32_t *)dmalloc((size_t)a->nnz * sizeof(uint32_t), "packed words")
+                   : nullptr;
+    if (pk) {
+        hipLaunchKernelGGL(k_vt_pack, dim3(elem_blocks(a->nnz)), dim3(256), 0, stream(), a->nnz,
+                           a->d_cols, idx, pk);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream()) != hipSuccess) {
+            dfree(pk);
+            dfree(tab);
+            dfree(idx);
+            return set_error(EDX_ERR_HIP, "packed form: packing failed");
+        }
+        a->d_apk = pk;
+        a->d_atab = tab;
+        a->natab = ntab;
+    } else {
+        dfree(tab);
+    }
     dfree(idx);
-    dfree(flag);
-    return rc;
+    return EDX_OK;
 }
 
 int csr_add_factored(edx_csr *a);

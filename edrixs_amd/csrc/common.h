@@ -128,6 +128,11 @@ struct edx_csr {
     int64_t fb_nnz;
     uint64_t *d_ftyp;       // nftyp types
     int nftyp;
+    // packed form of every entry (projections of the multi-shift solve):
+    // column | value number << 24, with the table of distinct values
+    uint32_t *d_apk;
+    double2 *d_atab;
+    int64_t natab;
 };
 
 // Lookup descriptor for a column space basis.

@@ -646,6 +646,8 @@ void edx_csr_free(edx_csr *a) {
     edx::dfree(a->d_fb_rowptr);
     edx::dfree(a->d_fb_pk);
     edx::dfree(a->d_ftyp);
+    edx::dfree(a->d_apk);
+    edx::dfree(a->d_atab);
     delete a;
 }
 

@@ -105,17 +105,53 @@ __global__ __launch_bounds__(256) void k_ms_scalars(int i, const double *alpha, 
     if (threadIdx.x == 0) ctl->nactive = cnt[0];
 }
 
-// (P v)[r] for a CSR P (nullptr rowptr: the identity), four entries at a time
-// (their columns and values, then four gathers in flight), summed in entry
-// order; element c of v is v[c * vs + vo] (column vo of an interleaved block
-// of vs vectors).
-__device__ __forceinline__ double2 ms_proj_row(int64_t r, const int64_t *prow, const int32_t *pcol,
-                                               const double2 *pval, const double2 *v, int vs,
+// A projection operator: a CSR with complex128 values, and where it has one
+// (csr_add_packed_all) its packed words (column | value number << 24) with
+// the table of distinct values, read instead of the columns and values.
+// prow == nullptr: the identity.
+struct MsP {
+    const int64_t *prow;
+    const int32_t *pcol;
+    const double2 *pval;
+    const uint32_t *ppk;
+    const double2 *ptab;
+};
+
+MsP ms_p(const edx_csr *P) {
+    if (!P) return MsP{};
+    return MsP{P->d_rowptr, P->d_cols, (const double2 *)P->d_vals, P->d_apk, P->d_atab};
+}
+
+// (P v)[r], four entries at a time (their columns and values, then four
+// gathers in flight), summed in entry order (the packed words give the same
+// values, so the same sums); element c of v is v[c * vs + vo] (column vo of an
+// interleaved block of vs vectors).
+__device__ __forceinline__ double2 ms_proj_row(int64_t r, const MsP &P, const double2 *v, int vs,
                                                int vo) {
-    if (!prow) return v[r * vs + vo];
+    if (!P.prow) return v[r * vs + vo];
     double2 w = make_double2(0, 0);
-    int64_t j = prow[r];
-    const int64_t je = prow[r + 1];
+    int64_t j = P.prow[r];
+    const int64_t je = P.prow[r + 1];
+    if (P.ppk) {
+        constexpr uint32_t mask = (1u << PK_COL_BITS) - 1;
+        for (; j + 4 <= je; j += 4) {
+            uint32_t wd[4];
+            double2 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) wd[u] = P.ppk[j + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = v[(int64_t)(wd[u] & mask) * vs + vo];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) w = cadd(w, cmul(P.ptab[wd[u] >> PK_COL_BITS], x[u]));
+        }
+        for (; j < je; ++j) {
+            const uint32_t wd = P.ppk[j];
+            w = cadd(w, cmul(P.ptab[wd >> PK_COL_BITS], v[(int64_t)(wd & mask) * vs + vo]));
+        }
+        return w;
+    }
+    const int32_t *pcol = P.pcol;
+    const double2 *pval = P.pval;
     for (; j + 4 <= je; j += 4) {
         int c[4];
         double2 a[4], x[4];
@@ -145,8 +181,7 @@ struct MsSlot {
     int active;      // the shift took this step (0 where the step did not run)
 };
 
-__global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *prow,
-                                                    const int32_t *pcol, const double2 *pval,
+__global__ __launch_bounds__(256) void k_ms_project(int64_t mP, MsP P,
                                                     const double2 *v, int vs, int vo,
                                                     const double *vscale, double2 *wout,
                                                     MsSlot *slot,
@@ -160,7 +195,7 @@ __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *p
     const double sc = *vscale;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < mP;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const double2 w = ms_proj_row(r, prow, pcol, pval, v, vs, vo);
+        const double2 w = ms_proj_row(r, P, v, vs, vo);
         wout[r] = make_double2(w.x * sc, w.y * sc);
     }
 }
@@ -171,9 +206,7 @@ __global__ __launch_bounds__(256) void k_ms_project(int64_t mP, const int64_t *p
 // (measured faster than workgroups of different entries side by side).
 constexpr int MS_PGROUP = 8;
 struct MsProjSet {
-    const int64_t *prow[MS_PGROUP];
-    const int32_t *pcol[MS_PGROUP];
-    const double2 *pval[MS_PGROUP];
+    MsP P[MS_PGROUP];
     int64_t mp[MS_PGROUP];
     double2 *wout[MS_PGROUP];       // this step's slot of w = s P v
     MsSlot *slot[MS_PGROUP];        // this step's slot of shift scalars [K]
@@ -206,7 +239,7 @@ __global__ __launch_bounds__(256) void k_ms_project_set(MsProjSet ps, const doub
          r += (int64_t)gridDim.x * blockDim.x)
         for (int e = 0; e < ps.n; ++e) {
             if (!s_on[e] || r >= ps.mp[e]) continue;
-            const double2 w = ms_proj_row(r, ps.prow[e], ps.pcol[e], ps.pval[e], v, vs, ps.vo[e]);
+            const double2 w = ms_proj_row(r, ps.P[e], v, vs, ps.vo[e]);
             ps.wout[e][r] = make_double2(w.x * s_sc[e], w.y * s_sc[e]);
         }
 }
@@ -217,8 +250,7 @@ __global__ __launch_bounds__(256) void k_ms_project_set(MsProjSet ps, const doub
 // step); v, vs, vo as in ms_proj_row.
 // Dynamic LDS: (NS + 1) K entries of inv_eta, zeta and active.
 template <int NS>
-__global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *prow,
-                                                   const int32_t *pcol, const double2 *pval,
+__global__ __launch_bounds__(256) void k_ms_update(int64_t mP, MsP P,
                                                    const double2 *v, int vs, int vo,
                                                    const double *vscale,
                                                    const double2 *wslots, const MsSlot *slots,
@@ -268,7 +300,7 @@ __global__ __launch_bounds__(256) void k_ms_update(int64_t mP, const int64_t *pr
             w[j] = s_step[j] ? wslots[(int64_t)j * mP + r] : make_double2(0, 0);
         w[NS] = make_double2(0, 0);
         if (s_step[NS]) {
-            const double2 p = ms_proj_row(r, prow, pcol, pval, v, vs, vo);
+            const double2 p = ms_proj_row(r, P, v, vs, vo);
             w[NS] = make_double2(p.x * sc, p.y * sc);
         }
         // four shifts at a time: their Q and out loads in flight together
@@ -367,8 +399,8 @@ int ms_window(int K) {
     return w;
 }
 
-int ms_update_launch(int ns, int64_t mp, const int64_t *pr, const int32_t *pc, const double2 *pv,
-                     const double2 *v, int vs, int vo, const double *vscale, const double2 *wslots,
+int ms_update_launch(int ns, int64_t mp, const MsP &P, const double2 *v, int vs, int vo,
+                     const double *vscale, const double2 *wslots,
                      const MsSlot *slots, const double *beta, const MsShift *sh, int K, double2 *Q,
                      double2 *out, const int *run, const MsCtl *ctl, int with_curr,
                      int bstride = 1) {
@@ -377,7 +409,7 @@ int ms_update_launch(int ns, int64_t mp, const int64_t *pr, const int32_t *pc, c
     hipStream_t s = edx::stream();
 #define EDX_MSU(NS)                                                                               \
     case NS:                                                                                      \
-        hipLaunchKernelGGL(k_ms_update<NS>, g, b, lds, s, mp, pr, pc, pv, v, vs, vo, vscale,      \
+        hipLaunchKernelGGL(k_ms_update<NS>, g, b, lds, s, mp, P, v, vs, vo, vscale,               \
                            wslots, slots, beta, bstride, sh, K, Q, out, run, ctl, with_curr);     \
         break;
     switch (ns) {
@@ -492,16 +524,14 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         for (int p = 0; p < nproj; ++p) {
             const edx_csr *P = proj[p];
             const int64_t mp = P ? P->m : n;
-            const int64_t *pr = P ? P->d_rowptr : nullptr;
-            const int32_t *pc = P ? P->d_cols : nullptr;
-            const double2 *pv = P ? (const double2 *)P->d_vals : nullptr;
+            const MsP mP = ms_p(P);
             if (j < win - 1)
-                hipLaunchKernelGGL(k_ms_project, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, pr, pc,
-                                   pv, v[curr], 1, 0, &S->coef[0].x,
+                hipLaunchKernelGGL(k_ms_project, dim3(vec_blocks(mp)), dim3(256), 0, s, mp, mP,
+                                   v[curr], 1, 0, &S->coef[0].x,
                                    wb.as<double2>() + woff[p] + (size_t)j * mp,
                                    slb.as<MsSlot>() + (size_t)j * K, sh, K, run + (i - 1), ctl);
             else
-                EDX_TRY(ms_update_launch(win - 1, mp, pr, pc, pv, v[curr], 1, 0, &S->coef[0].x,
+                EDX_TRY(ms_update_launch(win - 1, mp, mP, v[curr], 1, 0, &S->coef[0].x,
                                          wb.as<double2>() + woff[p], slb.as<MsSlot>(),
                                          d_beta + (i - win), sh, K, qb.as<double2>() + qoff[p],
                                          out[p], run + (i - 1), ctl, 1));
@@ -525,9 +555,7 @@ int shifted_fom_run(const edx_csr *h, int precision, const double2 *b, int K, co
         for (int p = 0; p < nproj; ++p) {
             const edx_csr *P = proj[p];
             const int64_t mp = P ? P->m : n;
-            EDX_TRY(ms_update_launch(nrem, mp, P ? P->d_rowptr : nullptr, P ? P->d_cols : nullptr,
-                                     P ? (const double2 *)P->d_vals : nullptr, v[curr], 1, 0,
-                                     &S->coef[0].x, wb.as<double2>() + woff[p], slb.as<MsSlot>(),
+            EDX_TRY(ms_update_launch(nrem, mp, ms_p(P), v[curr], 1, 0, &S->coef[0].x, wb.as<double2>() + woff[p], slb.as<MsSlot>(),
                                      d_beta + (last - nrem), sh, K, qb.as<double2>() + qoff[p],
                                      out[p], run + (last - 1), ctl, 0));
         }
@@ -659,9 +687,7 @@ int shifted_fom_batch_run(const edx_csr *h, int precision, int R, const double2 
             for (int e = 0; e < ps.n; ++e) {
                 const int r = r0 + e;
                 const edx_csr *P = proj[r];
-                ps.prow[e] = P ? P->d_rowptr : nullptr;
-                ps.pcol[e] = P ? P->d_cols : nullptr;
-                ps.pval[e] = P ? (const double2 *)P->d_vals : nullptr;
+                ps.P[e] = ms_p(P);
                 ps.mp[e] = P ? P->m : n;
                 ps.mmax = std::max(ps.mmax, ps.mp[e]);
                 ps.wout[e] = wb.as<double2>() + woff[r] + (size_t)j * ps.mp[e];
@@ -678,7 +704,7 @@ int shifted_fom_batch_run(const edx_csr *h, int precision, int R, const double2 
         if (j == win - 1)
             for (int r = 0; r < R; ++r) {
                 const int64_t mp = proj[r] ? proj[r]->m : n;
-                EDX_TRY(ms_update_launch(win, mp, nullptr, nullptr, nullptr, nullptr, R, r,
+                EDX_TRY(ms_update_launch(win, mp, MsP{}, nullptr, R, r,
                                          nullptr, wb.as<double2>() + woff[r],
                                          slb.as<MsSlot>() + (size_t)r * win * K,
                                          st.beta + (size_t)(i - win) * R + r, sh + (size_t)r * K,
@@ -706,7 +732,7 @@ int shifted_fom_batch_run(const edx_csr *h, int precision, int R, const double2 
     const int nrem = last % win;   // the last window's saved steps, not yet applied
     for (int r = 0; r < R && nrem; ++r) {
         const int64_t mp = proj[r] ? proj[r]->m : n;
-        EDX_TRY(ms_update_launch(nrem, mp, nullptr, nullptr, nullptr, nullptr, R, r, nullptr,
+        EDX_TRY(ms_update_launch(nrem, mp, MsP{}, nullptr, R, r, nullptr,
                                  wb.as<double2>() + woff[r], slb.as<MsSlot>() + (size_t)r * win * K,
                                  st.beta + (size_t)(last - nrem) * R + r, sh + (size_t)r * K, K,
                                  qb.as<double2>() + qoff[r], out[r],
@@ -754,6 +780,8 @@ int edx_shifted_fom_project(const edx_csr *h, int precision, const void *d_b, in
                       "projection %d has %lld columns, matrix has %lld rows", p,
                       (long long)proj[p]->n, (long long)h->n);
     }
+    for (int p = 0; p < nproj; ++p)
+        if (proj[p]) EDX_TRY(edx::csr_add_packed_all(const_cast<edx_csr *>(proj[p])));
     std::vector<double2 *> outs(nproj);
     for (int p = 0; p < nproj; ++p) outs[p] = (double2 *)d_out[p];
     return shifted_fom_run(h, precision, (const double2 *)d_b, nshift, sigma, nproj, proj,
@@ -784,6 +812,8 @@ int edx_shifted_fom_project_batch(const edx_csr *h, int precision, int nrhs, con
                       "projection %d has %lld columns, matrix has %lld rows", r,
                       (long long)proj[r]->n, (long long)h->n);
     }
+    for (int r = 0; r < nrhs; ++r)
+        if (proj[r]) EDX_TRY(edx::csr_add_packed_all(const_cast<edx_csr *>(proj[r])));
     std::vector<double2 *> outs(nrhs);
     for (int r = 0; r < nrhs; ++r) outs[r] = (double2 *)d_out[r];
     return shifted_fom_batch_run(h, precision, nrhs, (const double2 *)d_B, nshift, sigma, proj,

@@ -464,11 +464,13 @@ namespace edx {
 // ---------------------------------------------------------------- profiling
 // Launches are bracketed by a ring of PROF_CAP event pairs.  Before a slot is
 // reused, its launch (PROF_CAP launches back, long finished: the host never
-// runs that far ahead of the device) is folded into per-(precision, kernel)
-// totals, so every launch of a profiled region counts however long it runs.
+// runs that far ahead of the device) is folded into per-(precision, kernel,
+// right-hand sides) totals, so every launch of a profiled region counts
+// however long it runs.
 struct ProfRec {
     int kind;      // precision
     int kernel;    // EDX_KERNEL_*
+    int nrhs;      // right-hand sides of the launch
     double bytes;
     double flops;
     bool stopped;  // the stop event was recorded (the launch was made)
@@ -478,11 +480,13 @@ struct ProfSum {
     double ms = 0.0, bytes = 0.0, flops = 0.0;
 };
 constexpr int PROF_KINDS = 8, PROF_KERNELS = 16;
+constexpr int PROF_RHS = 17;   // buckets 1..16 right-hand sides, 0: more than 16
+inline int prof_rhs_bucket(int nrhs) { return nrhs >= 1 && nrhs < PROF_RHS ? nrhs : 0; }
 static bool g_prof = false;
 static std::vector<hipEvent_t> g_ev;      // pairs: 2k start, 2k+1 stop
 static std::vector<ProfRec> g_rec;        // PROF_CAP slots
 static uint64_t g_prof_n = 0;             // launches recorded since edx_profile(1)
-static ProfSum g_sum[PROF_KINDS][PROF_KERNELS];   // launches folded out of the ring
+static ProfSum g_sum[PROF_KINDS][PROF_KERNELS][PROF_RHS];   // launches folded out of the ring
 constexpr size_t PROF_CAP = 16384;
 
 static int prof_fold(size_t slot, ProfSum &acc) {
@@ -498,15 +502,17 @@ static int prof_fold(size_t slot, ProfSum &acc) {
     return EDX_OK;
 }
 
-static hipEvent_t prof_start(int kind, int kernel, double bytes, double flops = 0.0) {
+static hipEvent_t prof_start(int kind, int kernel, double bytes, double flops = 0.0,
+                             int nrhs = 1) {
     if (!g_prof || g_ev.size() < 2 * PROF_CAP) return nullptr;
     if (kind < 0 || kind >= PROF_KINDS || kernel < 0 || kernel >= PROF_KERNELS) return nullptr;
     const size_t slot = (size_t)(g_prof_n % PROF_CAP);
     if (g_prof_n >= PROF_CAP) {
         const ProfRec &old = g_rec[slot];
-        if (prof_fold(slot, g_sum[old.kind][old.kernel]) != EDX_OK) return nullptr;
+        if (prof_fold(slot, g_sum[old.kind][old.kernel][prof_rhs_bucket(old.nrhs)]) != EDX_OK)
+            return nullptr;
     }
-    g_rec[slot] = ProfRec{kind, kernel, bytes, flops, false};
+    g_rec[slot] = ProfRec{kind, kernel, nrhs, bytes, flops, false};
     ++g_prof_n;
     hipEvent_t e = g_ev[2 * slot];
     (void)hipEventRecord(e, stream());
@@ -519,24 +525,27 @@ static void prof_stop(hipEvent_t started) {
     g_rec[slot].stopped = hipEventRecord(g_ev[2 * slot + 1], stream()) == hipSuccess;
 }
 
-// Totals of one precision and one kernel (-1: all kernels): the folded
-// launches plus those still in the ring.
-static int prof_total(int precision, int kernel, ProfSum *out) {
+// Totals of one precision, one kernel (-1: all kernels) and one count of
+// right-hand sides (-1: all; above 16: every launch of more than 16): the
+// folded launches plus those still in the ring.
+static int prof_total(int precision, int kernel, int nrhs, ProfSum *out) {
     ProfSum acc;
     if (precision >= 0 && precision < PROF_KINDS)
         for (int k = 0; k < PROF_KERNELS; ++k)
-            if (kernel < 0 || kernel == k) {
-                const ProfSum &s = g_sum[precision][k];
-                acc.n += s.n;
-                acc.ms += s.ms;
-                acc.bytes += s.bytes;
-                acc.flops += s.flops;
-            }
+            for (int b = 0; b < PROF_RHS; ++b)
+                if ((kernel < 0 || kernel == k) && (nrhs < 0 || prof_rhs_bucket(nrhs) == b)) {
+                    const ProfSum &s = g_sum[precision][k][b];
+                    acc.n += s.n;
+                    acc.ms += s.ms;
+                    acc.bytes += s.bytes;
+                    acc.flops += s.flops;
+                }
     const uint64_t first = g_prof_n > PROF_CAP ? g_prof_n - PROF_CAP : 0;
     for (uint64_t i = first; i < g_prof_n; ++i) {
         const size_t slot = (size_t)(i % PROF_CAP);
         if (g_rec[slot].kind != precision) continue;
         if (kernel >= 0 && g_rec[slot].kernel != kernel) continue;
+        if (nrhs >= 0 && prof_rhs_bucket(g_rec[slot].nrhs) != prof_rhs_bucket(nrhs)) continue;
         EDX_TRY(prof_fold(slot, acc));
     }
     *out = acc;
@@ -777,7 +786,7 @@ int spmm_launch(const edx_csr *a, int precision, int nrhs, const void *X, double
     if (blocks > cap) blocks = cap;
     dim3 grid((unsigned)blocks, (unsigned)((nrhs + NR - 1) / NR));
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMM_CM,
-                               spmv_bytes(a, precision, nrhs, Z != nullptr, false));
+                               spmv_bytes(a, precision, nrhs, Z != nullptr, false), 0.0, nrhs);
 #define EDX_SPMM_CASE(GG)                                                                       \
     case GG:                                                                                    \
         if (precision == EDX_C128)                                                              \
@@ -816,8 +825,9 @@ int edx_profile(int enable) {
             edx::g_rec.resize(edx::PROF_CAP);
         }
         edx::g_prof_n = 0;
-        for (auto &row : edx::g_sum)
-            for (auto &s : row) s = edx::ProfSum{};
+        for (auto &plane : edx::g_sum)
+            for (auto &row : plane)
+                for (auto &s : row) s = edx::ProfSum{};
     }
     edx::g_prof = enable != 0;
     return EDX_OK;
@@ -825,9 +835,14 @@ int edx_profile(int enable) {
 
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
                      double *total_bytes) {
+    return edx_profile_read_rhs(precision, kernel, -1, launches, total_ms, total_bytes);
+}
+
+int edx_profile_read_rhs(int precision, int kernel, int nrhs, int64_t *launches,
+                         double *total_ms, double *total_bytes) {
     EDX_HIP(hipStreamSynchronize(edx::stream()));
     edx::ProfSum s;
-    EDX_TRY(edx::prof_total(precision, kernel, &s));
+    EDX_TRY(edx::prof_total(precision, kernel, nrhs, &s));
     if (launches) *launches = s.n;
     if (total_ms) *total_ms = s.ms;
     if (total_bytes) *total_bytes = s.bytes;
@@ -838,7 +853,7 @@ int edx_profile_read_flops(int precision, int kernel, double *total_flops) {
     EDX_CHECK(total_flops, EDX_ERR_ARG, "null argument");
     EDX_HIP(hipStreamSynchronize(edx::stream()));
     edx::ProfSum s;
-    EDX_TRY(edx::prof_total(precision, kernel, &s));
+    EDX_TRY(edx::prof_total(precision, kernel, -1, &s));
     *total_flops = s.flops;
     return EDX_OK;
 }

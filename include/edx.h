@@ -190,7 +190,9 @@ int edx_csr_factored_blocks(const edx_csr *a, int64_t *blocks, int *types);
  * edx_profile(1) clears and starts, edx_profile(0) stops;
  * edx_profile_read synchronises and returns the launch count, summed kernel
  * milliseconds and summed bytes of the launches in one precision
- * (edx_precision) and one kernel (edx_kernel, or -1 for all). */
+ * (edx_precision) and one kernel (edx_kernel, or -1 for all);
+ * edx_profile_read_rhs the same for the launches with nrhs right-hand sides
+ * (-1: all; any count above 16 reads every launch of more than 16). */
 enum edx_kernel {
     EDX_KERNEL_SPMV = 0,      /* k_spmv: one vector */
     EDX_KERNEL_SPMM_IL = 1,   /* k_spmm_il: K < 8 interleaved right-hand sides */
@@ -205,6 +207,8 @@ enum edx_kernel {
 int edx_profile(int enable);
 int edx_profile_read(int precision, int kernel, int64_t *launches, double *total_ms,
                      double *total_bytes);
+int edx_profile_read_rhs(int precision, int kernel, int nrhs, int64_t *launches,
+                         double *total_ms, double *total_bytes);
 /* Summed algorithmic floating-point operations of the same launches (nonzero
  * for the matrix-core kernels only: 512 real flops per row and basis block). */
 int edx_profile_read_flops(int precision, int kernel, double *total_flops);

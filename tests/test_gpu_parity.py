@@ -673,6 +673,34 @@ def test_shifted_fom_batch_matches_dense_and_single_solves(core, precision):
                                [None] * 65, precision=precision)
 
 
+def test_shifted_fom_packed_projection_equals_csr_projection(core, monkeypatch):
+    # the projection read as packed words (column | value number, the table of
+    # its few distinct values) gives the complex128 CSR's sums bit for bit: the
+    # same values in the same order; one operator made before packing is
+    # switched on, so it keeps the CSR
+    m, fi, fn, Rn, Dout, b, sig = _rixs_case()
+    bi, bn = core.Basis.from_words(fi, m.norb), core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    bd = _to_dev(core, b)
+    outs = []
+    for off in ("1", "0"):
+        monkeypatch.setenv("EDX_NO_PACKED_PROJ", off)
+        Df = core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1])
+        (Y,), it, res = core.shifted_fom(Hn, bd, sig, projections=(Df,), tol=1e-10, maxit=4000)
+        (Yb,), itb, resb = core.shifted_fom_batch(Hn, bd.reshape(1, -1), sig, (Df,), tol=1e-10,
+                                                  maxit=4000)
+        outs.append((Y.cpu().numpy(), it, res, Yb.cpu().numpy(), itb, resb))
+    monkeypatch.delenv("EDX_NO_PACKED_PROJ")
+    (Y0, it0, r0, Yb0, itb0, rb0), (Y1, it1, r1, Yb1, itb1, rb1) = outs
+    assert np.array_equal(Y0, Y1) and np.array_equal(it0, it1) and np.array_equal(r0, r1)
+    assert np.array_equal(Yb0, Yb1) and np.array_equal(itb0, itb1) and np.array_equal(rb0, rb1)
+    Dd = Dout.todense()
+    A = Rn.todense()
+    for k in range(len(sig)):
+        x = np.linalg.solve(sig[k] * np.eye(A.shape[0]) - A, b)
+        assert np.linalg.norm(Y1[k] - Dd @ x) < 1e-7 * np.linalg.norm(Dd @ x)
+
+
 def test_shifted_fom_windowed_updates_equal_per_step_updates(core, monkeypatch):
     # the projected updates made in windows of 2, 3 and 8 steps (one pass over
     # the shifts' vectors per window) against one pass per step, for a run

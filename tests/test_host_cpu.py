@@ -55,11 +55,14 @@ def test_bench_reads_measured_traffic():
     sys.path.insert(0, ROOT)
     import bench
     for kernel, form in (("k_spmv_fac", "c128i"), ("k_spmv", "c128i"), ("k_spmv", "c64d"),
-                         ("k_spmm_rl", "c128i")):
+                         ("k_spmm_rl_k16", "c128i"), ("k_spmm_fac_k3", "c128i"),
+                         ("k_spmm_fac_k2", "c128i")):
         v = bench.measured_traffic(kernel, form, 24)
         assert v is not None and 1e9 < v < 1e11
     assert bench.measured_traffic("k_spmv", "c128i", 20) is None
     assert bench.measured_traffic("k_nope", "c128i", 24) is None
+    assert bench.traffic_key("k_spmm_fac", 3) == "k_spmm_fac_k3"
+    assert bench.traffic_key("k_spmv_fac", 1) == "k_spmv_fac"
 
 
 def test_precision_names():
