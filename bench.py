@@ -974,8 +974,9 @@ def main():
             "workload": ("norb=24 N=12 SIAM (10 d + 14 bath, 2p core), end to end: CSR builds of "
                          "H_i (%d rows) and H_n (%d rows), ground state, isotropic XAS (3 batched "
                          "Krylov recurrences, >= %d steps, to converged continued fractions) and RIXS at %d incident energies x 2 polarizations "
-                         "(one multi-shift Galerkin recurrence over H_n per polarization, projected by "
-                         "D_out, + batched final-state Krylov over H_i to converged continued fractions), "
+                         "(one multi-shift Galerkin recurrence over H_n for both polarizations, a 2-vector "
+                         "block, projected by D_out, + batched final-state Krylov over H_i to converged "
+                         "continued fractions), "
                          "continued fractions on %d points"
                          % (work.bi.n, work.bn.n * m.ncore, args.nkryl, args.n_in, args.nw)),
             "form": main_form, "spectra_per_step": nspec,

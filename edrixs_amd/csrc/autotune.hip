@@ -224,7 +224,7 @@ int tune_bspmm(const edx_csr *a, int precision, int K, BSched *sched) {
             if (bsched_blocks(a, BSched{4, bpc}) <= bspmm_blocks(a->m))
                 cand.push_back(BSched{4, bpc});
     if (fac_spmm_ok(a, precision, K))   // slot-factored, workgroups per CU
-        for (int bpc : {4, 8})
+        for (int bpc : {4, 5, 6, 8})
             if (bsched_blocks(a, BSched{3, bpc}) <= bspmm_blocks(a->m))
                 cand.push_back(BSched{3, bpc});
     const size_t nk = (size_t)a->n * K;
