@@ -181,13 +181,14 @@ class Workload:
             self.w_in = np.linspace(edge - 2.0, edge + 8.0, a.n_in)
         # RIXS: one multi-shift recurrence over H_n serves every incident energy; the
         # incident polarizations run as one interleaved recurrence (a 2-vector block
-        # product per step), each projected by its own D_out; only D_out x is kept
+        # product per step) where H_n has the slot-factored form, else one each,
+        # each projected by its own D_out; only D_out x is kept
         t0 = time.perf_counter()
         npol = len(self.POLS)
         sigma = np.array([w + egs + 1j * self.GAMMA_C for w in self.w_in])
         B = torch.stack([Din[p].matvec(gs) for p in range(npol)]).contiguous()
-        proj, it_b, res_b = c.shifted_fom_batch(Hn, B, sigma, Dout, tol=a.rixs_tol,
-                                                maxit=a.rixs_maxit, precision=precision)
+        proj, it_b, res_b = c.shifted_fom_rhs(Hn, B, sigma, Dout, tol=a.rixs_tol,
+                                              maxit=a.rixs_maxit, precision=precision)
         iters = [int(v) for v in it_b.reshape(-1)]
         res = [float(v) for v in res_b.reshape(-1)]
         torch.cuda.synchronize()
@@ -975,8 +976,8 @@ def main():
                          "H_i (%d rows) and H_n (%d rows), ground state, isotropic XAS (3 batched "
                          "Krylov recurrences, >= %d steps, to converged continued fractions) and RIXS at %d incident energies x 2 polarizations "
                          "(one multi-shift Galerkin recurrence over H_n for both polarizations, a 2-vector "
-                         "block, projected by D_out, + batched final-state Krylov over H_i to converged "
-                         "continued fractions), "
+                         "block where H_n has the slot-factored form, projected by D_out, + batched "
+                         "final-state Krylov over H_i to converged continued fractions), "
                          "continued fractions on %d points"
                          % (work.bi.n, work.bn.n * m.ncore, args.nkryl, args.n_in, args.nw)),
             "form": main_form, "spectra_per_step": nspec,

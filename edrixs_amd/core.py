@@ -744,6 +744,31 @@ def shifted_fom_batch(H, B, sigma, projections, tol=1e-8, maxit=2000, precision=
     return outs, it, res
 
 
+def shifted_fom_rhs(H, B, sigma, projections, tol=1e-8, maxit=2000, precision="c128"):
+    """shifted_fom for each row b_r of B (R, n) with its projection, in the
+    results layout of shifted_fom_batch: one interleaved recurrence where the
+    block product pays (the rule of converged_spectra: 2 to 4 rows over a
+    matrix with the slot-factored form in the indexed precision, 8 or more
+    rows, or a block within SPLIT_BLOCK_BYTES), else one recurrence per row
+    (a larger interleaved block of a few rows overflows the last-level cache
+    and its row-wise product loses to the single-vector ones)."""
+    T = torch()
+    B = B.contiguous()
+    R = B.shape[0]
+    fac_block = 2 <= R <= 4 and _prec(precision) == EDX_C128I and H.factored() is not None
+    if R > 1 and (fac_block or R >= 8 or R * H.n * 16 <= SPLIT_BLOCK_BYTES):
+        return shifted_fom_batch(H, B, sigma, projections, tol=tol, maxit=maxit,
+                                 precision=precision)
+    outs, its, rss = [], [], []
+    for r in range(R):
+        (Y,), it, res = shifted_fom(H, B[r], sigma, projections=(projections[r],), tol=tol,
+                                    maxit=maxit, precision=precision)
+        outs.append(Y)
+        its.append(it)
+        rss.append(res)
+    return outs, np.stack(its), np.stack(rss)
+
+
 def minres(A, b, tol=1e-10, maxit=500, precision="c128"):
     T = torch()
     _check_vec(b, A.n)

@@ -673,6 +673,28 @@ def test_shifted_fom_batch_matches_dense_and_single_solves(core, precision):
                                [None] * 65, precision=precision)
 
 
+def test_shifted_fom_rhs_chooses_block_or_separate_recurrences(core, monkeypatch):
+    # a small block runs as one interleaved recurrence (shifted_fom_batch's
+    # results bit for bit); with the block-size limit at zero, as one
+    # recurrence per row (shifted_fom's results bit for bit)
+    m, fi, fn, Rn, Dout, b, sig = _rixs_case()
+    bi, bn = core.Basis.from_words(fi, m.norb), core.Basis.from_words(fn, m.norb)
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n)
+    Df = core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[1])
+    Bd = _to_dev(core, np.stack([b, np.roll(b, 3)]))
+    kw = dict(tol=1e-10, maxit=4000, precision="c128")
+    o1, i1, r1 = core.shifted_fom_rhs(Hn, Bd, sig, (Df, None), **kw)
+    o2, i2, r2 = core.shifted_fom_batch(Hn, Bd, sig, (Df, None), **kw)
+    assert all(np.array_equal(x.cpu().numpy(), y.cpu().numpy()) for x, y in zip(o1, o2))
+    assert np.array_equal(i1, i2) and np.array_equal(r1, r2)
+    monkeypatch.setattr(core, "SPLIT_BLOCK_BYTES", 0)
+    o3, i3, r3 = core.shifted_fom_rhs(Hn, Bd, sig, (Df, None), **kw)
+    for r, P in ((0, Df), (1, None)):
+        (Y,), it, res = core.shifted_fom(Hn, Bd[r], sig, projections=(P,), **kw)
+        assert np.array_equal(o3[r].cpu().numpy(), Y.cpu().numpy())
+        assert np.array_equal(i3[r], it) and np.array_equal(r3[r], res)
+
+
 def test_shifted_fom_packed_projection_equals_csr_projection(core, monkeypatch):
     # the projection read as packed words (column | value number, the table of
     # its few distinct values) gives the complex128 CSR's sums bit for bit: the
