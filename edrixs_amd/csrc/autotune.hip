@@ -20,8 +20,8 @@
 // workgroup), so two processes that picked different schedules for the same
 // matrix would round differently.  Choices therefore persist in a text file,
 // one line per choice keyed also by the device (architecture and CU count):
-// $EDX_TUNE_CACHE, else $XDG_CACHE_HOME/edrixs_amd/tune-v1.txt, else
-// $HOME/.cache/edrixs_amd/tune-v1.txt.  A process reads it on its first
+// $EDX_TUNE_CACHE, else $XDG_CACHE_HOME/edrixs_amd/tune-v2.txt, else
+// $HOME/.cache/edrixs_amd/tune-v2.txt.  A process reads it on its first
 // lookup and appends what it tunes; the first line of a key wins, so every
 // later process runs the schedule the first one chose.  EDX_TUNE_CACHE=0 (or
 // empty) keeps choices in memory only.
@@ -50,9 +50,9 @@ std::string tune_cache_path() {
     const char *p = getenv("EDX_TUNE_CACHE");
     if (p) return (p[0] == 0 || (p[0] == '0' && p[1] == 0)) ? std::string() : std::string(p);
     const char *x = getenv("XDG_CACHE_HOME");
-    if (x && x[0]) return std::string(x) + "/edrixs_amd/tune-v1.txt";
+    if (x && x[0]) return std::string(x) + "/edrixs_amd/tune-v2.txt";
     const char *h = getenv("HOME");
-    if (h && h[0]) return std::string(h) + "/.cache/edrixs_amd/tune-v1.txt";
+    if (h && h[0]) return std::string(h) + "/.cache/edrixs_amd/tune-v2.txt";
     return std::string();
 }
 

@@ -673,6 +673,39 @@ def test_shifted_fom_batch_matches_dense_and_single_solves(core, precision):
                                [None] * 65, precision=precision)
 
 
+def test_shifted_fom_batch_over_the_factored_block_matches_single_solves(core, monkeypatch):
+    # the path of the benchmark step: two right-hand sides over H_n in the
+    # interleaved core-hole layout with its slot-factored form (indexed
+    # precision), the product the factored 2-vector block (forced: a matrix
+    # this size is below the autotuner's threshold), each projected by its own
+    # emission operator, against one plain complex128 recurrence per right-hand
+    # side in the reference layout (itself pinned to dense solves above)
+    m, fn, bn, A, R = _siam_hn(core)
+    fi = H.fock_words(m.norb, m.nocc)
+    bi = core.Basis.from_words(fi, m.norb)
+    lay = "interleaved"
+    Hn = core.build_ham_n(bn, m.norb, m.ncore, m.hop_n, m.coul_n, layout=lay)
+    Hn.prepare("c128i")
+    assert Hn.factored() is not None
+    Df = [core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[k], layout=lay) for k in (1, 2)]
+    Dr = [core.build_transop_f(bi, bn, m.norb, m.ncore, m.trans_f[k]) for k in (1, 2)]
+    p = core.corehole_permutation(len(fn), m.ncore)       # x_inter = x_ref[p]
+    rng = np.random.default_rng(41)
+    Bref = rng.normal(size=(2, A.n)) + 1j * rng.normal(size=(2, A.n))
+    sig = np.linspace(-10.0, 20.0, 6) + 0.5j
+    monkeypatch.setenv("EDX_FORCE_BSCHED", "3:4")
+    outs, it, res = core.shifted_fom_rhs(Hn, _to_dev(core, Bref[:, p]), sig, Df, tol=1e-10,
+                                         maxit=4000, precision="c128i")
+    monkeypatch.delenv("EDX_FORCE_BSCHED")
+    for r in range(2):
+        (Ys,), its, rss = core.shifted_fom(A, _to_dev(core, Bref[r]), sig, projections=(Dr[r],),
+                                           tol=1e-10, maxit=4000, precision="c128")
+        Y, Ys = outs[r].cpu().numpy(), Ys.cpu().numpy()
+        assert np.all(res[r] <= 1e-10) and np.all(it[r] > 0) and np.all(it[r] < 4000)
+        for j in range(len(sig)):
+            assert np.linalg.norm(Y[j] - Ys[j]) < 1e-7 * np.linalg.norm(Ys[j]), (r, j)
+
+
 def test_shifted_fom_rhs_chooses_block_or_separate_recurrences(core, monkeypatch):
     # a small block runs as one interleaved recurrence (shifted_fom_batch's
     # results bit for bit); with the block-size limit at zero, as one
