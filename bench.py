@@ -340,7 +340,7 @@ def timed(work, precision, steps, warmup, reserve=lambda t: 0.0):
         for r in range(1, 18):
             got = c.profile_read(precision, k, r)
             if got[0]:
-                prof["by_rhs"][(k, r)] = got
+                prof["by_rhs"][(k, r)] = got + (c.profile_read_alg(precision, k, r),)
     c.profile(False)
     return dt, done, out, prof
 
@@ -908,8 +908,9 @@ def main():
     # (one template instance) with the most kernel time
     by_rhs = profs[main_form]["by_rhs"]
     kname, krhs = max(by_rhs, key=lambda kr: by_rhs[kr][1])
-    n, ms, by = by_rhs[(kname, krhs)]
-    gbs = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    n, ms, by, aby = by_rhs[(kname, krhs)]
+    gbs = aby / (ms * 1e-3) / 1e9 if ms > 0 else 0.0          # algorithmic (CSR) bytes
+    sgbs = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0          # the stored form's bytes
     m = work.model
     forms_out = {}
     ref = results.get("c128")
@@ -924,10 +925,11 @@ def main():
                 kern[k] = {"launches": pn, "total_s": pms * 1e-3, "avg_us": pms / pn * 1e3,
                            "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
         inst = {}
-        for (k, nr), (pn, pms, pby) in sorted(profs[f]["by_rhs"].items()):
+        for (k, nr), (pn, pms, pby, paby) in sorted(profs[f]["by_rhs"].items()):
             inst[traffic_key(k, nr)] = {"launches": pn, "total_s": pms * 1e-3,
                                         "avg_us": pms / pn * 1e3,
-                                        "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
+                                        "alg_gbs": paby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0,
+                                        "stored_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
         d = {"spectra_per_s": nspec * nsteps[f] / times[f],
              "ms_per_step": times[f] / nsteps[f] * 1e3, "steps": nsteps[f],
              "phases_s": r["info"]["phases_s"],
@@ -991,30 +993,20 @@ def main():
             "kernel": "%s, %d right-hand side%s (%s form), every such launch of the timed region"
                       % (kname, krhs, "" if krhs == 1 else "s", main_form),
             "nrhs": krhs,
-            "launches": n, "avg_us": ms / max(n, 1) * 1e3, "bytes_per_launch": by / max(n, 1),
-            "bytes_counted": ("the stored form's bytes: index words, row pointers, value table, "
-                              "diagonal, x once, y written, the epilogue's addend and dot vector"),
+            "launches": n, "avg_us": ms / max(n, 1) * 1e3, "bytes_per_launch": aby / max(n, 1),
+            "bytes_counted": ("algorithmic bytes of the product as the complex128 CSR of the same "
+                              "FP64 operator: nnz*(16+4) + 8*(m+1) + K*(16*n + 16*m*(1+a)), a = 1 "
+                              "(the Krylov epilogue accumulates into y)"),
+            "stored_form": {
+                "achieved": sgbs, "frac": sgbs / HBM_PEAK_GBS, "unit": "GB/s",
+                "bytes_per_launch": by / max(n, 1),
+                "bytes_counted": ("the bytes the launch's storage form reads and writes: index "
+                                  "words, row pointers, value and type tables, diagonal, and per "
+                                  "right-hand side x once, y written, the epilogue's addend and "
+                                  "dot vector")},
         },
         "forms": forms_out,
     }
-    inf = results[main_form]["info"]
-    if kname in ("k_spmv_fac", "k_spmm_fac") and inf.get("fac_n") and ms > 0:
-        # the same launches priced as the complex128 CSR the reference's
-        # matvec_csr streams (src/spmv.f90:73-103) once per vector: 16-byte value
-        # and 4-byte column per nonzero (diagonal included), 8-byte row
-        # pointers, the same vector terms
-        mh, (nc, hv, rr, nb, nt) = inf["m_n"], inf["fac_n"]
-        mv = mh // nc
-        stored = ((hv + rr + nb) * 4.0 + (mv + 1) * 8.0 + ((mv + 1) * 8.0 if nb else 0.0) +
-                  ((mh + 1) * 8.0 if rr else 0.0) + nt * 8.0 + mh * 16.0)
-        csr = krhs * (inf["nnz_n"] * 20.0 + (mh + 1) * 8.0)
-        cby = by + n * (csr - stored)
-        line["roofline"]["csr_equivalent"] = {
-            "achieved": cby / (ms * 1e-3) / 1e9, "unit": "GB/s", "bytes_per_launch": cby / max(n, 1),
-            "bytes_counted": ("the complex128 CSR of H_n (20 bytes per nonzero, row pointers), "
-                              "read once per vector as the reference's single-vector product "
-                              "does, in place of the factored form's words, pointers and "
-                              "diagonal; the same vector terms")}
     if "c128" in results:
         line["speedup_vs_fp64"] = (times["c128"] / nsteps["c128"]) / (t_main / nsteps[main_form])
     if not args.no_cpu:

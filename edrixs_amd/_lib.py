@@ -99,6 +99,7 @@ _SIGS = {
                               ctypes.POINTER(_D)]),
     "edx_profile_read_rhs": (_I, [_I, _I, _I, ctypes.POINTER(_I64), ctypes.POINTER(_D),
                                   ctypes.POINTER(_D)]),
+    "edx_profile_read_alg": (_I, [_I, _I, _I, ctypes.POINTER(_D)]),
     "edx_profile_read_flops": (_I, [_I, _I, ctypes.POINTER(_D)]),
     "edx_set_corehole_layout": (_I, [_I, ctypes.POINTER(_I)]),
     "edx_lanczos_eigs": (_I, [_P, _I, _I, _I, _D, ctypes.c_uint64, _I, _I, _P, _P,

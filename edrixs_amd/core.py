@@ -836,6 +836,18 @@ def profile_read(precision="c128", kernel=None, nrhs=None):
     return n.value, ms.value, by.value
 
 
+def profile_read_alg(precision="c128", kernel=None, nrhs=None):
+    """Summed algorithmic bytes of the same launches counted as the complex128
+    CSR of the same operator (20 bytes per nonzero, row pointers, per vector x
+    once and y written, and read where the epilogue accumulates), whatever
+    form they read."""
+    b = ctypes.c_double()
+    k = -1 if kernel is None else {**KERNELS, **BLOCK_KERNELS}[kernel]
+    _lib.call("edx_profile_read_alg", _prec(precision), k, -1 if nrhs is None else int(nrhs),
+              ctypes.byref(b))
+    return b.value
+
+
 def profile_read_flops(kernel, precision="c128"):
     """Summed algorithmic flops of the recorded launches of a matrix-core kernel."""
     f = ctypes.c_double()

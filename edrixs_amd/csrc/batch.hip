@@ -1075,7 +1075,7 @@ int bspmm(const edx_csr *a, int precision, int K, const double2 *X64, const floa
     hipEvent_t pe = prof_start(precision, sched.rl == 3 ? EDX_KERNEL_SPMM_FAC
                                           : sched.rl == 0 ? EDX_KERNEL_SPMM_IL
                                                           : EDX_KERNEL_SPMM_RL,
-                               bytes, 0.0, K);
+                               bytes, 0.0, K, csr_alg_bytes(a, K, e.z != nullptr));
     EDX_TRY(bspmm_launch(a, precision, K, X64, X32, e, sched));
     prof_stop(pe);
     return EDX_OK;

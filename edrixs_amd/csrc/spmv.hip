@@ -471,13 +471,14 @@ struct ProfRec {
     int kind;      // precision
     int kernel;    // EDX_KERNEL_*
     int nrhs;      // right-hand sides of the launch
-    double bytes;
+    double bytes;  // the bytes the launch's storage form reads and writes
     double flops;
+    double abytes; // algorithmic bytes of the product as a complex128 CSR (csr_alg_bytes)
     bool stopped;  // the stop event was recorded (the launch was made)
 };
 struct ProfSum {
     int64_t n = 0;
-    double ms = 0.0, bytes = 0.0, flops = 0.0;
+    double ms = 0.0, bytes = 0.0, flops = 0.0, abytes = 0.0;
 };
 constexpr int PROF_KINDS = 8, PROF_KERNELS = 16;
 constexpr int PROF_RHS = 17;   // buckets 1..16 right-hand sides, 0: more than 16
@@ -499,11 +500,12 @@ static int prof_fold(size_t slot, ProfSum &acc) {
     acc.ms += t;
     acc.bytes += r.bytes;
     acc.flops += r.flops;
+    acc.abytes += r.abytes;
     return EDX_OK;
 }
 
 static hipEvent_t prof_start(int kind, int kernel, double bytes, double flops = 0.0,
-                             int nrhs = 1) {
+                             int nrhs = 1, double abytes = 0.0) {
     if (!g_prof || g_ev.size() < 2 * PROF_CAP) return nullptr;
     if (kind < 0 || kind >= PROF_KINDS || kernel < 0 || kernel >= PROF_KERNELS) return nullptr;
     const size_t slot = (size_t)(g_prof_n % PROF_CAP);
@@ -512,7 +514,7 @@ static hipEvent_t prof_start(int kind, int kernel, double bytes, double flops = 
         if (prof_fold(slot, g_sum[old.kind][old.kernel][prof_rhs_bucket(old.nrhs)]) != EDX_OK)
             return nullptr;
     }
-    g_rec[slot] = ProfRec{kind, kernel, nrhs, bytes, flops, false};
+    g_rec[slot] = ProfRec{kind, kernel, nrhs, bytes, flops, abytes, false};
     ++g_prof_n;
     hipEvent_t e = g_ev[2 * slot];
     (void)hipEventRecord(e, stream());
@@ -539,6 +541,7 @@ static int prof_total(int precision, int kernel, int nrhs, ProfSum *out) {
                     acc.ms += s.ms;
                     acc.bytes += s.bytes;
                     acc.flops += s.flops;
+                    acc.abytes += s.abytes;
                 }
     const uint64_t first = g_prof_n > PROF_CAP ? g_prof_n - PROF_CAP : 0;
     for (uint64_t i = first; i < g_prof_n; ++i) {
@@ -550,6 +553,17 @@ static int prof_total(int precision, int kernel, int nrhs, ProfSum *out) {
     }
     *out = acc;
     return EDX_OK;
+}
+
+// Algorithmic bytes of a product with K right-hand sides counted as the
+// complex128 CSR of the same operator, whatever form the launch reads: 16-byte
+// value and 4-byte column per nonzero (diagonal included), 8-byte row
+// pointers, and per right-hand side x once and y written, read as well where
+// the epilogue accumulates into it (nnz (16 + 4) + 8 (m + 1) + K (16 n +
+// 16 m (1 + acc))).
+double csr_alg_bytes(const edx_csr *a, int K, bool acc) {
+    return (double)a->nnz * 20.0 + (double)(a->m + 1) * 8.0 +
+           K * ((double)a->n * 16.0 + (double)a->m * 16.0 * (acc ? 2.0 : 1.0));
 }
 
 // Algorithmic HBM bytes of one product with K right-hand sides: every stored
@@ -646,7 +660,8 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
             return set_error(EDX_ERR_ARG, "SpMV schedule %d needs the slot-factored form", G);
         int nb = 0;
         hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV_FAC,
-                                   fac_bytes(a, e_in.z != nullptr, e_in.dvec != nullptr));
+                                   fac_bytes(a, e_in.z != nullptr, e_in.dvec != nullptr), 0.0, 1,
+                                   csr_alg_bytes(a, 1, e_in.z != nullptr));
         EDX_TRY(fac_launch(a, (const double2 *)x, e_in, &nb));
         prof_stop(pe);
         if (nblocks_out) *nblocks_out = nb;
@@ -673,7 +688,8 @@ int spmv_launch(const edx_csr *a, int precision, const void *x, const SpmvEpi &e
     const bool packed = precision == EDX_C128I && pipe && a->d_off_pk;
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMV,
                                spmv_bytes(a, precision, 1, e.z != nullptr, e.dvec != nullptr) -
-                                   (packed ? 2.0 * (double)a->off_nnz : 0.0));
+                                   (packed ? 2.0 * (double)a->off_nnz : 0.0),
+                               0.0, 1, csr_alg_bytes(a, 1, e.z != nullptr));
     if (pipe) {
 #define EDX_PIPE_CASE(CODE, GG, UU)                                                               \
     case CODE:                                                                                    \
@@ -786,7 +802,8 @@ int spmm_launch(const edx_csr *a, int precision, int nrhs, const void *X, double
     if (blocks > cap) blocks = cap;
     dim3 grid((unsigned)blocks, (unsigned)((nrhs + NR - 1) / NR));
     hipEvent_t pe = prof_start(precision, EDX_KERNEL_SPMM_CM,
-                               spmv_bytes(a, precision, nrhs, Z != nullptr, false), 0.0, nrhs);
+                               spmv_bytes(a, precision, nrhs, Z != nullptr, false), 0.0, nrhs,
+                               csr_alg_bytes(a, nrhs, Z != nullptr));
 #define EDX_SPMM_CASE(GG)                                                                       \
     case GG:                                                                                    \
         if (precision == EDX_C128)                                                              \
@@ -846,6 +863,15 @@ int edx_profile_read_rhs(int precision, int kernel, int nrhs, int64_t *launches,
     if (launches) *launches = s.n;
     if (total_ms) *total_ms = s.ms;
     if (total_bytes) *total_bytes = s.bytes;
+    return EDX_OK;
+}
+
+int edx_profile_read_alg(int precision, int kernel, int nrhs, double *alg_bytes) {
+    EDX_CHECK(alg_bytes, EDX_ERR_ARG, "null argument");
+    EDX_HIP(hipStreamSynchronize(edx::stream()));
+    edx::ProfSum s;
+    EDX_TRY(edx::prof_total(precision, kernel, nrhs, &s));
+    *alg_bytes = s.abytes;
     return EDX_OK;
 }
 

@@ -192,7 +192,11 @@ int edx_csr_factored_blocks(const edx_csr *a, int64_t *blocks, int *types);
  * milliseconds and summed bytes of the launches in one precision
  * (edx_precision) and one kernel (edx_kernel, or -1 for all);
  * edx_profile_read_rhs the same for the launches with nrhs right-hand sides
- * (-1: all; any count above 16 reads every launch of more than 16). */
+ * (-1: all; any count above 16 reads every launch of more than 16);
+ * edx_profile_read_alg the summed algorithmic bytes of those launches counted
+ * as the complex128 CSR of the same operator, whatever form they read:
+ *   nnz*(16 + 4) + (m+1)*8 + K*(n*16 + m*16*(1 + acc)),
+ * acc = 1 where the epilogue accumulates into y. */
 enum edx_kernel {
     EDX_KERNEL_SPMV = 0,      /* k_spmv: one vector */
     EDX_KERNEL_SPMM_IL = 1,   /* k_spmm_il: K < 8 interleaved right-hand sides */
@@ -209,6 +213,7 @@ int edx_profile_read(int precision, int kernel, int64_t *launches, double *total
                      double *total_bytes);
 int edx_profile_read_rhs(int precision, int kernel, int nrhs, int64_t *launches,
                          double *total_ms, double *total_bytes);
+int edx_profile_read_alg(int precision, int kernel, int nrhs, double *alg_bytes);
 /* Summed algorithmic floating-point operations of the same launches (nonzero
  * for the matrix-core kernels only: 512 real flops per row and basis block). */
 int edx_profile_read_flops(int precision, int kernel, double *total_flops);
