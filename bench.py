@@ -332,8 +332,9 @@ def timed(work, precision, steps, warmup, reserve=lambda t: 0.0):
             {k: round(v, 3) for k, v in out["info"]["phases_s"].items()}))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    prof = {k: c.profile_read(precision, k) for k in c.KERNELS}
-    prof["all"] = c.profile_read(precision)
+    prof = {k: c.profile_read(precision, k) + (c.profile_read_alg(precision, k),)
+            for k in c.KERNELS}
+    prof["all"] = c.profile_read(precision) + (c.profile_read_alg(precision),)
     # the same per count of right-hand sides (one template instance each)
     prof["by_rhs"] = {}
     for k in c.KERNELS:
@@ -920,10 +921,11 @@ def main():
         for k, v in profs[f].items():
             if k == "by_rhs":
                 continue
-            pn, pms, pby = v
+            pn, pms, pby, paby = v
             if pn:
                 kern[k] = {"launches": pn, "total_s": pms * 1e-3, "avg_us": pms / pn * 1e3,
-                           "alg_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
+                           "alg_gbs": paby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0,
+                           "stored_gbs": pby / (pms * 1e-3) / 1e9 if pms > 0 else 0.0}
         inst = {}
         for (k, nr), (pn, pms, pby, paby) in sorted(profs[f]["by_rhs"].items()):
             inst[traffic_key(k, nr)] = {"launches": pn, "total_s": pms * 1e-3,
