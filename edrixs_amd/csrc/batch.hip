@@ -1544,15 +1544,26 @@ int kb_keep(edx_krylov_batch *q, int nkeep, const int *keep) {
         q->K = K;
         return rc;
     }
-    const BKrylov &ns = q->st;
-    if (done > 0)
-        EDX_HIP(edx::copy_sync(ns.alpha, na.data(), na.size() * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(edx::copy_sync(ns.beta, nb.data(), nb.size() * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(edx::copy_sync(ns.norm, nt.data(), nt.size() * sizeof(double), hipMemcpyHostToDevice));
-    EDX_HIP(edx::copy_sync(ns.run, nr.data(), nr.size() * sizeof(int), hipMemcpyHostToDevice));
-    EDX_HIP(edx::copy_sync(ns.neff, nn.data(), nkeep * sizeof(int), hipMemcpyHostToDevice));
+    // the vector blocks go in first: a failed upload below leaves a batch whose
+    // vectors and scalars both have nkeep columns, marked as not advancing
     std::swap(q->vb.p, vb.p);        // the old blocks are freed with the locals
     std::swap(q->x32.p, x32.p);
+    const BKrylov &ns = q->st;
+    const bool ok =
+        (done == 0 || edx::copy_sync(ns.alpha, na.data(), na.size() * sizeof(double),
+                                     hipMemcpyHostToDevice) == hipSuccess) &&
+        edx::copy_sync(ns.beta, nb.data(), nb.size() * sizeof(double), hipMemcpyHostToDevice) ==
+            hipSuccess &&
+        edx::copy_sync(ns.norm, nt.data(), nt.size() * sizeof(double), hipMemcpyHostToDevice) ==
+            hipSuccess &&
+        edx::copy_sync(ns.run, nr.data(), nr.size() * sizeof(int), hipMemcpyHostToDevice) ==
+            hipSuccess &&
+        edx::copy_sync(ns.neff, nn.data(), nkeep * sizeof(int), hipMemcpyHostToDevice) ==
+            hipSuccess;
+    if (!ok) {
+        q->live = false;
+        return edx::set_error(EDX_ERR_HIP, "batch compaction: upload of the kept histories failed");
+    }
     return EDX_OK;
 }
 
