@@ -5,6 +5,9 @@
 //                               (src/rixs_driver.f90:127-150: build omega - H_n,
 //                               pminres_csr) and the emission operator applied to
 //                               its solution (src/rixs_driver.f90:163-185)
+//   edx_shifted_fom_project_batch  the same for several right-hand sides (the
+//                               driver's incident polarizations) in one
+//                               interleaved recurrence over the multi-RHS SpMM
 //
 // The Krylov space K_m(H, b) is the same for every shift sigma, so one Lanczos
 // recurrence on H (the build_krylov_mp recurrence, src/lanczos.f90:210-303)
