@@ -104,3 +104,105 @@ def test_serving_admission_by_measured_bytes():
     # the oldest past max_wait_s goes first, even if larger
     assert eng.pick(reqs, running=0, now=5.5, in_use=0.0) is reqs[0]
     assert eng.pick([], running=0, now=0.0, in_use=0.0) is None
+
+
+def test_shifted_fom_rhs_block_rule(monkeypatch):
+    # which multi-shift form shifted_fom_rhs picks, without a device: the
+    # batched recurrence for 2-4 rows over a matrix with the slot-factored form
+    # in the indexed precision, for 8 or more rows, or for a block within
+    # SPLIT_BLOCK_BYTES; one recurrence per row otherwise
+    import torch
+    from edrixs_amd import core
+
+    class FakeH:
+        def __init__(self, n, fac):
+            self.n, self._fac = n, fac
+
+        def factored(self):
+            return (6, 1, 0, 1, 1) if self._fac else None
+
+    calls = []
+
+    def batch(H, B, sigma, projections, **kw):
+        calls.append(("batch", B.shape[0]))
+        R, K = B.shape[0], len(sigma)
+        return [None] * R, np.zeros((R, K), np.int32), np.zeros((R, K))
+
+    def single(H, b, sigma, projections=(None,), **kw):
+        calls.append(("single", 1))
+        return [None], np.zeros(len(sigma), np.int32), np.zeros(len(sigma))
+
+    monkeypatch.setattr(core, "shifted_fom_batch", batch)
+    monkeypatch.setattr(core, "shifted_fom", single)
+    big = 15_000_000
+    cases = [(FakeH(big, True), 2, "c128i", ["batch"]),
+             (FakeH(big, True), 2, "c128", ["single"] * 2),      # no factored product in c128
+             (FakeH(big, False), 2, "c128i", ["single"] * 2),
+             (FakeH(big, True), 5, "c128i", ["single"] * 5),     # the factored block stops at 4
+             (FakeH(big, False), 8, "c64d", ["batch"]),
+             (FakeH(1000, False), 3, "c128", ["batch"]),          # small block: one recurrence
+             (FakeH(big, True), 1, "c128i", ["single"])]
+    for H, R, prec, want in cases:
+        calls.clear()
+        B = torch.zeros((R, 4), dtype=torch.complex128)
+        outs, it, res = core.shifted_fom_rhs(H, B, [1 + 1j, 2 + 1j], [None] * R, precision=prec)
+        assert [c for c, _ in calls] == want, (H.n, R, prec, calls)
+        assert it.shape == (R, 2) and res.shape == (R, 2) and len(outs) == R
+
+
+def test_converged_spectra_compacts_the_block(monkeypatch):
+    # converged_spectra without a device: a fake resumable batch whose vector j
+    # reaches a final spectrum at step stop[j]; the block keeps only the
+    # vectors still running after each chunk, in their order, and every
+    # spectrum is taken at its own stopping chunk
+    import torch
+    from edrixs_amd import core
+    stop = [40, 120, 40, 80, 120]
+    keeps = []
+
+    class FakeBatch:
+        def __init__(self, H, V0, precision="c128", capacity=256):
+            self.ids = [int(v) for v in V0[:, 0].real]
+            self.K = len(self.ids)
+            self.done = 0
+
+        def advance(self, nsteps):
+            self.done += nsteps
+            return self.done
+
+        def coefficients(self, count=None):
+            n = self.done if count is None else count
+            al = [np.full(n, float(i)) for i in self.ids]
+            return al, [np.ones(n) for _ in self.ids], np.ones(self.K)
+
+        def keep(self, idx):
+            keeps.append(list(idx))
+            self.ids = [self.ids[j] for j in idx]
+            self.K = len(self.ids)
+
+        def close(self):
+            pass
+
+    def fake_cf(al, be, nr, e_gs, grid, gamma):
+        j, n = int(al[0]), len(al)
+        return np.array([1.0 if n >= stop[j] else 1.0 + 1.0 / n])
+
+    class FakeH:
+        n = 100
+
+        def factored(self):
+            return None
+
+    monkeypatch.setattr(core, "KrylovBatch", FakeBatch)
+    monkeypatch.setattr(core, "cf_spectrum", fake_cf)
+    V0 = torch.zeros((5, 100), dtype=torch.complex128)
+    V0[:, 0] = torch.arange(5, dtype=torch.float64)
+    spec, steps, change = core.converged_spectra(FakeH(), V0, 0.0, None, 0.1, chunk=20,
+                                                 min_steps=20, max_steps=400)
+    # a spectrum is final once unchanged over a chunk: vector j stops one
+    # chunk after stop[j]
+    assert steps == [60, 140, 60, 100, 140]
+    # after 60 steps vectors 0 and 2 are final: keep 1, 3, 4; after 100 the
+    # second of those (vector 3): keep positions 0 and 2 (vectors 1 and 4)
+    assert keeps == [[1, 3, 4], [0, 2]]
+    assert all(c <= 1e-8 for c in change)
